@@ -1,0 +1,142 @@
+/*
+ * sfb.h -- C ABI of libsfb_mi355x.so, the MI355X-native catalogue -> SFB
+ * coefficient transform (cat2amln) and the host setup it needs.
+ *
+ * Conventions
+ *  - Every pointer argument is a plain host pointer unless the function name
+ *    ends in _device; those take device pointers and an optional hipStream_t
+ *    passed as void* (NULL = the plan's own stream).
+ *  - Arrays follow the reference's Julia memory layout (column-major):
+ *      rtp       3 x N, (r, theta, phi) per galaxy, i.e. r0 th0 ph0 r1 ...
+ *      win dense npix x (lmax+1) x nmax   (win_rhat_ln, src/windows.jl:244-257)
+ *      win_b     (lmax+1) x nmax          (separable w_ln, src/windows.jl:260-270)
+ *      knl       nmax x (lmax+1), NaN where there is no mode
+ *      anlm      getnlmsize complex numbers, interleaved (re, im), ordered
+ *                n slowest, then l, then m = 0..l (src/modes.jl:80-82,223-232)
+ *  - Basis-function spline tables are concatenated per (n,l) pair in the same
+ *    n-major, l-minor order ("map order"); knot_off has nmaps+1 entries.
+ *  - Pixel indices are 0-based HEALPix ring-scheme indices.
+ *  - Every int-returning function returns SFB_OK or an SFB_ERR_* code; the text
+ *    of the last error on the calling thread is returned by sfb_last_error().
+ *  - A plan may be used by one host thread at a time; different plans may run
+ *    concurrently.  Calls are synchronous at return unless named _device.
+ */
+#ifndef SFB_MI355X_H
+#define SFB_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFB_OK 0
+#define SFB_ERR_ARG 1        /* bad argument (null pointer, size, kind) */
+#define SFB_ERR_LMAX 2       /* lmax > 4 nside: src/healpix_helpers.jl:51-53 error("exiting") */
+#define SFB_ERR_NONFINITE 3  /* non-finite a_nlm: src/cat2anlm.jl:308 @assert */
+#define SFB_ERR_SETUP 4      /* root finding or normalisation failed: SphericalBesselGNLs.jl:126-127,:416 */
+#define SFB_ERR_DEVICE 5     /* HIP runtime error (includes device out-of-memory) */
+#define SFB_ERR_NOMEM 6      /* host allocation failed */
+
+#define SFB_PREC_FP64 0      /* plain path: reference method, FP64 throughout */
+#define SFB_PREC_MIXED 1     /* FP64 delta maps and phi-transform, Legendre steps on FP32 MFMA
+                                with split operands for the first analysis pass */
+
+#define SFB_WIN_DENSE 0      /* win_a = W[npix, lmax+1, nmax] */
+#define SFB_WIN_SEPARABLE 1  /* win_a = mask[npix], win_b = w_ln[lmax+1, nmax] */
+
+typedef struct sfb_modes sfb_modes;
+typedef struct sfb_plan sfb_plan;
+
+/* Text of the last error raised on this thread ("" if none). */
+const char* sfb_last_error(void);
+
+/* Library version string. */
+const char* sfb_version(void);
+
+/* ---- host setup: AnlmModes ---------------------------------------------- */
+
+/* AnlmModes(kmax, rmin, rmax; nside, boundary=potential): src/modes.jl:122-158,
+ * roots src/SphericalBesselGNLs.jl:309-336, c/d :404-421, splines :558-579.
+ * nside <= 0 selects estimate_nside(lmax) (src/modes.jl:68). */
+int sfb_modes_create_kmax(sfb_modes** out, double kmax, double rmin, double rmax, int64_t nside);
+
+/* AnlmModes(nmax, lmax, rmin, rmax; nside): src/modes.jl:161-175,
+ * roots src/SphericalBesselGNLs.jl:294-306. */
+int sfb_modes_create_nl(sfb_modes** out, int64_t nmax, int64_t lmax, double rmin, double rmax,
+                        int64_t nside);
+
+/* Sizes: nmax, lmax, nside, number of (n,l) maps, getnlmsize (src/modes.jl:183-189),
+ * total spline knots. */
+int sfb_modes_sizes(const sfb_modes* m, int64_t* nmax, int64_t* lmax, int64_t* nside,
+                    int64_t* nmaps, int64_t* nlmsize, int64_t* nknots);
+
+/* Copies out lmax_n[nmax], nmax_l[lmax+1], knl[nmax*(lmax+1)], cnl, dnl (same shape),
+ * knot_off[nmaps+1], knot_x/y/ypp[nknots].  Any output pointer may be NULL. */
+int sfb_modes_tables(const sfb_modes* m, int64_t* lmax_n, int64_t* nmax_l, double* knl, double* cnl,
+                     double* dnl, int64_t* knot_off, double* knot_x, double* knot_y, double* knot_ypp);
+
+void sfb_modes_destroy(sfb_modes* m);
+
+/* Spline1D{3} build (src/Splines.jl:206-271): fills ypp[n] from knots x[n], y[n]. */
+int sfb_spline_build(int64_t n, const double* x, const double* y, double* ypp);
+
+/* Spline1D{3} evaluation (src/Splines.jl:61-69) and derivative (:367-377) at r[nr]. */
+int sfb_spline_eval(int64_t n, const double* x, const double* y, const double* ypp, int64_t nr,
+                    const double* r, double* out, int32_t derivative);
+
+/* ---- the transform -------------------------------------------------------- */
+
+/* Plan for cat2amln on one device.  The spline tables are taken as given
+ * (from the reference's amodes.basisfunctions.gnl[n,l+1].x/.y/.ypp, or from
+ * sfb_modes_tables), so the knots are bit-identical to the caller's.
+ * niter is the map2alm Jacobi iteration count (src/healpix_helpers.jl:50: 3).
+ * Fails with SFB_ERR_LMAX when lmax > 4 nside. */
+int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int64_t nmax, const int64_t* lmax_n,
+                    const int64_t* knot_off, const double* knot_x, const double* knot_y,
+                    const double* knot_ypp, int32_t niter, int32_t precision, int32_t device);
+
+void sfb_plan_destroy(sfb_plan* p);
+
+/* Sizes the plan works with: number of maps, getnlmsize, npix. */
+int sfb_plan_sizes(const sfb_plan* p, int64_t* nmaps, int64_t* nlmsize, int64_t* npix);
+
+/* Switches the Legendre-step precision of an existing plan (SFB_PREC_*). */
+int sfb_plan_set_precision(sfb_plan* p, int32_t precision);
+
+/* cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight): src/cat2anlm.jl:257-310.
+ * w == NULL means unit weights (the reference default ones(N)).  N may be 0
+ * (field2anlm_v1, src/cat2anlm.jl:315-321).  anlm_out holds 2*nlmsize doubles. */
+int sfb_cat2amln(sfb_plan* p, int64_t N, const double* rtp, const double* w, double nbar,
+                 int32_t win_kind, const double* win_a, const double* win_b, double* anlm_out);
+
+/* Same transform with every array already in device memory; enqueued on
+ * stream (NULL = the plan's stream) and synchronised before returning, so the
+ * non-finite check can be reported. */
+int sfb_cat2amln_device(sfb_plan* p, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                        int32_t win_kind, const double* d_win_a, const double* d_win_b,
+                        double* d_anlm_out, void* stream);
+
+/* ---- pieces of the path, exposed for parity checks ------------------------ */
+
+/* ang2pixRing (Healpix.jl, called at src/cat2anlm.jl:263) on the GPU, 0-based. */
+int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const double* theta, const double* phi,
+                     int32_t* pix_out);
+
+/* The FP64 delta map of map q (the (n,l) pair in map order) exactly as the
+ * reference forms it before map2alm (src/cat2anlm.jl:284-291), length npix. */
+int sfb_delta_map(sfb_plan* p, int64_t N, const double* rtp, const double* w, double nbar,
+                  int32_t win_kind, const double* win_a, const double* win_b, int64_t q,
+                  double* map_out);
+
+/* map2alm with niter refinements (Healpix.jl map2alm!, src/healpix_helpers.jl:50-56)
+ * of nmap FP64 maps [nmap][npix] on the GPU with the plan's nside/lmax and
+ * precision; alm_out is [nmap][nalm] interleaved complex in the Healpix m-major
+ * index m(2 lmax+1-m)/2 + l.  nmap must not exceed the plan's map count. */
+int sfb_map2alm(sfb_plan* p, int64_t nmap, const double* maps, double* alm_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFB_MI355X_H */

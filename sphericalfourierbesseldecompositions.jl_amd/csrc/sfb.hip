@@ -1,0 +1,1017 @@
+// libsfb_mi355x: MI355X (gfx950) implementation of SuperFaB's cat2amln
+// (src/cat2anlm.jl:257-310 in hsgg/SphericalFourierBesselDecompositions.jl 0.5.19).
+//
+// Pipeline of one call (all on one HIP stream):
+//   k_prep        ang2pix (ring scheme) + sortable r key per galaxy
+//   rocprim       stable radix sort by r, then stable radix sort by pixel
+//                 -> galaxies grouped by pixel, r-ordered inside a pixel, which is
+//                 the reference's summation order (sortout, cat2anlm.jl:50-71)
+//   k_gather      r, w in that order; k_pixel_start: first galaxy of every pixel
+//   k_scatter     delta[q][p] = (sum_{i in p} w_i g_q(r_i)) * 1/(nbar dOmega) - W_q(p)
+//                 with the reference's spline formula and rounding (cat2anlm.jl:284-291)
+//   k_phi_dft     per ring pair (north k, south 4 nside-2-k): F_m = sum_j delta_j e^{-i m phi_j},
+//                 stored as E = F_N + F_S and O = F_N - F_S for m = 0..lmax
+//   Legendre      a_{l m} = dOmega sum_k lambda_lm(theta_k) (E or O by parity of l+m):
+//                 one batched FP64 MFMA GEMM per (m, parity) -- rows = (map, re/im),
+//                 K = ring pairs, columns = l
+//   niter times   synthesis G = a Lambda^T (same GEMM), k_residual R = F - DFT(alm2map(a))
+//                 computed in ring-Fourier space with the exact aliasing of short rings,
+//                 analysis a += dOmega R Lambda  (Healpix map2alm! niter loop,
+//                 src/healpix_helpers.jl:50-56)
+//   k_pack        anlm[getidx(n,l,m)] = a^{(n,l)}_{l m}, m = 0..l, plus a non-finite flag
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "sfb_internal.h"
+
+using sfb::fail;
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(SFB_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+static const double kPi = 3.141592653589793;
+
+// ---------------------------------------------------------------------------
+// device-side tables
+// ---------------------------------------------------------------------------
+struct MapInfo {
+    int64_t koff;   // first knot of this map
+    int64_t soff;   // first segment of this map (= koff - q)
+    int32_t nk;     // knots
+    int32_t n, l;   // radial index (1-based) and degree
+    int32_t pad;
+    double xfirst, xlast, invdx, yfirst, ylast;
+};
+
+struct RingDev {
+    int32_t npr, shift;
+    int64_t start_n, start_s;
+    int64_t tw_off;  // offset of this ring length's twiddle table
+};
+
+struct GemmDesc {
+    const double* A;  // A[k * lda + i]
+    const double* B;  // B[k * ldb + n]
+    double* C;        // C[n * ldc + i]
+    int32_t K, N;
+    int64_t lda, ldb, ldc;
+};
+
+// ---------------------------------------------------------------------------
+// ang2pix, ring scheme (Gorski et al. 2005), same operation order as the
+// host restatement so the indices are bit-identical for identical cos(theta).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t ang2pix_ring_dev(int64_t nside, double theta, double phi)
+{
+    const double twopi = 6.283185307179586;
+    const double halfpi = 1.5707963267948966;
+    const int64_t npix = 12 * nside * nside;
+    const int64_t ncap = 2 * nside * (nside - 1);
+    double z = cos(theta);
+    double za = fabs(z);
+    double ph = phi;
+    if (!(ph >= 0.0 && ph < twopi)) {
+        ph = fmod(ph, twopi);
+        if (ph < 0.0) ph = __dadd_rn(ph, twopi);
+    }
+    double tt = __ddiv_rn(ph, halfpi);
+    if (za <= 2.0 / 3.0) {
+        int64_t nl4 = 4 * nside;
+        double t1 = __dmul_rn((double)nside, __dadd_rn(0.5, tt));
+        double t2 = __dmul_rn(__dmul_rn((double)nside, z), 0.75);
+        int64_t jp = (int64_t)floor(__dsub_rn(t1, t2));
+        int64_t jm = (int64_t)floor(__dadd_rn(t1, t2));
+        int64_t ir = nside + 1 + jp - jm;
+        int64_t kshift = 1 - (ir & 1);
+        int64_t ip = (jp + jm - nside + kshift + 1) / 2;
+        ip = ((ip % nl4) + nl4) % nl4;
+        return ncap + (ir - 1) * nl4 + ip;
+    } else {
+        double tp = __dsub_rn(tt, floor(tt));
+        double tmp = __dmul_rn((double)nside, __dsqrt_rn(__dmul_rn(3.0, __dsub_rn(1.0, za))));
+        int64_t jp = (int64_t)floor(__dmul_rn(tp, tmp));
+        int64_t jm = (int64_t)floor(__dmul_rn(__dsub_rn(1.0, tp), tmp));
+        int64_t ir = jp + jm + 1;
+        int64_t ip = (int64_t)floor(__dmul_rn(tt, (double)ir));
+        ip = ((ip % (4 * ir)) + 4 * ir) % (4 * ir);
+        if (z > 0) return 2 * ir * (ir - 1) + ip;
+        return npix - 2 * ir * (ir + 1) + ip;
+    }
+}
+
+// Order-preserving unsigned key of a double (negative values flipped).
+__device__ __forceinline__ uint64_t sortable_bits(double v)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void k_prep(int64_t N, const double* __restrict__ rtp, int64_t nside, uint64_t* __restrict__ key_r,
+                       uint32_t* __restrict__ idx, uint32_t* __restrict__ pix)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double r = rtp[3 * i], th = rtp[3 * i + 1], ph = rtp[3 * i + 2];
+    key_r[i] = sortable_bits(r);
+    idx[i] = (uint32_t)i;
+    pix[i] = (uint32_t)ang2pix_ring_dev(nside, th, ph);
+}
+
+__global__ void k_ang2pix(int64_t N, const double* __restrict__ th, const double* __restrict__ ph, int64_t nside,
+                          int32_t* __restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    out[i] = (int32_t)ang2pix_ring_dev(nside, th[i], ph[i]);
+}
+
+__global__ void k_gather_pix(int64_t N, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ pix,
+                             uint32_t* __restrict__ pixkey)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    pixkey[k] = pix[idx[k]];
+}
+
+__global__ void k_gather(int64_t N, const uint32_t* __restrict__ idx, const double* __restrict__ rtp,
+                         const double* __restrict__ w, double* __restrict__ r_s, double* __restrict__ w_s)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    uint32_t i = idx[k];
+    r_s[k] = rtp[3 * (int64_t)i];
+    w_s[k] = w ? w[i] : 1.0;
+}
+
+// pstart[p] = first sorted galaxy with pixel >= p, p = 0..npix.
+__global__ void k_pixel_start(int64_t npix, int64_t N, const uint32_t* __restrict__ pix_sorted,
+                              int32_t* __restrict__ pstart)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > npix) return;
+    int64_t lo = 0, hi = N;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)pix_sorted[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    pstart[p] = (int32_t)lo;
+}
+
+// ---------------------------------------------------------------------------
+// g_nl(r): Spline1D{3} with 'boundary' extrapolation (Splines.jl:61-78,327-357).
+// seg = (x0, h, a, b, c, d) per segment, computed on the host with the
+// reference's expressions; value = d + t(c + t(b + t a)) with fma (muladd).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double spline_dev(const MapInfo& mi, const double* __restrict__ kx,
+                                             const double* __restrict__ seg, double r)
+{
+    if (r < mi.xfirst) return mi.yfirst;
+    if (r > mi.xlast) return mi.ylast;
+    if (!(r == r)) return r;
+    const double* x = kx + mi.koff;
+    int i = (int)(__dmul_rn(__dsub_rn(r, mi.xfirst), mi.invdx));
+    i = min(max(i, 0), mi.nk - 2);
+    while (i > 0 && r < x[i]) --i;
+    while (i < mi.nk - 2 && r >= x[i + 1]) ++i;
+    const double* s = seg + 6 * (mi.soff + i);
+    double t = __ddiv_rn(__dsub_rn(r, s[0]), s[1]);
+    return fma(t, fma(t, fma(t, s[2], s[3]), s[4]), s[5]);
+}
+
+// delta[q][p] for every map q (blockIdx.y) and pixel p.
+template <bool DENSE>
+__global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __restrict__ pstart,
+                                                 const double* __restrict__ r_s, const double* __restrict__ w_s,
+                                                 const MapInfo* __restrict__ maps, const double* __restrict__ kx,
+                                                 const double* __restrict__ seg, double inv,
+                                                 const double* __restrict__ win_a, const double* __restrict__ win_b,
+                                                 int64_t lmax, int32_t q0, double* __restrict__ delta)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t q = q0 + (int32_t)blockIdx.y;
+    if (p >= npix) return;
+    const MapInfo mi = maps[q];
+    double s = 0.0;
+    int32_t g1 = pstart[p + 1];
+    for (int32_t g = pstart[p]; g < g1; ++g) {
+        double val = spline_dev(mi, kx, seg, r_s[g]);
+        s = __dadd_rn(s, __dmul_rn(w_s[g], val));
+    }
+    double v = __dmul_rn(s, inv);
+    int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
+    double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], win_b[col]);
+    delta[(int64_t)q * npix + p] = __dsub_rn(v, W);
+}
+
+// ---------------------------------------------------------------------------
+// phi-transform of one ring pair for TQ maps.  X[(m*2+par)*K + k][i], i = 2q+c.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_phi_dft(const double* __restrict__ delta, int64_t npix, int32_t nq,
+                                                 int64_t M2, int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                 const double2* __restrict__ tw, double* __restrict__ X, int32_t TQ)
+{
+    extern __shared__ double sm[];
+    const int k = blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev R = rings[k];
+    const int npr = R.npr;
+    const int ld = npr + 1;
+    const bool eq = (k == K - 1);
+    double* dN = sm;
+    double* dS = sm + TQ * ld;
+    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
+        int ql = e / npr, j = e - ql * npr;
+        int q = q0 + ql;
+        double vn = 0.0, vs = 0.0;
+        if (q < nq) {
+            vn = delta[(int64_t)q * npix + R.start_n + j];
+            if (!eq) vs = delta[(int64_t)q * npix + R.start_s + j];
+        }
+        dN[ql * ld + j] = vn;
+        dS[ql * ld + j] = vs;
+    }
+    __syncthreads();
+    const double2* twr = tw + R.tw_off;
+    const int n2 = 2 * npr;
+    for (int e = threadIdx.x; e < (L + 1) * TQ; e += blockDim.x) {
+        int m = e / TQ, ql = e - m * TQ;
+        int q = q0 + ql;
+        if (q >= nq) continue;
+        int kk = (int)(((int64_t)m * R.shift) % n2);
+        int dk = (int)(((int64_t)2 * m) % n2);
+        double fnr = 0, fni = 0, fsr = 0, fsi = 0;
+        const double* an = dN + ql * ld;
+        const double* as = dS + ql * ld;
+        for (int j = 0; j < npr; ++j) {
+            double2 c = twr[kk];
+            double vn = an[j], vs = as[j];
+            fnr = fma(vn, c.x, fnr);
+            fni = fma(-vn, c.y, fni);
+            fsr = fma(vs, c.x, fsr);
+            fsi = fma(-vs, c.y, fsi);
+            kk += dk;
+            if (kk >= n2) kk -= n2;
+        }
+        int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+        int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+        if (eq) {
+            X[be] = fnr;
+            X[be + 1] = fni;
+            X[bo] = fnr;
+            X[bo + 1] = fni;
+        } else {
+            X[be] = fnr + fsr;
+            X[be + 1] = fni + fsi;
+            X[bo] = fnr - fsr;
+            X[bo + 1] = fni - fsi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta C[n][i]
+// for rows i < rows.  Block tile 64(i) x 64(n), k-tile 16, 4 waves of 32 x 32.
+// v_mfma_f64_16x16x4f64 operand layout: lane l holds A[row l%16][k l/16],
+// B[k l/16][col l%16]; result register v of lane l is D[4(l/16)+v][l%16].
+// The MFMA "rows" are the output columns n so the stores run along i.
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+                                                  double beta)
+{
+    const GemmDesc d = descs[blockIdx.z];
+    const int i0 = blockIdx.x * GBM;
+    const int n0 = blockIdx.y * GBN;
+    if (n0 >= d.N) return;
+    __shared__ double As[GBK][GBM + 1];
+    __shared__ double Bs[GBK][GBN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32;
+    const int wn = (wave >> 1) * 32;
+    double4_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < d.K; k0 += GBK) {
+        for (int e = tid; e < GBK * GBM; e += 256) {
+            int kk = e / GBM, ii = e - kk * GBM;
+            int gk = k0 + kk, gi = i0 + ii;
+            As[kk][ii] = (gk < d.K && gi < rows) ? d.A[(int64_t)gk * d.lda + gi] : 0.0;
+        }
+        for (int e = tid; e < GBK * GBN; e += 256) {
+            int kk = e / GBN, nn = e - kk * GBN;
+            int gk = k0 + kk, gn = n0 + nn;
+            Bs[kk][nn] = (gk < d.K && gn < d.N) ? d.B[(int64_t)gk * d.ldb + gn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GBK; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            double fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                fa[t] = Bs[kk][wn + t * 16 + (lane & 15)];
+                fb[t] = As[kk][wi + t * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+                    acc[tn][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[tn], fb[ti], acc[tn][ti], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int n = n0 + wn + tn * 16 + 4 * (lane >> 4) + v;
+                int i = i0 + wi + ti * 16 + (lane & 15);
+                if (n < d.N && i < rows) {
+                    double* c = d.C + (int64_t)n * d.ldc + i;
+                    double val = alpha * acc[tn][ti][v];
+                    if (beta != 0.0) val = val + beta * *c;
+                    *c = val;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Residual in ring-Fourier space.  For a ring with npr pixels and phase
+// phi_0 = shift*pi/npr, the DFT of the synthesised ring is
+//   FS_m = npr * sum_{m' = m mod npr, |m'| <= lmax} G_{m'} e^{i (m'-m) phi_0},
+// G_{-m} = conj(G_m), and e^{i t npr phi_0} = (-1)^{t shift}.
+// G holds the parity halves Ge, Go per ring pair: north = Ge + Go, south = Ge - Go.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ X0, const double* __restrict__ G,
+                                                  double* __restrict__ R, const RingDev* __restrict__ rings,
+                                                  int32_t K, int32_t L, int64_t M2, int32_t nq)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const int m = blockIdx.z;
+    if (q >= nq) return;
+    const RingDev Rg = rings[k];
+    const int npr = Rg.npr;
+    const bool eq = (k == K - 1);
+    double snr = 0, sni = 0, ssr = 0, ssi = 0;
+    const int mstart = m - ((m + L) / npr) * npr;
+    for (int mp = mstart; mp <= L; mp += npr) {
+        const int t = (mp - m) / npr;
+        const double sg = ((t * Rg.shift) & 1) ? -1.0 : 1.0;
+        const int ma = mp < 0 ? -mp : mp;
+        const double cj = mp < 0 ? -1.0 : 1.0;
+        const double* ge = G + ((int64_t)(ma * 2 + 0) * K + k) * M2 + 2 * q;
+        double er = ge[0], ei = ge[1], orr = 0.0, oi = 0.0;
+        if (ma < L) {  // the odd half of m' = lmax is empty
+            const double* go = G + ((int64_t)(ma * 2 + 1) * K + k) * M2 + 2 * q;
+            orr = go[0];
+            oi = go[1];
+        }
+        snr += sg * (er + orr);
+        sni += sg * cj * (ei + oi);
+        ssr += sg * (er - orr);
+        ssi += sg * cj * (ei - oi);
+    }
+    snr *= npr;
+    sni *= npr;
+    ssr *= npr;
+    ssi *= npr;
+    const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+    const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+    if (eq) {
+        double rr = X0[be] - snr, ri = X0[be + 1] - sni;
+        R[be] = rr;
+        R[be + 1] = ri;
+        R[bo] = rr;
+        R[bo + 1] = ri;
+    } else {
+        R[be] = X0[be] - (snr + ssr);
+        R[be + 1] = X0[be + 1] - (sni + ssi);
+        R[bo] = X0[bo] - (snr - ssr);
+        R[bo + 1] = X0[bo + 1] - (sni - ssi);
+    }
+}
+
+// anlm[base_q + m] = a^{(q)}_{l_q, m}, m = 0..l_q.
+__global__ void k_pack(const double* __restrict__ a, const int64_t* __restrict__ aoff, const MapInfo* __restrict__ maps,
+                       const int64_t* __restrict__ out_base, int32_t nq, int64_t M2, double* __restrict__ out,
+                       int32_t* __restrict__ flag)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (q >= nq) return;
+    const int l = maps[q].l;
+    if (m > l) return;
+    const int par = (l - m) & 1, j = (l - m) >> 1;
+    const double* src = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
+    double re = src[0], im = src[1];
+    out[2 * (out_base[q] + m)] = re;
+    out[2 * (out_base[q] + m) + 1] = im;
+    if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
+}
+
+// Healpix-layout alm of the first nq maps: alm[q][m(2L+1-m)/2 + l].
+__global__ void k_pack_alm(const double* __restrict__ a, const int64_t* __restrict__ aoff, int32_t nq, int32_t L,
+                           int64_t M2, double* __restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (q >= nq) return;
+    const int64_t nalm = (int64_t)(L + 1) * (L + 2) / 2;
+    for (int l = m; l <= L; ++l) {
+        const int par = (l - m) & 1, j = (l - m) >> 1;
+        const double* src = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
+        int64_t idx = (int64_t)m * (2 * L + 1 - m) / 2 + l;
+        out[2 * ((int64_t)q * nalm + idx)] = src[0];
+        out[2 * ((int64_t)q * nalm + idx) + 1] = src[1];
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need)
+    {
+        if (need <= bytes && p) return SFB_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t want = need ? need : 16;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(SFB_ERR_DEVICE, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        }
+        bytes = want;
+        return SFB_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+struct sfb_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t nside = 0, npix = 0, lmax = 0, nmax = 0, nmaps = 0, nlmsize = 0;
+    int32_t K = 0;       // ring pairs (northern rings incl. equator)
+    int64_t M2 = 0;      // 2 * nmaps
+    int32_t niter = 3, precision = 0, TQ = 8;
+    std::vector<MapInfo> maps;
+    std::vector<int64_t> out_base, aoff, lam_off;
+    std::vector<int32_t> nb;  // per (m, par): number of l
+    int64_t a_size = 0, x_size = 0, lam_size = 0;
+    // tables
+    DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
+    // work
+    DevBuf d_delta, d_X0, d_G, d_R, d_a;
+    // catalogue
+    DevBuf d_keyr, d_keyr2, d_idx, d_idx2, d_pix, d_pixkey, d_pixkey2, d_rs, d_ws, d_pstart, d_sorttmp;
+    // host-API staging
+    DevBuf h_rtp, h_w, h_wa, h_wb, h_out;
+    int64_t cat_cap = -1;
+};
+
+static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_off, const double* kx,
+                       const double* ky, const double* kypp)
+{
+    const int64_t L = P->lmax;
+    // maps
+    P->maps.resize((size_t)P->nmaps);
+    P->out_base.resize((size_t)P->nmaps);
+    int64_t q = 0, base = 0;
+    std::vector<double> seg;
+    seg.reserve((size_t)(knot_off[P->nmaps] - P->nmaps) * 6);
+    for (int64_t n = 1; n <= P->nmax; ++n) {
+        for (int64_t l = 0; l <= lmax_n[n - 1]; ++l, ++q) {
+            MapInfo& mi = P->maps[(size_t)q];
+            int64_t o = knot_off[q], nk = knot_off[q + 1] - knot_off[q];
+            if (nk < 4) return fail(SFB_ERR_ARG, "map %lld has %lld knots (need >= 4)", (long long)q, (long long)nk);
+            const double* x = kx + o;
+            const double* y = ky + o;
+            const double* ypp = kypp + o;
+            for (int64_t i = 1; i < nk; ++i)
+                if (!(x[i] > x[i - 1])) return fail(SFB_ERR_ARG, "knots of map %lld are not increasing", (long long)q);
+            mi.koff = o;
+            mi.soff = o - q;
+            mi.nk = (int32_t)nk;
+            mi.n = (int32_t)n;
+            mi.l = (int32_t)l;
+            mi.pad = 0;
+            mi.xfirst = x[0];
+            mi.xlast = x[nk - 1];
+            mi.invdx = (double)(nk - 1) / (x[nk - 1] - x[0]);
+            mi.yfirst = y[0];
+            mi.ylast = y[nk - 1];
+            for (int64_t i = 0; i + 1 < nk; ++i) {
+                // get_abcdh (Splines.jl:327-344)
+                double h = x[i + 1] - x[i];
+                double h2 = h * h;
+                double b = h2 * ypp[i] / 2.0;
+                double a = h2 * (ypp[i + 1] - ypp[i]) / 6.0;
+                double d = y[i];
+                double c = y[i + 1] - a - b - d;
+                seg.push_back(x[i]);
+                seg.push_back(h);
+                seg.push_back(a);
+                seg.push_back(b);
+                seg.push_back(c);
+                seg.push_back(d);
+            }
+            P->out_base[(size_t)q] = base + l * (l + 1) / 2;
+        }
+        base += (lmax_n[n - 1] + 1) * (lmax_n[n - 1] + 2) / 2;
+    }
+    P->nlmsize = base;
+    int rc;
+    if ((rc = P->d_maps.ensure(sizeof(MapInfo) * P->maps.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_maps.p, P->maps.data(), sizeof(MapInfo) * P->maps.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_kx.ensure(sizeof(double) * (size_t)knot_off[P->nmaps]))) return rc;
+    HIP_TRY(hipMemcpy(P->d_kx.p, kx, sizeof(double) * (size_t)knot_off[P->nmaps], hipMemcpyHostToDevice));
+    if ((rc = P->d_seg.ensure(sizeof(double) * seg.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_seg.p, seg.data(), sizeof(double) * seg.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_outbase.ensure(sizeof(int64_t) * (size_t)P->nmaps))) return rc;
+    HIP_TRY(hipMemcpy(P->d_outbase.p, P->out_base.data(), sizeof(int64_t) * (size_t)P->nmaps, hipMemcpyHostToDevice));
+
+    // rings and twiddles
+    std::vector<sfb::RingPair> rp = sfb::ring_pairs(P->nside);
+    std::vector<RingDev> rd((size_t)P->K);
+    std::vector<double2> tw((size_t)(4 * P->nside * (P->nside + 1)));
+    for (int64_t i = 1; i <= P->nside; ++i) {
+        int64_t npr = 4 * i, off = 4 * i * (i - 1);
+        for (int64_t k = 0; k < 2 * npr; ++k) {
+            long double ang = 3.141592653589793238462643383279502884L * (long double)k / (long double)npr;
+            tw[(size_t)(off + k)] = make_double2((double)cosl(ang), (double)sinl(ang));
+        }
+    }
+    for (int32_t k = 0; k < P->K; ++k) {
+        const sfb::RingPair& R = rp[(size_t)k];
+        RingDev& D = rd[(size_t)k];
+        D.npr = (int32_t)R.npr;
+        D.shift = R.shift;
+        D.start_n = R.start_n;
+        D.start_s = R.start_s;
+        int64_t i = R.npr / 4;
+        D.tw_off = 4 * i * (i - 1);
+    }
+    if ((rc = P->d_rings.ensure(sizeof(RingDev) * rd.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_rings.p, rd.data(), sizeof(RingDev) * rd.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_tw.ensure(sizeof(double2) * tw.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_tw.p, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+
+    // lambda tables per (m, parity): Lam[k][j] and LamT[j][k], l = m + 2j + par
+    P->nb.assign((size_t)(2 * (L + 1)), 0);
+    P->lam_off.assign((size_t)(2 * (L + 1) + 1), 0);
+    P->aoff.assign((size_t)(2 * (L + 1) + 1), 0);
+    for (int64_t m = 0; m <= L; ++m)
+        for (int par = 0; par < 2; ++par) {
+            int32_t n = (int32_t)((L - m - par) >= 0 ? (L - m - par) / 2 + 1 : 0);
+            size_t b = (size_t)(m * 2 + par);
+            P->nb[b] = n;
+            P->lam_off[b + 1] = P->lam_off[b] + (int64_t)n * P->K;
+            P->aoff[b + 1] = P->aoff[b] + (int64_t)n * P->M2;
+        }
+    P->lam_size = P->lam_off.back();
+    P->a_size = P->aoff.back();
+    std::vector<double> lam((size_t)P->lam_size), lamT((size_t)P->lam_size), col((size_t)(L + 1));
+    for (int32_t k = 0; k < P->K; ++k) {
+        const sfb::RingPair& R = rp[(size_t)k];
+        for (int64_t m = 0; m <= L; ++m) {
+            sfb::lambda_column(L, m, R.z, R.sth, col.data());
+            for (int64_t l = m; l <= L; ++l) {
+                int par = (int)((l - m) & 1);
+                int64_t j = (l - m) >> 1;
+                size_t b = (size_t)(m * 2 + par);
+                int64_t n = P->nb[b];
+                lam[(size_t)(P->lam_off[b] + (int64_t)k * n + j)] = col[(size_t)(l - m)];
+                lamT[(size_t)(P->lam_off[b] + j * P->K + k)] = col[(size_t)(l - m)];
+            }
+        }
+    }
+    if ((rc = P->d_lam.ensure(sizeof(double) * lam.size()))) return rc;
+    if ((rc = P->d_lamT.ensure(sizeof(double) * lamT.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_lam.p, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_lamT.p, lamT.data(), sizeof(double) * lamT.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_aoff.ensure(sizeof(int64_t) * P->aoff.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_aoff.p, P->aoff.data(), sizeof(int64_t) * P->aoff.size(), hipMemcpyHostToDevice));
+
+    // work buffers
+    P->x_size = (L + 1) * 2 * (int64_t)P->K * P->M2;
+    if ((rc = P->d_delta.ensure(sizeof(double) * (size_t)(P->nmaps * P->npix)))) return rc;
+    if ((rc = P->d_X0.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
+    if ((rc = P->d_G.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
+    if ((rc = P->d_R.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
+    if ((rc = P->d_a.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
+    if ((rc = P->d_flag.ensure(sizeof(int32_t)))) return rc;
+
+    // GEMM descriptors: analysis A = X (k = ring pair), B = Lam, C = a;
+    // synthesis A = a (k = l index), B = LamT, C = G.
+    std::vector<GemmDesc> ana, syn;
+    for (int64_t b = 0; b < 2 * (L + 1); ++b) {
+        int32_t n = P->nb[(size_t)b];
+        GemmDesc d;
+        d.A = P->d_X0.as<double>() + b * (int64_t)P->K * P->M2;
+        d.B = P->d_lam.as<double>() + P->lam_off[(size_t)b];
+        d.C = P->d_a.as<double>() + P->aoff[(size_t)b];
+        d.K = P->K;
+        d.N = n;
+        d.lda = P->M2;
+        d.ldb = n;
+        d.ldc = P->M2;
+        ana.push_back(d);
+        GemmDesc s;
+        s.A = P->d_a.as<double>() + P->aoff[(size_t)b];
+        s.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
+        s.C = P->d_G.as<double>() + b * (int64_t)P->K * P->M2;
+        s.K = n;
+        s.N = n > 0 ? P->K : 0;
+        s.lda = P->M2;
+        s.ldb = P->K;
+        s.ldc = P->M2;
+        syn.push_back(s);
+    }
+    // the residual-driven analyses read R instead of X0: a second descriptor set
+    std::vector<GemmDesc> anaR = ana;
+    for (int64_t b = 0; b < 2 * (L + 1); ++b) anaR[(size_t)b].A = P->d_R.as<double>() + b * (int64_t)P->K * P->M2;
+    ana.insert(ana.end(), anaR.begin(), anaR.end());
+    if ((rc = P->d_desc_ana.ensure(sizeof(GemmDesc) * ana.size()))) return rc;
+    if ((rc = P->d_desc_syn.ensure(sizeof(GemmDesc) * syn.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_ana.p, ana.data(), sizeof(GemmDesc) * ana.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_desc_syn.p, syn.data(), sizeof(GemmDesc) * syn.size(), hipMemcpyHostToDevice));
+    return SFB_OK;
+}
+
+extern "C" const char* sfb_version(void) { return "sfb_mi355x 0.1 (gfx950)"; }
+
+extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int64_t nmax, const int64_t* lmax_n,
+                               const int64_t* knot_off, const double* knot_x, const double* knot_y,
+                               const double* knot_ypp, int32_t niter, int32_t precision, int32_t device)
+{
+    sfb::clear_error();
+    if (!out) return fail(SFB_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!lmax_n || !knot_off || !knot_x || !knot_y || !knot_ypp) return fail(SFB_ERR_ARG, "null table pointer");
+    if (nside < 1 || nside > 1024 || (nside & (nside - 1))) return fail(SFB_ERR_ARG, "nside must be a power of two <= 1024");
+    if (lmax < 0 || nmax < 1 || niter < 0) return fail(SFB_ERR_ARG, "bad lmax/nmax/niter");
+    if (lmax > 4 * nside)
+        return fail(SFB_ERR_LMAX, "alm.lmax > 4*nside is a poor choice (lmax=%lld, nside=%lld)", (long long)lmax,
+                    (long long)nside);
+    if (precision != SFB_PREC_FP64 && precision != SFB_PREC_MIXED) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
+    int64_t nmaps = 0;
+    for (int64_t n = 0; n < nmax; ++n) {
+        if (lmax_n[n] < 0 || lmax_n[n] > lmax) return fail(SFB_ERR_ARG, "lmax_n[%lld] out of range", (long long)n);
+        nmaps += lmax_n[n] + 1;
+    }
+    if (nmaps > 65535) return fail(SFB_ERR_ARG, "more than 65535 (n,l) maps");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SFB_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    sfb_plan* P = new (std::nothrow) sfb_plan();
+    if (!P) return fail(SFB_ERR_NOMEM, "out of host memory");
+    P->device = device;
+    P->nside = nside;
+    P->npix = 12 * nside * nside;
+    P->lmax = lmax;
+    P->nmax = nmax;
+    P->nmaps = nmaps;
+    P->K = (int32_t)(2 * nside);
+    P->M2 = 2 * nmaps;
+    P->niter = niter;
+    P->precision = precision;
+    P->TQ = (int32_t)std::max<int64_t>(1, std::min<int64_t>(8, 65536 / (16 * (4 * nside + 1))));
+    hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete P;
+        return fail(SFB_ERR_DEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    int rc = plan_tables(P, lmax_n, knot_off, knot_x, knot_y, knot_ypp);
+    if (rc) {
+        sfb_plan_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return SFB_OK;
+}
+
+extern "C" void sfb_plan_destroy(sfb_plan* P)
+{
+    if (!P) return;
+    (void)hipSetDevice(P->device);
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    DevBuf* bufs[] = {&P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
+                      &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
+                      &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
+                      &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
+                      &P->h_rtp, &P->h_w, &P->h_wa, &P->h_wb, &P->h_out};
+    for (DevBuf* b : bufs) b->release();
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+}
+
+extern "C" int sfb_plan_sizes(const sfb_plan* P, int64_t* nmaps, int64_t* nlmsize, int64_t* npix)
+{
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nmaps) *nmaps = P->nmaps;
+    if (nlmsize) *nlmsize = P->nlmsize;
+    if (npix) *npix = P->npix;
+    return SFB_OK;
+}
+
+extern "C" int sfb_plan_set_precision(sfb_plan* P, int32_t precision)
+{
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (precision != SFB_PREC_FP64 && precision != SFB_PREC_MIXED) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
+    P->precision = precision;
+    return SFB_OK;
+}
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Sort the catalogue and build the pixel segments.  Leaves r_s, w_s, pstart.
+static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, hipStream_t st)
+{
+    int rc;
+    if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
+    if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
+    if (N == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->d_pstart.as<int32_t>(),
+                           P->npix + 1, 0);
+        LAUNCH_CHECK();
+        if ((rc = P->d_rs.ensure(8)) || (rc = P->d_ws.ensure(8))) return rc;
+        return SFB_OK;
+    }
+    size_t n = (size_t)N;
+    if ((rc = P->d_keyr.ensure(8 * n)) || (rc = P->d_keyr2.ensure(8 * n)) || (rc = P->d_idx.ensure(4 * n)) ||
+        (rc = P->d_idx2.ensure(4 * n)) || (rc = P->d_pix.ensure(4 * n)) || (rc = P->d_pixkey.ensure(4 * n)) ||
+        (rc = P->d_pixkey2.ensure(4 * n)) || (rc = P->d_rs.ensure(8 * n)) || (rc = P->d_ws.ensure(8 * n)))
+        return rc;
+    hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, N, d_rtp, P->nside, P->d_keyr.as<uint64_t>(),
+                       P->d_idx.as<uint32_t>(), P->d_pix.as<uint32_t>());
+    LAUNCH_CHECK();
+    // 1) stable sort by r
+    size_t tmp1 = 0, tmp2 = 0;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
+                                      P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
+    unsigned pbits = 1;
+    while (((int64_t)1 << pbits) < P->npix) ++pbits;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp2, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
+                                      P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
+    if ((rc = P->d_sorttmp.ensure(std::max(tmp1, tmp2)))) return rc;
+    tmp1 = P->d_sorttmp.bytes;
+    HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
+                                      P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
+    hipLaunchKernelGGL(k_gather_pix, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx2.as<uint32_t>(),
+                       P->d_pix.as<uint32_t>(), P->d_pixkey.as<uint32_t>());
+    LAUNCH_CHECK();
+    // 2) stable sort by pixel: (pixel, r, original index) order
+    tmp2 = P->d_sorttmp.bytes;
+    HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp2, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
+                                      P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
+    hipLaunchKernelGGL(k_gather, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx.as<uint32_t>(), d_rtp, d_w,
+                       P->d_rs.as<double>(), P->d_ws.as<double>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pixel_start, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->npix, N,
+                       P->d_pixkey2.as<uint32_t>(), P->d_pstart.as<int32_t>());
+    LAUNCH_CHECK();
+    return SFB_OK;
+}
+
+static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
+                         hipStream_t st)
+{
+    double domega = 4.0 * kPi / (double)P->npix;
+    double inv = 1.0 / (nbar * domega);
+    dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
+    if (win_kind == SFB_WIN_DENSE)
+        hipLaunchKernelGGL(k_scatter<true>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
+                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta.as<double>());
+    else
+        hipLaunchKernelGGL(k_scatter<false>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
+                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta.as<double>());
+    LAUNCH_CHECK();
+    return SFB_OK;
+}
+
+// delta (first nq maps) -> a, with niter refinements.
+static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
+{
+    const int32_t K = P->K, L = (int32_t)P->lmax;
+    const int32_t rows = 2 * nq;
+    const double domega = 4.0 * kPi / (double)P->npix;
+    size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
+    hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)K, nblk(nq, P->TQ)), dim3(256), shm, st, P->d_delta.as<double>(),
+                       P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), P->d_tw.as<double2>(),
+                       P->d_X0.as<double>(), P->TQ);
+    LAUNCH_CHECK();
+    const unsigned nbat = (unsigned)(2 * (L + 1));
+    const GemmDesc* ana = P->d_desc_ana.as<GemmDesc>();
+    const GemmDesc* anaR = ana + nbat;
+    const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>();
+    const int32_t maxnb = P->nb[0];
+    dim3 ga(nblk(rows, GBM), nblk(maxnb, GBN), nbat);
+    hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, ana, rows, domega, 0.0);
+    LAUNCH_CHECK();
+    dim3 gs(nblk(rows, GBM), nblk(K, GBN), nbat);
+    dim3 gr(nblk(nq, 256), (unsigned)K, (unsigned)(L + 1));
+    for (int it = 0; it < P->niter; ++it) {
+        hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_residual, gr, dim3(256), 0, st, P->d_X0.as<double>(), P->d_G.as<double>(),
+                           P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, L, P->M2, nq);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, anaR, rows, domega, 1.0);
+        LAUNCH_CHECK();
+    }
+    return SFB_OK;
+}
+
+static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, const double* wb)
+{
+    if (win_kind != SFB_WIN_DENSE && win_kind != SFB_WIN_SEPARABLE) return fail(SFB_ERR_ARG, "unknown window kind %d", win_kind);
+    if (!wa) return fail(SFB_ERR_ARG, "window array is NULL");
+    if (win_kind == SFB_WIN_SEPARABLE && !wb) return fail(SFB_ERR_ARG, "separable window needs w_ln");
+    (void)P;
+    return SFB_OK;
+}
+
+extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                                   int32_t win_kind, const double* d_win_a, const double* d_win_b, double* d_out,
+                                   void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (N < 0 || (N > 0 && !d_rtp) || !d_out) return fail(SFB_ERR_ARG, "bad catalogue or output pointer");
+    int rc = check_window(P, win_kind, d_win_a, d_win_b);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
+    if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st))) return rc;
+    if ((rc = sht_stage(P, (int32_t)P->nmaps, st))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 1, 0);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
+                       P->d_a.as<double>(), P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
+                       P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
+    LAUNCH_CHECK();
+    int32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, P->d_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (flag) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
+    return SFB_OK;
+}
+
+static int stage_inputs(sfb_plan* P, int64_t N, const double* rtp, const double* w, int32_t win_kind,
+                        const double* wa, const double* wb, const double** d_rtp, const double** d_w,
+                        const double** d_wa, const double** d_wb)
+{
+    int rc;
+    size_t nw = (win_kind == SFB_WIN_DENSE) ? (size_t)(P->npix * (P->lmax + 1) * P->nmax) : (size_t)P->npix;
+    size_t nb = (size_t)((P->lmax + 1) * P->nmax);
+    if ((rc = P->h_rtp.ensure(sizeof(double) * (size_t)(3 * std::max<int64_t>(N, 1))))) return rc;
+    if ((rc = P->h_wa.ensure(sizeof(double) * nw))) return rc;
+    if (N > 0) HIP_TRY(hipMemcpyAsync(P->h_rtp.p, rtp, sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->h_wa.p, wa, sizeof(double) * nw, hipMemcpyHostToDevice, P->stream));
+    *d_rtp = P->h_rtp.as<double>();
+    *d_wa = P->h_wa.as<double>();
+    *d_w = nullptr;
+    *d_wb = nullptr;
+    if (w && N > 0) {
+        if ((rc = P->h_w.ensure(sizeof(double) * (size_t)N))) return rc;
+        HIP_TRY(hipMemcpyAsync(P->h_w.p, w, sizeof(double) * (size_t)N, hipMemcpyHostToDevice, P->stream));
+        *d_w = P->h_w.as<double>();
+    }
+    if (win_kind == SFB_WIN_SEPARABLE) {
+        if ((rc = P->h_wb.ensure(sizeof(double) * nb))) return rc;
+        HIP_TRY(hipMemcpyAsync(P->h_wb.p, wb, sizeof(double) * nb, hipMemcpyHostToDevice, P->stream));
+        *d_wb = P->h_wb.as<double>();
+    }
+    return SFB_OK;
+}
+
+extern "C" int sfb_cat2amln(sfb_plan* P, int64_t N, const double* rtp, const double* w, double nbar, int32_t win_kind,
+                            const double* win_a, const double* win_b, double* anlm_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (N < 0 || (N > 0 && !rtp) || !anlm_out) return fail(SFB_ERR_ARG, "bad catalogue or output pointer");
+    int rc = check_window(P, win_kind, win_a, win_b);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    const double *d_rtp, *d_w, *d_wa, *d_wb;
+    if ((rc = stage_inputs(P, N, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
+    if ((rc = P->h_out.ensure(sizeof(double) * 2 * (size_t)P->nlmsize))) return rc;
+    rc = sfb_cat2amln_device(P, N, d_rtp, d_w, nbar, win_kind, d_wa, d_wb, P->h_out.as<double>(), P->stream);
+    if (rc && rc != SFB_ERR_NONFINITE) return rc;
+    HIP_TRY(hipMemcpyAsync(anlm_out, P->h_out.p, sizeof(double) * 2 * (size_t)P->nlmsize, hipMemcpyDeviceToHost,
+                           P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    if (rc) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
+    return SFB_OK;
+}
+
+extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const double* w, double nbar,
+                             int32_t win_kind, const double* win_a, const double* win_b, int64_t q, double* map_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (q < 0 || q >= P->nmaps || !map_out || N < 0 || (N > 0 && !rtp)) return fail(SFB_ERR_ARG, "bad arguments");
+    int rc = check_window(P, win_kind, win_a, win_b);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    const double *d_rtp, *d_w, *d_wa, *d_wb;
+    if ((rc = stage_inputs(P, N, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
+    if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream))) return rc;
+    if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream))) return rc;
+    HIP_TRY(hipMemcpyAsync(map_out, P->d_delta.as<double>() + q * P->npix, sizeof(double) * (size_t)P->npix,
+                           hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_map2alm(sfb_plan* P, int64_t nmap, const double* maps, double* alm_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nmap < 1 || nmap > P->nmaps || !maps || !alm_out) return fail(SFB_ERR_ARG, "bad map2alm arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    int rc;
+    HIP_TRY(hipMemcpyAsync(P->d_delta.p, maps, sizeof(double) * (size_t)(nmap * P->npix), hipMemcpyHostToDevice,
+                           P->stream));
+    if ((rc = sht_stage(P, (int32_t)nmap, P->stream))) return rc;
+    int64_t nalm = (P->lmax + 1) * (P->lmax + 2) / 2;
+    if ((rc = P->h_out.ensure(sizeof(double) * 2 * (size_t)(nalm * nmap)))) return rc;
+    hipLaunchKernelGGL(k_pack_alm, dim3(nblk(nmap, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, P->stream,
+                       P->d_a.as<double>(), P->d_aoff.as<int64_t>(), (int32_t)nmap, (int32_t)P->lmax, P->M2,
+                       P->h_out.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(alm_out, P->h_out.p, sizeof(double) * 2 * (size_t)(nalm * nmap), hipMemcpyDeviceToHost,
+                           P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const double* theta, const double* phi,
+                                int32_t* pix_out)
+{
+    sfb::clear_error();
+    if (N < 0 || (N > 0 && (!theta || !phi || !pix_out)) || nside < 1 || nside > (1 << 13))
+        return fail(SFB_ERR_ARG, "bad ang2pix arguments");
+    if (N == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(device));
+    double *dt = nullptr, *dp = nullptr;
+    int32_t* dq = nullptr;
+    HIP_TRY(hipMalloc(&dt, sizeof(double) * (size_t)N));
+    hipError_t e = hipMalloc(&dp, sizeof(double) * (size_t)N);
+    if (e == hipSuccess) e = hipMalloc(&dq, sizeof(int32_t) * (size_t)N);
+    if (e == hipSuccess) e = hipMemcpy(dt, theta, sizeof(double) * (size_t)N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, phi, sizeof(double) * (size_t)N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ang2pix, dim3(nblk(N, 256)), dim3(256), 0, 0, N, dt, dp, nside, dq);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(pix_out, dq, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost);
+    (void)hipFree(dt);
+    (void)hipFree(dp);
+    (void)hipFree(dq);
+    if (e != hipSuccess) return fail(SFB_ERR_DEVICE, "ang2pix: %s", hipGetErrorString(e));
+    return SFB_OK;
+}

@@ -1,0 +1,148 @@
+"""cat2amln on the MI355X -- host mirror of src/cat2anlm.jl:257-312.
+
+``cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight)`` has the reference's
+positional signature and return value (a complex vector of length
+getnlmsize(amodes), ordered n, l, m).  Keyword arguments select the map2alm
+iteration count (default 3, src/healpix_helpers.jl:50), the Legendre-step
+precision and the device.  Plans (device tables, work buffers) are cached per
+(amodes, niter, precision, device).
+"""
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import _lib as L
+from .modes import AnlmModes, getnlmsize
+from .windows import SeparableArray
+
+_PRECISIONS = {"fp64": L.SFB_PREC_FP64, "mixed": L.SFB_PREC_MIXED}
+
+
+class Plan:
+    """A device plan for one AnlmModes (C ABI sfb_plan)."""
+
+    def __init__(self, amodes: AnlmModes, niter=3, precision="fp64", device=0):
+        self.amodes = amodes
+        self.niter = int(niter)
+        self.precision = precision
+        self.device = int(device)
+        h = ctypes.c_void_p()
+        L.check(L.lib().sfb_plan_create(ctypes.byref(h), amodes.nside, amodes.lmax, amodes.nmax,
+                                        L.ptr(amodes.lmax_n), L.ptr(amodes.knot_off), L.ptr(amodes.knot_x),
+                                        L.ptr(amodes.knot_y), L.ptr(amodes.knot_ypp), self.niter,
+                                        _PRECISIONS[precision], self.device))
+        self._h = h
+        self._fin = weakref.finalize(self, L.lib().sfb_plan_destroy, h)
+        self.nlmsize = getnlmsize(amodes)
+        self.npix = 12 * amodes.nside ** 2
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, precision):
+        L.check(L.lib().sfb_plan_set_precision(self._h, _PRECISIONS[precision]))
+        self.precision = precision
+
+    def close(self):
+        self._fin()
+
+    # -- helpers --------------------------------------------------------
+    def _window(self, win):
+        am = self.amodes
+        if isinstance(win, SeparableArray):
+            a = L.f64(np.asarray(win.arr1).reshape(-1))
+            b = L.f64(np.asarray(win.arr2).reshape(-1, order="F"))
+            if a.size != self.npix or b.size != (am.lmax + 1) * am.nmax:
+                raise ValueError("separable window must be (mask[npix], w_ln[lmax+1, nmax])")
+            return L.SFB_WIN_SEPARABLE, a, b
+        w = np.asarray(win, dtype=np.float64)
+        if w.shape != (self.npix, am.lmax + 1, am.nmax):
+            raise ValueError(f"dense window must have shape {(self.npix, am.lmax + 1, am.nmax)}, got {w.shape}")
+        return L.SFB_WIN_DENSE, L.f64(w.reshape(-1, order="F")), None
+
+    @staticmethod
+    def _catalogue(rtp, weight):
+        rtp = np.asarray(rtp, dtype=np.float64)
+        if rtp.size == 0:
+            return 0, np.zeros(3), None
+        if rtp.ndim != 2 or rtp.shape[0] != 3:
+            raise ValueError("rθϕ must be a 3 x N array")
+        N = rtp.shape[1]
+        host = L.f64(rtp.T)  # N x 3 row-major == 3 x N column-major
+        w = None
+        if weight is not None and len(weight) > 0:
+            w = L.f64(weight)
+            if w.size != N:
+                raise ValueError("weight must have one entry per galaxy")
+        return N, host, w
+
+    def cat2amln(self, rtp, nbar, win_rhat_ln, weight=None):
+        N, host, w = self._catalogue(rtp, weight)
+        kind, a, b = self._window(win_rhat_ln)
+        out = np.empty(2 * self.nlmsize)
+        L.check(L.lib().sfb_cat2amln(self._h, N, L.ptr(host), L.ptr(w), float(nbar), kind, L.ptr(a), L.ptr(b),
+                                     L.ptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def cat2amln_device(self, N, d_rtp, d_w, nbar, win_kind, d_win_a, d_win_b, d_out, stream=None):
+        """All arrays are device pointers (ints) or torch CUDA tensors."""
+        def dp(t):
+            if t is None:
+                return None
+            return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
+        L.check(L.lib().sfb_cat2amln_device(self._h, int(N), dp(d_rtp), dp(d_w), float(nbar), int(win_kind),
+                                            dp(d_win_a), dp(d_win_b), dp(d_out),
+                                            None if stream is None else ctypes.c_void_p(stream)))
+
+    def delta_map(self, rtp, nbar, win_rhat_ln, weight, q):
+        N, host, w = self._catalogue(rtp, weight)
+        kind, a, b = self._window(win_rhat_ln)
+        out = np.empty(self.npix)
+        L.check(L.lib().sfb_delta_map(self._h, N, L.ptr(host), L.ptr(w), float(nbar), kind, L.ptr(a), L.ptr(b),
+                                      int(q), L.ptr(out)))
+        return out
+
+    def map2alm(self, maps):
+        maps = L.f64(np.atleast_2d(maps))
+        nmap = maps.shape[0]
+        nalm = (self.amodes.lmax + 1) * (self.amodes.lmax + 2) // 2
+        out = np.empty(2 * nalm * nmap)
+        L.check(L.lib().sfb_map2alm(self._h, nmap, L.ptr(maps), L.ptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(nmap, nalm)
+
+
+_plans: dict = {}
+
+
+def get_plan(amodes, niter=3, precision="fp64", device=0) -> Plan:
+    key = (id(amodes), int(niter), precision, int(device))
+    p = _plans.get(key)
+    if p is None or p.amodes is not amodes:
+        p = Plan(amodes, niter, precision, device)
+        _plans[key] = p
+    return p
+
+
+def cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight=None, *, niter=3, precision="fp64", device=0):
+    """Catalogue -> SFB coefficients a_nlm (src/cat2anlm.jl:257-310).
+
+    rθϕ: 3 x N (r, θ, ϕ); weight: N weights or None/empty for unit weights;
+    win_rhat_ln: dense (npix, lmax+1, nmax) array or SeparableArray(mask, w_ln).
+    Raises AssertionError on a non-finite result, like the reference."""
+    return get_plan(amodes, niter, precision, device).cat2amln(rθϕ, nbar, win_rhat_ln, weight)
+
+
+cat2nlm = cat2amln
+
+
+def ang2pix_ring(nside, theta, phi, device=0):
+    """0-based ring-scheme pixel of (θ, ϕ), computed on the GPU."""
+    th = L.f64(theta).reshape(-1)
+    ph = L.f64(phi).reshape(-1)
+    out = np.empty(th.size, dtype=np.int32)
+    L.check(L.lib().sfb_ang2pix_ring(int(device), int(nside), th.size, L.ptr(th), L.ptr(ph), L.ptr(out)))
+    return out
