@@ -135,6 +135,16 @@ int sfb_delta_map(sfb_plan* p, int64_t N, const double* rtp, const double* w, do
  * index m(2 lmax+1-m)/2 + l.  nmap must not exceed the plan's map count. */
 int sfb_map2alm(sfb_plan* p, int64_t nmap, const double* maps, double* alm_out);
 
+/* The batched Legendre GEMM kernel on one matrix: C[n][i] = sum_k A[k][i] B[k][n],
+ * A is K x rows, B is K x N, C is N x rows, all row-major.  No reference
+ * counterpart: it checks the MFMA kernel that carries the map2alm Legendre sums. */
+int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
+
+/* The per-ring phi-transform of nmap maps [nmap][npix]: x_out[((m*2+par)*K + k)*2*nmap + 2q+c],
+ * K = 2 nside ring pairs, par 0 = F_north + F_south, 1 = F_north - F_south,
+ * F_m = sum_j map_j e^{-i m phi_j} (the ring DFT inside libsharp's map2alm). */
+int sfb_phi_transform(sfb_plan* p, int64_t nmap, const double* maps, double* x_out);
+
 #ifdef __cplusplus
 }
 #endif

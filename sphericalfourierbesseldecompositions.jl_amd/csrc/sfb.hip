@@ -122,15 +122,23 @@ __device__ __forceinline__ uint64_t sortable_bits(double v)
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
+// flag[1] |= 1 when an angle is not finite (the reference's floor(Int, NaN) throws).
 __global__ void k_prep(int64_t N, const double* __restrict__ rtp, int64_t nside, uint64_t* __restrict__ key_r,
-                       uint32_t* __restrict__ idx, uint32_t* __restrict__ pix)
+                       uint32_t* __restrict__ idx, uint32_t* __restrict__ pix, int32_t* __restrict__ flag)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double r = rtp[3 * i], th = rtp[3 * i + 1], ph = rtp[3 * i + 2];
     key_r[i] = sortable_bits(r);
     idx[i] = (uint32_t)i;
-    pix[i] = (uint32_t)ang2pix_ring_dev(nside, th, ph);
+    int64_t p = 0;
+    if (isfinite(th) && isfinite(ph)) {
+        p = ang2pix_ring_dev(nside, th, ph);
+        p = min(max(p, (int64_t)0), 12 * nside * nside - 1);
+    } else {
+        atomicOr(flag + 1, 1);
+    }
+    pix[i] = (uint32_t)p;
 }
 
 __global__ void k_ang2pix(int64_t N, const double* __restrict__ th, const double* __restrict__ ph, int64_t nside,
@@ -289,7 +297,7 @@ __global__ __launch_bounds__(256) void k_phi_dft(const double* __restrict__ delt
 // Batched FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta C[n][i]
 // for rows i < rows.  Block tile 64(i) x 64(n), k-tile 16, 4 waves of 32 x 32.
 // v_mfma_f64_16x16x4f64 operand layout: lane l holds A[row l%16][k l/16],
-// B[k l/16][col l%16]; result register v of lane l is D[4(l/16)+v][l%16].
+// B[k l/16][col l%16]; result register v of lane l is D[l/16 + 4v][l%16].
 // The MFMA "rows" are the output columns n so the stores run along i.
 // ---------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16;
@@ -344,7 +352,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int n = n0 + wn + tn * 16 + 4 * (lane >> 4) + v;
+                int n = n0 + wn + tn * 16 + (lane >> 4) + 4 * v;
                 int i = i0 + wi + ti * 16 + (lane & 15);
                 if (n < d.N && i < rows) {
                     double* c = d.C + (int64_t)n * d.ldc + i;
@@ -637,7 +645,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     if ((rc = P->d_G.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
     if ((rc = P->d_R.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
     if ((rc = P->d_a.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
-    if ((rc = P->d_flag.ensure(sizeof(int32_t)))) return rc;
+    if ((rc = P->d_flag.ensure(2 * sizeof(int32_t)))) return rc;
 
     // GEMM descriptors: analysis A = X (k = ring pair), B = Lam, C = a;
     // synthesis A = a (k = l index), B = LamT, C = G.
@@ -769,6 +777,8 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
+    LAUNCH_CHECK();
     if (N == 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->d_pstart.as<int32_t>(),
                            P->npix + 1, 0);
@@ -782,7 +792,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         (rc = P->d_pixkey2.ensure(4 * n)) || (rc = P->d_rs.ensure(8 * n)) || (rc = P->d_ws.ensure(8 * n)))
         return rc;
     hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, N, d_rtp, P->nside, P->d_keyr.as<uint64_t>(),
-                       P->d_idx.as<uint32_t>(), P->d_pix.as<uint32_t>());
+                       P->d_idx.as<uint32_t>(), P->d_pix.as<uint32_t>(), P->d_flag.as<int32_t>());
     LAUNCH_CHECK();
     // 1) stable sort by r
     size_t tmp1 = 0, tmp2 = 0;
@@ -886,16 +896,15 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st))) return rc;
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st))) return rc;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 1, 0);
-    LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
                        P->d_a.as<double>(), P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
                        P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
     LAUNCH_CHECK();
-    int32_t flag = 0;
-    HIP_TRY(hipMemcpyAsync(&flag, P->d_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    int32_t flag[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (flag) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
+    if (flag[1]) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
+    if (flag[0]) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
     return SFB_OK;
 }
 
@@ -1013,5 +1022,64 @@ extern "C" int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const 
     (void)hipFree(dp);
     (void)hipFree(dq);
     if (e != hipSuccess) return fail(SFB_ERR_DEVICE, "ang2pix: %s", hipGetErrorString(e));
+    return SFB_OK;
+}
+
+extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
+                            double* C)
+{
+    sfb::clear_error();
+    if (rows < 1 || K < 1 || N < 1 || rows > (1 << 24) || N > (1 << 24) || !A || !B || !C)
+        return fail(SFB_ERR_ARG, "bad gemm arguments");
+    HIP_TRY(hipSetDevice(device));
+    DevBuf dA, dB, dC, dD;
+    int rc;
+    if ((rc = dA.ensure(sizeof(double) * (size_t)(K * rows))) || (rc = dB.ensure(sizeof(double) * (size_t)(K * N))) ||
+        (rc = dC.ensure(sizeof(double) * (size_t)(N * rows))) || (rc = dD.ensure(sizeof(GemmDesc))))
+        return rc;
+    hipError_t e = hipMemcpy(dA.p, A, sizeof(double) * (size_t)(K * rows), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB.p, B, sizeof(double) * (size_t)(K * N), hipMemcpyHostToDevice);
+    GemmDesc d;
+    d.A = dA.as<double>();
+    d.B = dB.as<double>();
+    d.C = dC.as<double>();
+    d.K = (int32_t)K;
+    d.N = (int32_t)N;
+    d.lda = rows;
+    d.ldb = N;
+    d.ldc = rows;
+    if (e == hipSuccess) e = hipMemcpy(dD.p, &d, sizeof d, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_gemm_f64, dim3(nblk(rows, GBM), nblk(N, GBN), 1), dim3(256), 0, 0, dD.as<GemmDesc>(),
+                           (int32_t)rows, 1.0, 0.0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(C, dC.p, sizeof(double) * (size_t)(N * rows), hipMemcpyDeviceToHost);
+    dA.release();
+    dB.release();
+    dC.release();
+    dD.release();
+    if (e != hipSuccess) return fail(SFB_ERR_DEVICE, "gemm: %s", hipGetErrorString(e));
+    return SFB_OK;
+}
+
+extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, double* x_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nmap < 1 || nmap > P->nmaps || !maps || !x_out) return fail(SFB_ERR_ARG, "bad phi_transform arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipMemcpyAsync(P->d_delta.p, maps, sizeof(double) * (size_t)(nmap * P->npix), hipMemcpyHostToDevice,
+                           P->stream));
+    size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
+    hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)P->K, nblk(nmap, P->TQ)), dim3(256), shm, P->stream,
+                       P->d_delta.as<double>(), P->npix, (int32_t)nmap, P->M2, P->K, (int32_t)P->lmax,
+                       P->d_rings.as<RingDev>(), P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQ);
+    LAUNCH_CHECK();
+    // out[(b * K + k) * 2 nmap + i]
+    HIP_TRY(hipMemcpy2DAsync(x_out, sizeof(double) * 2 * (size_t)nmap, P->d_X0.p, sizeof(double) * (size_t)P->M2,
+                             sizeof(double) * 2 * (size_t)nmap, (size_t)(2 * (P->lmax + 1) * P->K),
+                             hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
     return SFB_OK;
 }

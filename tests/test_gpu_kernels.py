@@ -1,0 +1,47 @@
+"""Unit checks of single HIP kernels of the path through their check entry points."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import sfb_amd as S
+from sfb_amd import _lib as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (64, 16, 64), (70, 37, 23), (130, 256, 71)])
+def test_gemm_f64_matches_numpy(rows, K, N):
+    rng = np.random.default_rng(rows + K + N)
+    A = rng.normal(size=(K, rows))
+    B = rng.normal(size=(K, N))
+    C = np.empty((N, rows))
+    L.check(L.lib().sfb_gemm_f64(0, rows, K, N, L.ptr(A), L.ptr(B), L.ptr(C)))
+    ref = B.T @ A
+    assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * K
+
+
+def test_phi_transform_matches_direct_dft():
+    nside, lmax = 8, 16
+    am = S.AnlmModes.from_nl(2, lmax, 500.0, 1000.0, nside=nside)
+    plan = S.Plan(am)
+    npix = 12 * nside * nside
+    maps = np.random.default_rng(3).normal(size=(2, npix))
+    K = 2 * nside
+    out = np.empty((2 * (lmax + 1), K, 4))
+    L.check(L.lib().sfb_phi_transform(plan.handle, 2, L.ptr(maps), L.ptr(out)))
+    for k in range(K):
+        st, npr, z, phi0 = O.ring_info(nside, k + 1)
+        sts = st if k == K - 1 else npix - npr - st
+        j = np.arange(npr)
+        for m in range(lmax + 1):
+            e = np.exp(-1j * m * (phi0 + 2 * np.pi * j / npr))
+            for q in range(2):
+                FN = np.sum(maps[q, st:st + npr] * e)
+                FS = 0 if k == K - 1 else np.sum(maps[q, sts:sts + npr] * e)
+                E, Od = (FN, FN) if k == K - 1 else (FN + FS, FN - FS)
+                got_e = out[2 * m, k, 2 * q] + 1j * out[2 * m, k, 2 * q + 1]
+                got_o = out[2 * m + 1, k, 2 * q] + 1j * out[2 * m + 1, k, 2 * q + 1]
+                assert abs(got_e - E) < 1e-12 * (1 + abs(E)), (k, m, q, got_e, E)
+                assert abs(got_o - Od) < 1e-12 * (1 + abs(Od)), (k, m, q, got_o, Od)
