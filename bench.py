@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""cat2amln throughput on one MI355X at N=1e6, kmax=0.1, nside=128.
+
+Workload: 1e6 galaxies uniform in volume in the shell 500 <= r <= 1500 Mpc/h
+(r^3 uniform, cos(theta) uniform, phi uniform, seed 0), unit weights,
+nbar = N/V, AnlmModes(kmax=0.1, rmin=500, rmax=1500; nside=128) with the
+potential boundary condition (lmax=140, nmax=32, 2514 (n,l) maps, 116345 a_nlm),
+full-sky separable window (mask = 1, phi(r) = 1, nr = 1000), map2alm niter=3.
+
+One step = one sfb_cat2amln_device call with the catalogue, the window and the
+output already in HBM: ang2pix, both sorts, the spline scatter, the phi-transform,
+all Legendre passes and the packing.  Plan setup (AnlmModes, splines, lambda
+tables) is outside the timed region and reported separately.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "sphericalfourierbesseldecompositions.jl_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "cat2amln Mgal/s at N=1e6,kmax=0.1,nside=128: mixed vs FP64 GPU, max rel err"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def catalogue(N, rmin, rmax, seed):
+    rng = np.random.Generator(np.random.Philox(seed))
+    u = rng.random((3, N))
+    r = np.cbrt(u[0] * (rmax ** 3 - rmin ** 3) + rmin ** 3)
+    th = np.arccos(np.clip(1.0 - 2.0 * u[1], -1.0, 1.0))
+    ph = 2 * np.pi * u[2]
+    return np.stack([r, th, ph])
+
+
+def algorithmic_work(am, N, niter):
+    """Per-launch algorithmic bytes / flops of every phase."""
+    L, nside = am.lmax, am.nside
+    npix, K, nmaps = 12 * nside * nside, 2 * nside, am.nmaps
+    M2 = 2 * nmaps
+    nlm_pairs = (L + 1) * (L + 2) // 2  # sum over (m, parity) of the l count
+    delta_bytes = nmaps * npix * 8
+    x_bytes = (L + 1) * 2 * K * M2 * 8
+    a_bytes = nlm_pairs * M2 * 8
+    return {
+        "scatter": {"bytes": delta_bytes + N * 16 + (npix + 1) * 4 + npix * 8,
+                    "flops": 20.0 * N * nmaps},
+        "phi": {"bytes": delta_bytes + x_bytes, "flops": 4.0 * npix * (L + 1) * nmaps},
+        # a = dOmega * X Lambda with N/S ring-pair symmetry: 2 flop x rows x K x sum n_b
+        "analysis": {"bytes": x_bytes + a_bytes, "flops": 2.0 * M2 * K * nlm_pairs},
+        "synthesis": {"bytes": x_bytes + a_bytes, "flops": 2.0 * M2 * K * nlm_pairs},
+        "residual": {"bytes": 3 * x_bytes, "flops": 0.0},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--cpu-maps", type=int, default=48, help="maps the CPU baseline sample runs")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the (n,l) maps would shard "
+              "across GPUs as described in DESIGN.md, but no multi-GPU path is built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import sfb_amd as S
+    from sfb_amd import _lib as L
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    rmin, rmax, kmax, nside, niter = 500.0, 1500.0, 0.1, 128, 3
+    N = args.N
+
+    t0 = time.perf_counter()
+    am = S.AnlmModes(kmax, rmin, rmax, nside=nside)
+    t_modes = time.perf_counter() - t0
+    log(f"AnlmModes: lmax={am.lmax} nmax={am.nmax} maps={am.nmaps} nlm={S.getnlmsize(am)} ({t_modes:.2f}s)")
+    wmodes = S.ConfigurationSpaceModes(rmin, rmax, 1000, nside)
+    npix = 12 * nside * nside
+    t0 = time.perf_counter()
+    win = S.win_rhat_ln(S.SeparableArray(np.ones(1000), np.ones(npix), "phi", "mask"), wmodes, am)
+    t_win = time.perf_counter() - t0
+    cat = catalogue(N, rmin, rmax, seed=0)
+    nbar = N / (4 * np.pi / 3 * (rmax ** 3 - rmin ** 3))
+
+    d_rtp = torch.from_numpy(np.ascontiguousarray(cat.T)).to(dev)
+    d_mask = torch.from_numpy(np.ascontiguousarray(win.mask, dtype=np.float64)).to(dev)
+    d_wln = torch.from_numpy(np.ascontiguousarray(win.w_ln.reshape(-1, order="F"))).to(dev)
+    nlm = S.getnlmsize(am)
+    d_out = torch.empty(2 * nlm, dtype=torch.float64, device=dev)
+
+    t0 = time.perf_counter()
+    plan = S.Plan(am, niter=niter, precision="fp64")
+    t_plan = time.perf_counter() - t0
+    log(f"plan: {t_plan:.2f}s, window: {t_win:.2f}s")
+
+    def run(prec):
+        plan.set_precision(prec)
+        for _ in range(args.warmup):
+            plan.cat2amln_device(N, d_rtp, None, nbar, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_out)
+        plan.profile(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plan.cat2amln_device(N, d_rtp, None, nbar, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_out)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        phases, calls = plan.phase_times()
+        plan.profile(False)
+        out = d_out.cpu().numpy()
+        log(f"[{prec}] {dt * 1e3:.2f} ms/step  " +
+            "  ".join(f"{k}={v[0] / calls:.2f}ms/{v[1] // max(calls, 1)}" for k, v in phases.items()))
+        return dt, phases, calls, out[0::2] + 1j * out[1::2]
+
+    results = {}
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    for prec in precs:
+        results[prec] = run(prec)
+
+    head = "mixed" if "mixed" in results else "fp64"
+    dt, phases, calls, anlm = results[head]
+    work = algorithmic_work(am, N, niter)
+
+    # dominant kernel of the headline path and its roofline
+    per_launch = {k: (v[0] / max(v[1], 1)) for k, v in phases.items() if v[1] > 0}
+    total = {k: v[0] for k, v in phases.items()}
+    dom = max((k for k in total if k in work), key=lambda k: total[k])
+    w = work[dom]
+    t_launch = per_launch[dom] * 1e-3
+    if dom in ("analysis", "synthesis") or (dom == "phi" and w["flops"] / w["bytes"] > 10):
+        peak = FP64_PEAK_TFLOPS if head == "fp64" else FP32_PEAK_TFLOPS
+        achieved = w["flops"] / t_launch / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
+    else:
+        achieved = w["bytes"] / t_launch / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+
+    # CPU baseline + accuracy on a bounded sample of maps
+    cpu = None
+    errs = {}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        om = O.OracleModes.__new__(O.OracleModes)
+        om.nside, om.lmax, om.nmax, om.lmax_n = am.nside, am.lmax, am.nmax, am.lmax_n
+        om.knot_off, om.knot_x, om.knot_y, om.knot_ypp = am.knot_off, am.knot_x, am.knot_y, am.knot_ypp
+        q1 = min(args.cpu_maps, am.nmaps)
+        t = time.perf_counter()
+        ref = O.cat2amln(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), q_range=(0, q1),
+                         nthreads=args.cpu_threads)
+        tc = time.perf_counter() - t
+        t_full = tc * am.nmaps / q1
+        cpu = {"value": N / t_full / 1e6, "unit": "Mgal/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"oracle C restatement (OpenMP over maps) on maps 0..{q1 - 1} of {am.nmaps} of the same "
+                         f"1e6-galaxy catalogue: {tc:.1f}s, scaled by {am.nmaps}/{q1}"}
+        sel = np.zeros(nlm, dtype=bool)
+        i = 0
+        for n in range(1, am.nmax + 1):
+            for l in range(int(am.lmax_n[n - 1]) + 1):
+                if am.map_index(n, l) < q1:
+                    sel[i:i + l + 1] = True
+                i += l + 1
+        for prec, (_, _, _, a) in results.items():
+            d = np.abs(a[sel] - ref[sel])
+            per_mode = d / np.abs(ref[sel])
+            errs[prec] = {"max_rel": float(per_mode.max()), "rms_rel": float(np.sqrt(np.mean(per_mode ** 2))),
+                          "max_rel_to_max": float(d.max() / np.abs(ref[sel]).max())}
+        log(f"cpu baseline: {cpu['value']:.5f} Mgal/s ({tc:.1f}s for {q1} maps); errors {errs}")
+
+    value = N / dt / 1e6
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mgal/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32/f64 mixed", "data": "synthetic",
+        "config": {"workload": "N=1e6 uniform shell 500-1500 Mpc/h, kmax=0.1, nside=128, lmax=140, nmax=32, "
+                               "2514 maps, niter=3, separable full-sky window",
+                   "path": head},
+        "fp64_Mgal_s": N / results["fp64"][0] / 1e6 if "fp64" in results else None,
+        "mixed_Mgal_s": N / results["mixed"][0] / 1e6 if "mixed" in results else None,
+        "speedup_mixed_over_fp64": (results["fp64"][0] / results["mixed"][0])
+        if ("mixed" in results and "fp64" in results) else None,
+        "max_rel_err": errs.get(head, {}).get("max_rel"),
+        "errors_vs_oracle": errs,
+        "phases_ms": {k: v[0] / max(calls, 1) for k, v in phases.items()},
+        "setup_s": {"AnlmModes": t_modes, "plan": t_plan},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

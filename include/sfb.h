@@ -117,6 +117,25 @@ int sfb_cat2amln_device(sfb_plan* p, int64_t N, const double* d_rtp, const doubl
                         int32_t win_kind, const double* d_win_a, const double* d_win_b,
                         double* d_anlm_out, void* stream);
 
+/* ---- per-phase timing ------------------------------------------------------ */
+
+#define SFB_PHASE_CATALOG 0    /* ang2pix, the two stable sorts, gathers, pixel segments */
+#define SFB_PHASE_SCATTER 1    /* spline evaluation + per-pixel sums -> delta maps */
+#define SFB_PHASE_PHI 2        /* per-ring phi-transform */
+#define SFB_PHASE_ANALYSIS 3   /* Legendre analysis GEMMs (1 + niter launches) */
+#define SFB_PHASE_SYNTHESIS 4  /* Legendre synthesis GEMMs (niter launches) */
+#define SFB_PHASE_RESIDUAL 5   /* ring-Fourier residual with aliasing (niter launches) */
+#define SFB_PHASE_PACK 6       /* a_nlm packing and the non-finite check */
+#define SFB_NPHASE 7
+
+/* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and
+ * clears the totals.  No reference counterpart (the reference prints @time). */
+int sfb_plan_profile(sfb_plan* p, int32_t enable);
+
+/* Accumulated milliseconds and launch counts per phase (SFB_NPHASE entries each)
+ * and the number of profiled calls since sfb_plan_profile. */
+int sfb_plan_phase_times(const sfb_plan* p, double* ms, int64_t* launches, int64_t* calls);
+
 /* ---- pieces of the path, exposed for parity checks ------------------------ */
 
 /* ang2pixRing (Healpix.jl, called at src/cat2anlm.jl:263) on the GPU, 0-based. */

@@ -513,7 +513,50 @@ struct sfb_plan {
     // host-API staging
     DevBuf h_rtp, h_w, h_wa, h_wb, h_out;
     int64_t cat_cap = -1;
+    // per-phase event timing (sfb_plan_profile)
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[SFB_NPHASE];
+    double prof_ms[SFB_NPHASE] = {};
+    int64_t prof_launches[SFB_NPHASE] = {};
+    int64_t prof_calls = 0;
 };
+
+// Brackets the launches of one phase with events on stream st when profiling is on.
+struct PhaseTimer {
+    sfb_plan* P;
+    int ph;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseTimer(sfb_plan* P_, int ph_, hipStream_t st_) : P(P_), ph(ph_), st(st_)
+    {
+        if (P->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, st);
+    }
+    ~PhaseTimer()
+    {
+        if (a && b) {
+            (void)hipEventRecord(b, st);
+            P->pend[ph].emplace_back(a, b);
+        }
+    }
+};
+
+// After the stream has drained: fold the pending event pairs into the totals.
+static void prof_collect(sfb_plan* P)
+{
+    for (int ph = 0; ph < SFB_NPHASE; ++ph) {
+        for (auto& e : P->pend[ph]) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+                P->prof_ms[ph] += ms;
+                P->prof_launches[ph] += 1;
+            }
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        P->pend[ph].clear();
+    }
+}
 
 static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_off, const double* kx,
                        const double* ky, const double* kypp)
@@ -742,6 +785,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
     if (!P) return;
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
+    prof_collect(P);
     DevBuf* bufs[] = {&P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
@@ -791,6 +835,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         (rc = P->d_idx2.ensure(4 * n)) || (rc = P->d_pix.ensure(4 * n)) || (rc = P->d_pixkey.ensure(4 * n)) ||
         (rc = P->d_pixkey2.ensure(4 * n)) || (rc = P->d_rs.ensure(8 * n)) || (rc = P->d_ws.ensure(8 * n)))
         return rc;
+    PhaseTimer timer(P, SFB_PHASE_CATALOG, st);
     hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, N, d_rtp, P->nside, P->d_keyr.as<uint64_t>(),
                        P->d_idx.as<uint32_t>(), P->d_pix.as<uint32_t>(), P->d_flag.as<int32_t>());
     LAUNCH_CHECK();
@@ -828,6 +873,7 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
     dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
+    PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
     if (win_kind == SFB_WIN_DENSE)
         hipLaunchKernelGGL(k_scatter<true>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
                            P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
@@ -847,28 +893,43 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     const int32_t rows = 2 * nq;
     const double domega = 4.0 * kPi / (double)P->npix;
     size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
-    hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)K, nblk(nq, P->TQ)), dim3(256), shm, st, P->d_delta.as<double>(),
-                       P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), P->d_tw.as<double2>(),
-                       P->d_X0.as<double>(), P->TQ);
-    LAUNCH_CHECK();
+    {
+        PhaseTimer timer(P, SFB_PHASE_PHI, st);
+        hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)K, nblk(nq, P->TQ)), dim3(256), shm, st,
+                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQ);
+        LAUNCH_CHECK();
+    }
     const unsigned nbat = (unsigned)(2 * (L + 1));
     const GemmDesc* ana = P->d_desc_ana.as<GemmDesc>();
     const GemmDesc* anaR = ana + nbat;
     const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>();
     const int32_t maxnb = P->nb[0];
     dim3 ga(nblk(rows, GBM), nblk(maxnb, GBN), nbat);
-    hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, ana, rows, domega, 0.0);
-    LAUNCH_CHECK();
+    {
+        PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
+        hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, ana, rows, domega, 0.0);
+        LAUNCH_CHECK();
+    }
     dim3 gs(nblk(rows, GBM), nblk(K, GBN), nbat);
     dim3 gr(nblk(nq, 256), (unsigned)K, (unsigned)(L + 1));
     for (int it = 0; it < P->niter; ++it) {
-        hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
-        LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_residual, gr, dim3(256), 0, st, P->d_X0.as<double>(), P->d_G.as<double>(),
-                           P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, L, P->M2, nq);
-        LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, anaR, rows, domega, 1.0);
-        LAUNCH_CHECK();
+        {
+            PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
+            hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
+            LAUNCH_CHECK();
+        }
+        {
+            PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
+            hipLaunchKernelGGL(k_residual, gr, dim3(256), 0, st, P->d_X0.as<double>(), P->d_G.as<double>(),
+                               P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, L, P->M2, nq);
+            LAUNCH_CHECK();
+        }
+        {
+            PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
+            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, anaR, rows, domega, 1.0);
+            LAUNCH_CHECK();
+        }
     }
     return SFB_OK;
 }
@@ -896,13 +957,20 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st))) return rc;
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st))) return rc;
-    hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
-                       P->d_a.as<double>(), P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
-                       P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
-    LAUNCH_CHECK();
+    {
+        PhaseTimer timer(P, SFB_PHASE_PACK, st);
+        hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
+                           P->d_a.as<double>(), P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
+                           P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
+        LAUNCH_CHECK();
+    }
     int32_t flag[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (P->prof) {
+        prof_collect(P);
+        P->prof_calls += 1;
+    }
     if (flag[1]) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
     if (flag[0]) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
     return SFB_OK;
@@ -1081,5 +1149,31 @@ extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, 
                              sizeof(double) * 2 * (size_t)nmap, (size_t)(2 * (P->lmax + 1) * P->K),
                              hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_plan_profile(sfb_plan* P, int32_t enable)
+{
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    prof_collect(P);
+    for (int ph = 0; ph < SFB_NPHASE; ++ph) {
+        P->prof_ms[ph] = 0.0;
+        P->prof_launches[ph] = 0;
+    }
+    P->prof_calls = 0;
+    P->prof = enable != 0;
+    return SFB_OK;
+}
+
+extern "C" int sfb_plan_phase_times(const sfb_plan* P, double* ms, int64_t* launches, int64_t* calls)
+{
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    for (int ph = 0; ph < SFB_NPHASE; ++ph) {
+        if (ms) ms[ph] = P->prof_ms[ph];
+        if (launches) launches[ph] = P->prof_launches[ph];
+    }
+    if (calls) *calls = P->prof_calls;
     return SFB_OK;
 }

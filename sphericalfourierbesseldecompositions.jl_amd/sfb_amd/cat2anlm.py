@@ -50,6 +50,18 @@ class Plan:
     def close(self):
         self._fin()
 
+    def profile(self, enable=True):
+        """Per-phase HIP-event timing of subsequent calls (clears the totals)."""
+        L.check(L.lib().sfb_plan_profile(self._h, int(bool(enable))))
+
+    def phase_times(self):
+        """{phase: (total_ms, launches)} and the number of profiled calls."""
+        ms = np.zeros(len(L.PHASES))
+        n = np.zeros(len(L.PHASES), dtype=np.int64)
+        calls = ctypes.c_int64()
+        L.check(L.lib().sfb_plan_phase_times(self._h, L.ptr(ms), L.ptr(n), ctypes.byref(calls)))
+        return {ph: (float(ms[i]), int(n[i])) for i, ph in enumerate(L.PHASES)}, calls.value
+
     # -- helpers --------------------------------------------------------
     def _window(self, win):
         am = self.amodes
