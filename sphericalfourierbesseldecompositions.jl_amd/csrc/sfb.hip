@@ -66,9 +66,10 @@ struct RingDev {
 };
 
 struct GemmDesc {
-    const double* A;  // A[k * lda + i]
-    const double* B;  // B[k * ldb + n]
-    double* C;        // C[n * ldc + i]
+    const double* A;    // A[k * lda + i]
+    const double* B;    // B[k * ldb + n]
+    double* C;          // C[n * ldc + i]
+    const double* Cin;  // beta term source (same layout as C); NULL = C itself
     int32_t K, N;
     int64_t lda, ldb, ldc;
 };
@@ -228,6 +229,148 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __
     delta[(int64_t)q * npix + p] = __dsub_rn(v, W);
 }
 
+// g_nl(r) from a table staged in LDS: tb[k] = (x_k, y_k, ypp_k, a_k) with
+// a_k = h^2 (ypp_{k+1} - ypp_k)/6 precomputed on the host; b, c, d and t are
+// formed here with the reference's expressions (Splines.jl:327-357).
+__device__ __forceinline__ double spline_lds(const MapInfo& mi, const double4* __restrict__ tb, double r)
+{
+    if (r < mi.xfirst) return mi.yfirst;
+    if (r > mi.xlast) return mi.ylast;
+    if (!(r == r)) return r;
+    int i = (int)(__dmul_rn(__dsub_rn(r, mi.xfirst), mi.invdx));
+    i = min(max(i, 0), mi.nk - 2);
+    double4 lo = tb[i];
+    double2 hi = *(const double2*)&tb[i + 1];
+    while (i > 0 && r < lo.x) {
+        --i;
+        lo = tb[i];
+        hi = *(const double2*)&tb[i + 1];
+    }
+    while (i < mi.nk - 2 && r >= hi.x) {
+        ++i;
+        lo = tb[i];
+        hi = *(const double2*)&tb[i + 1];
+    }
+    const double h = __dsub_rn(hi.x, lo.x);
+    const double t = __ddiv_rn(__dsub_rn(r, lo.x), h);
+    const double h2 = __dmul_rn(h, h);
+    const double b = __dmul_rn(__dmul_rn(h2, lo.z), 0.5);
+    const double a = lo.w;
+    const double d = lo.y;
+    const double c = __dsub_rn(__dsub_rn(__dsub_rn(hi.y, a), b), d);
+    return fma(t, fma(t, fma(t, a, b), c), d);
+}
+
+// Four independent evaluations of spline_lds, written out so the LDS reads and
+// the divisions of the four galaxies overlap.
+__device__ __forceinline__ void spline_lds4(const MapInfo& mi, const double4* __restrict__ tb, const double r[4],
+                                            double out[4])
+{
+    int i[4];
+    double4 lo[4];
+    double2 hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int v = (int)(__dmul_rn(__dsub_rn(r[u], mi.xfirst), mi.invdx));
+        i[u] = min(max(v, 0), mi.nk - 2);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        lo[u] = tb[i[u]];
+        hi[u] = *(const double2*)&tb[i[u] + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        while (i[u] > 0 && r[u] < lo[u].x) {
+            --i[u];
+            lo[u] = tb[i[u]];
+            hi[u] = *(const double2*)&tb[i[u] + 1];
+        }
+        while (i[u] < mi.nk - 2 && r[u] >= hi[u].x) {
+            ++i[u];
+            lo[u] = tb[i[u]];
+            hi[u] = *(const double2*)&tb[i[u] + 1];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double h = __dsub_rn(hi[u].x, lo[u].x);
+        const double t = __ddiv_rn(__dsub_rn(r[u], lo[u].x), h);
+        const double h2 = __dmul_rn(h, h);
+        const double b = __dmul_rn(__dmul_rn(h2, lo[u].z), 0.5);
+        const double a = lo[u].w;
+        const double d = lo[u].y;
+        const double c = __dsub_rn(__dsub_rn(__dsub_rn(hi[u].y, a), b), d);
+        double v = fma(t, fma(t, fma(t, a, b), c), d);
+        // boundary extrapolation and NaN, as in spline_lds
+        v = (r[u] < mi.xfirst) ? mi.yfirst : v;
+        v = (r[u] > mi.xlast) ? mi.ylast : v;
+        out[u] = (r[u] == r[u]) ? v : r[u];
+    }
+}
+
+// delta maps with the map's spline table in LDS.  Block (q, chunk): the table of
+// map q, then pixel tiles of PT pixels; the tile's galaxies are evaluated one per
+// thread into wg[], then every pixel sums its own galaxies in order.
+template <bool DENSE>
+__global__ __launch_bounds__(1024) void k_scatter_lds(int64_t npix, int64_t pix_per_chunk, int32_t PT, int32_t WG,
+                                                     int32_t tbl_cap, const int32_t* __restrict__ pstart,
+                                                     const double* __restrict__ r_s, const double* __restrict__ w_s,
+                                                     const MapInfo* __restrict__ maps, const double4* __restrict__ rec,
+                                                     double inv, const double* __restrict__ win_a,
+                                                     const double* __restrict__ win_b, int64_t lmax,
+                                                     double* __restrict__ delta)
+{
+    extern __shared__ double4 sm4[];
+    const int q = blockIdx.x;
+    const MapInfo mi = maps[q];
+    double4* tb = sm4;
+    double* wg = (double*)(sm4 + tbl_cap);
+    for (int k = threadIdx.x; k < mi.nk; k += blockDim.x) tb[k] = rec[mi.koff + k];
+    __syncthreads();
+    const int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
+    const double wcol = DENSE ? 0.0 : win_b[col];
+    const int64_t pb = (int64_t)blockIdx.y * pix_per_chunk;
+    const int64_t pe = min(npix, pb + pix_per_chunk);
+    for (int64_t p0 = pb; p0 < pe; p0 += PT) {
+        const int64_t p1 = min(p0 + (int64_t)PT, pe);
+        const int32_t G0 = pstart[p0], G1 = pstart[p1];
+        const int64_t p = p0 + threadIdx.x;
+        const bool own = (int)threadIdx.x < PT && p < p1;
+        double s = 0.0;
+        if (G1 - G0 <= WG) {
+            const int ng = G1 - G0;
+            const int stride = blockDim.x;
+            for (int j0 = threadIdx.x; j0 < ng; j0 += 4 * stride) {
+                double rr[4], ww[4], vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = min(j0 + u * stride, ng - 1);
+                    rr[u] = r_s[G0 + j];
+                    ww[u] = w_s[G0 + j];
+                }
+                spline_lds4(mi, tb, rr, vv);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j0 + u * stride < ng) wg[j0 + u * stride] = __dmul_rn(ww[u], vv[u]);
+            }
+            __syncthreads();
+            if (own) {
+                const int32_t a = pstart[p] - G0, b = pstart[p + 1] - G0;
+                for (int32_t j = a; j < b; ++j) s = __dadd_rn(s, wg[j]);
+            }
+            __syncthreads();
+        } else if (own) {
+            const int32_t b = pstart[p + 1];
+            for (int32_t g = pstart[p]; g < b; ++g) s = __dadd_rn(s, __dmul_rn(w_s[g], spline_lds(mi, tb, r_s[g])));
+        }
+        if (own) {
+            const double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], wcol);
+            delta[(int64_t)q * npix + p] = __dsub_rn(__dmul_rn(s, inv), W);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // phi-transform of one ring pair for TQ maps.  X[(m*2+par)*K + k][i], i = 2q+c.
 // ---------------------------------------------------------------------------
@@ -293,6 +436,167 @@ __global__ __launch_bounds__(256) void k_phi_dft(const double* __restrict__ delt
     }
 }
 
+// Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
+// every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
+// each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
+__device__ __forceinline__ void store_folded(double* __restrict__ X, int64_t M2, int32_t K, int32_t k, bool eq,
+                                             int npr, int shift, const double2* __restrict__ twr, int L, int kf,
+                                             int q, double xnr, double xni, double xsr, double xsi)
+{
+    const int n2 = 2 * npr;
+    for (int fam = 0; fam < 2; ++fam) {
+        if (fam == 1 && (kf == 0 || 2 * kf == npr)) break;
+        const double cj = fam ? -1.0 : 1.0;
+        for (int m = fam ? npr - kf : kf; m <= L; m += npr) {
+            const double2 ph = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
+            const double anr = xnr, ani = cj * xni, asr = xsr, asi = cj * xsi;
+            const double fnr = anr * ph.x + ani * ph.y, fni = ani * ph.x - anr * ph.y;
+            const double fsr = asr * ph.x + asi * ph.y, fsi = asi * ph.x - asr * ph.y;
+            const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+            const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+            if (eq) {
+                X[be] = fnr;
+                X[be + 1] = fni;
+                X[bo] = fnr;
+                X[bo + 1] = fni;
+            } else {
+                X[be] = fnr + fsr;
+                X[be + 1] = fni + fsi;
+                X[bo] = fnr - fsr;
+                X[bo + 1] = fni - fsi;
+            }
+        }
+    }
+}
+
+// Equatorial ring pairs (npr = 4 nside, a power of two): real FFT of length npr
+// through a complex radix-2 FFT of length npr/2 in LDS, 2*TQ sequences per block.
+__global__ __launch_bounds__(256) void k_phi_fft(const double* __restrict__ delta, int64_t npix, int32_t nq,
+                                                 int64_t M2, int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                 const double2* __restrict__ tw, double* __restrict__ X, int32_t TQ,
+                                                 int32_t k0, int32_t logM)
+{
+    extern __shared__ double2 zs[];
+    const int k = k0 + blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev R = rings[k];
+    const int N = R.npr, M = N >> 1;
+    const bool eq = (k == K - 1);
+    const double2* twr = tw + R.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
+    const int nf = 2 * TQ;
+    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+        const int f = e / M, jj = e - f * M;
+        const int ql = f >> 1, side = f & 1;
+        const int q = q0 + ql;
+        double re = 0.0, im = 0.0;
+        if (q < nq && !(side && eq)) {
+            const int64_t base = (int64_t)q * npix + (side ? R.start_s : R.start_n) + 2 * jj;
+            re = delta[base];
+            im = delta[base + 1];
+        }
+        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+        zs[f * M + rj] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int st = 1; st <= logM; ++st) {
+        const int half = 1 << (st - 1);
+        const int tstep = N / half;
+        for (int e = threadIdx.x; e < nf * (M >> 1); e += blockDim.x) {
+            const int f = e / (M >> 1), b = e - f * (M >> 1);
+            const int grp = b / half, pos = b - grp * half;
+            const int i0 = f * M + grp * 2 * half + pos, i1 = i0 + half;
+            const double2 w = twr[pos * tstep];
+            const double2 a = zs[i0], c = zs[i1];
+            const double cr = c.x * w.x + c.y * w.y;
+            const double ci = c.y * w.x - c.x * w.y;
+            zs[i0] = make_double2(a.x + cr, a.y + ci);
+            zs[i1] = make_double2(a.x - cr, a.y - ci);
+        }
+        __syncthreads();
+    }
+    // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M
+    const int nm = min(L, M);
+    for (int e = threadIdx.x; e < TQ * (nm + 1); e += blockDim.x) {
+        const int ql = e / (nm + 1), m = e - ql * (nm + 1);
+        const int q = q0 + ql;
+        if (q >= nq) continue;
+        double xr[2], xi[2];
+        for (int side = 0; side < 2; ++side) {
+            const double2* z = zs + (2 * ql + side) * M;
+            const double2 zm = z[m & (M - 1)];
+            const double2 zc = z[(M - m) & (M - 1)];
+            const double er = 0.5 * (zm.x + zc.x), ei = 0.5 * (zm.y - zc.y);
+            const double orr = 0.5 * (zm.y + zc.y), oi = -0.5 * (zm.x - zc.x);
+            const double2 w = twr[2 * m];
+            xr[side] = er + (w.x * orr + w.y * oi);
+            xi[side] = ei + (w.x * oi - w.y * orr);
+        }
+        store_folded(X, M2, K, k, eq, N, R.shift, twr, L, m, q, xr[0], xi[0], xr[1], xi[1]);
+    }
+}
+
+// Polar ring pairs (npr = 4i < 4 nside): direct DFT of the min(L, npr/2)+1
+// distinct coefficients, two maps and both rings per work item.
+__global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ delta, int64_t npix, int32_t nq,
+                                                   int64_t M2, int32_t K, int32_t L,
+                                                   const RingDev* __restrict__ rings, const double2* __restrict__ tw,
+                                                   double* __restrict__ X, int32_t TQ)
+{
+    extern __shared__ double2 ps[];
+    const int k = blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev R = rings[k];
+    const int npr = R.npr, n2 = 2 * npr, ld = npr + 1;
+    double2* twl = ps;                          // 2 npr twiddles
+    double* dN = (double*)(ps + n2);            // [TQ][ld]
+    double* dS = dN + TQ * ld;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) twl[e] = tw[R.tw_off + e];
+    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
+        const int ql = e / npr, j = e - ql * npr;
+        const int q = q0 + ql;
+        double vn = 0.0, vs = 0.0;
+        if (q < nq) {
+            vn = delta[(int64_t)q * npix + R.start_n + j];
+            vs = delta[(int64_t)q * npix + R.start_s + j];
+        }
+        dN[ql * ld + j] = vn;
+        dS[ql * ld + j] = vs;
+    }
+    __syncthreads();
+    const int nu = min(L, npr / 2) + 1;
+    const int G = TQ / 2;
+    for (int e = threadIdx.x; e < nu * G; e += blockDim.x) {
+        const int g = e / nu, kf = e - g * nu;
+        const int qa = 2 * g;
+        double a0r = 0, a0i = 0, b0r = 0, b0i = 0, a1r = 0, a1i = 0, b1r = 0, b1i = 0;
+        const double* n0 = dN + qa * ld;
+        const double* n1 = n0 + ld;
+        const double* s0 = dS + qa * ld;
+        const double* s1 = s0 + ld;
+        int idx = 0;
+        const int didx = (2 * kf) % n2;
+        for (int j = 0; j < npr; ++j) {
+            const double2 w = twl[idx];
+            const double c = w.x, sn = w.y;
+            const double x0 = n0[j], x1 = n1[j], y0 = s0[j], y1 = s1[j];
+            a0r = fma(x0, c, a0r);
+            a0i = fma(-x0, sn, a0i);
+            a1r = fma(x1, c, a1r);
+            a1i = fma(-x1, sn, a1i);
+            b0r = fma(y0, c, b0r);
+            b0i = fma(-y0, sn, b0i);
+            b1r = fma(y1, c, b1r);
+            b1i = fma(-y1, sn, b1i);
+            idx += didx;
+            if (idx >= n2) idx -= n2;
+        }
+        if (q0 + qa < nq)
+            store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa, a0r, a0i, b0r, b0i);
+        if (q0 + qa + 1 < nq)
+            store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa + 1, a1r, a1i, b1r, b1i);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta C[n][i]
 // for rows i < rows.  Block tile 64(i) x 64(n), k-tile 16, 4 waves of 32 x 32.
@@ -355,24 +659,26 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
                 int n = n0 + wn + tn * 16 + (lane >> 4) + 4 * v;
                 int i = i0 + wi + ti * 16 + (lane & 15);
                 if (n < d.N && i < rows) {
-                    double* c = d.C + (int64_t)n * d.ldc + i;
+                    const int64_t o = (int64_t)n * d.ldc + i;
                     double val = alpha * acc[tn][ti][v];
-                    if (beta != 0.0) val = val + beta * *c;
-                    *c = val;
+                    if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
+                    d.C[o] = val;
                 }
             }
 }
 
 // ---------------------------------------------------------------------------
-// Residual in ring-Fourier space.  For a ring with npr pixels and phase
-// phi_0 = shift*pi/npr, the DFT of the synthesised ring is
+// Aliasing part of alm -> map -> alm on the short rings.  For a ring with npr
+// pixels and phase phi_0 = shift*pi/npr, the DFT of the synthesised ring is
 //   FS_m = npr * sum_{m' = m mod npr, |m'| <= lmax} G_{m'} e^{i (m'-m) phi_0},
-// G_{-m} = conj(G_m), and e^{i t npr phi_0} = (-1)^{t shift}.
+// G_{-m} = conj(G_m), e^{i t npr phi_0} = (-1)^{t shift}.  The m' = m term is
+// carried by the dense per-m matrix C0; this kernel writes minus the m' != m
+// terms in E/O form for the first K_al ring pairs (the rings with npr <= 2 lmax).
 // G holds the parity halves Ge, Go per ring pair: north = Ge + Go, south = Ge - Go.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ X0, const double* __restrict__ G,
-                                                  double* __restrict__ R, const RingDev* __restrict__ rings,
-                                                  int32_t K, int32_t L, int64_t M2, int32_t nq)
+__global__ __launch_bounds__(256) void k_alias(const double* __restrict__ G, double* __restrict__ R,
+                                               const RingDev* __restrict__ rings, int32_t K, int32_t K_al, int32_t L,
+                                               int64_t M2, int32_t nq)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -384,14 +690,15 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ X0,
     double snr = 0, sni = 0, ssr = 0, ssi = 0;
     const int mstart = m - ((m + L) / npr) * npr;
     for (int mp = mstart; mp <= L; mp += npr) {
+        if (mp == m) continue;
         const int t = (mp - m) / npr;
         const double sg = ((t * Rg.shift) & 1) ? -1.0 : 1.0;
         const int ma = mp < 0 ? -mp : mp;
         const double cj = mp < 0 ? -1.0 : 1.0;
-        const double* ge = G + ((int64_t)(ma * 2 + 0) * K + k) * M2 + 2 * q;
+        const double* ge = G + ((int64_t)(ma * 2 + 0) * K_al + k) * M2 + 2 * q;
         double er = ge[0], ei = ge[1], orr = 0.0, oi = 0.0;
         if (ma < L) {  // the odd half of m' = lmax is empty
-            const double* go = G + ((int64_t)(ma * 2 + 1) * K + k) * M2 + 2 * q;
+            const double* go = G + ((int64_t)(ma * 2 + 1) * K_al + k) * M2 + 2 * q;
             orr = go[0];
             oi = go[1];
         }
@@ -404,19 +711,18 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ X0,
     sni *= npr;
     ssr *= npr;
     ssi *= npr;
-    const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
-    const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+    const int64_t be = ((int64_t)(m * 2 + 0) * K_al + k) * M2 + 2 * q;
+    const int64_t bo = ((int64_t)(m * 2 + 1) * K_al + k) * M2 + 2 * q;
     if (eq) {
-        double rr = X0[be] - snr, ri = X0[be + 1] - sni;
-        R[be] = rr;
-        R[be + 1] = ri;
-        R[bo] = rr;
-        R[bo + 1] = ri;
+        R[be] = -snr;
+        R[be + 1] = -sni;
+        R[bo] = -snr;
+        R[bo + 1] = -sni;
     } else {
-        R[be] = X0[be] - (snr + ssr);
-        R[be + 1] = X0[be + 1] - (sni + ssi);
-        R[bo] = X0[bo] - (snr - ssr);
-        R[bo + 1] = X0[bo + 1] - (sni - ssi);
+        R[be] = -(snr + ssr);
+        R[be + 1] = -(sni + ssi);
+        R[bo] = -(snr - ssr);
+        R[bo + 1] = -(sni - ssi);
     }
 }
 
@@ -500,19 +806,27 @@ struct sfb_plan {
     int32_t K = 0;       // ring pairs (northern rings incl. equator)
     int64_t M2 = 0;      // 2 * nmaps
     int32_t niter = 3, precision = 0, TQ = 8;
+    int32_t TQF = 1, TQP = 2, logM = 1;  // phi-transform tiles (FFT / polar)
     std::vector<MapInfo> maps;
     std::vector<int64_t> out_base, aoff, lam_off;
     std::vector<int32_t> nb;  // per (m, par): number of l
     int64_t a_size = 0, x_size = 0, lam_size = 0;
     // tables
+    DevBuf d_rec;  // per knot (x, y, ypp, a) for the LDS scatter
+    int32_t tbl_cap = 0;
+    int64_t lds_max = 65536;
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
     // work
-    DevBuf d_delta, d_X0, d_G, d_R, d_a;
+    DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
+    int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
+    std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
+    double* a_final = nullptr;     // the a buffer holding the last result
     // catalogue
     DevBuf d_keyr, d_keyr2, d_idx, d_idx2, d_pix, d_pixkey, d_pixkey2, d_rs, d_ws, d_pstart, d_sorttmp;
     // host-API staging
     DevBuf h_rtp, h_w, h_wa, h_wb, h_out;
     int64_t cat_cap = -1;
+    int64_t ncat = 0;
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[SFB_NPHASE];
@@ -609,7 +923,28 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         base += (lmax_n[n - 1] + 1) * (lmax_n[n - 1] + 2) / 2;
     }
     P->nlmsize = base;
+    std::vector<double> rec;
+    rec.reserve((size_t)knot_off[P->nmaps] * 4);
+    P->tbl_cap = 0;
+    for (int64_t qq = 0; qq < P->nmaps; ++qq) {
+        int64_t o = knot_off[qq], nk = knot_off[qq + 1] - knot_off[qq];
+        P->tbl_cap = std::max<int32_t>(P->tbl_cap, (int32_t)nk);
+        for (int64_t i = 0; i < nk; ++i) {
+            double a = 0.0;
+            if (i + 1 < nk) {
+                double h = kx[o + i + 1] - kx[o + i];
+                double h2 = h * h;
+                a = h2 * (kypp[o + i + 1] - kypp[o + i]) / 6.0;
+            }
+            rec.push_back(kx[o + i]);
+            rec.push_back(ky[o + i]);
+            rec.push_back(kypp[o + i]);
+            rec.push_back(a);
+        }
+    }
     int rc;
+    if ((rc = P->d_rec.ensure(sizeof(double) * rec.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_rec.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_maps.ensure(sizeof(MapInfo) * P->maps.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_maps.p, P->maps.data(), sizeof(MapInfo) * P->maps.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_kx.ensure(sizeof(double) * (size_t)knot_off[P->nmaps]))) return rc;
@@ -681,49 +1016,117 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     if ((rc = P->d_aoff.ensure(sizeof(int64_t) * P->aoff.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_aoff.p, P->aoff.data(), sizeof(int64_t) * P->aoff.size(), hipMemcpyHostToDevice));
 
+    // ring pairs that alias: npr <= 2 lmax (they are the first K_al pairs)
+    P->K_al = 0;
+    for (int32_t k = 0; k < P->K; ++k)
+        if (rd[(size_t)k].npr <= 2 * L) P->K_al = k + 1;
+    // I - C0 per (m, parity): C0[j][j'] = dOmega sum_rings npr lambda_j lambda_j'
+    // (north and south rings give the same product within one parity)
+    const double domega = 4.0 * 3.141592653589793 / (double)P->npix;
+    P->ic0_off.assign((size_t)(2 * (L + 1) + 1), 0);
+    for (size_t b = 0; b < (size_t)(2 * (L + 1)); ++b)
+        P->ic0_off[b + 1] = P->ic0_off[b] + (int64_t)P->nb[b] * P->nb[b];
+    std::vector<double> ic0((size_t)P->ic0_off.back());
+    {
+        std::vector<double> wk((size_t)P->K);
+        for (int32_t k = 0; k < P->K; ++k) wk[(size_t)k] = domega * rd[(size_t)k].npr * ((k == P->K - 1) ? 1.0 : 2.0);
+        for (size_t b = 0; b < (size_t)(2 * (L + 1)); ++b) {
+            const int64_t n = P->nb[b];
+            const double* lb = lam.data() + P->lam_off[b];
+            double* out = ic0.data() + P->ic0_off[b];
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t jp = j; jp < n; ++jp) {
+                    long double acc = 0.0L;
+                    for (int32_t k = 0; k < P->K; ++k)
+                        acc += (long double)wk[(size_t)k] * lb[(size_t)(k * n + j)] * lb[(size_t)(k * n + jp)];
+                    double v = (j == jp ? 1.0 : 0.0) - (double)acc;
+                    out[j * n + jp] = v;
+                    out[jp * n + j] = v;
+                }
+        }
+    }
+    if ((rc = P->d_ic0.ensure(sizeof(double) * std::max<size_t>(1, ic0.size())))) return rc;
+    if (!ic0.empty()) HIP_TRY(hipMemcpy(P->d_ic0.p, ic0.data(), sizeof(double) * ic0.size(), hipMemcpyHostToDevice));
+
     // work buffers
     P->x_size = (L + 1) * 2 * (int64_t)P->K * P->M2;
+    const int64_t xal = (L + 1) * 2 * (int64_t)std::max(P->K_al, 1) * P->M2;
     if ((rc = P->d_delta.ensure(sizeof(double) * (size_t)(P->nmaps * P->npix)))) return rc;
     if ((rc = P->d_X0.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
-    if ((rc = P->d_G.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
-    if ((rc = P->d_R.ensure(sizeof(double) * (size_t)P->x_size))) return rc;
+    if ((rc = P->d_G.ensure(sizeof(double) * (size_t)xal))) return rc;
+    if ((rc = P->d_R.ensure(sizeof(double) * (size_t)xal))) return rc;
     if ((rc = P->d_a.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
+    if ((rc = P->d_a1.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
+    if ((rc = P->d_a2.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
     if ((rc = P->d_flag.ensure(2 * sizeof(int32_t)))) return rc;
 
-    // GEMM descriptors: analysis A = X (k = ring pair), B = Lam, C = a;
-    // synthesis A = a (k = l index), B = LamT, C = G.
-    std::vector<GemmDesc> ana, syn;
-    for (int64_t b = 0; b < 2 * (L + 1); ++b) {
+    // GEMM descriptors.  First analysis: A = X0 (k = ring pair), B = Lam, C = b (d_a).
+    // Iteration slot s reads a_in and writes a_out:
+    //   syn: G_al = a_in LamT (first K_al ring pairs)       C0: a_out = a_in (I - C0) + b
+    //   al : a_out += dOmega R_al Lam[first K_al ring pairs]
+    const int64_t nbat = 2 * (L + 1);
+    std::vector<GemmDesc> ana, syn, c0, al;
+    double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
+    double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
+    for (int64_t b = 0; b < nbat; ++b) {
         int32_t n = P->nb[(size_t)b];
-        GemmDesc d;
+        GemmDesc d{};
         d.A = P->d_X0.as<double>() + b * (int64_t)P->K * P->M2;
         d.B = P->d_lam.as<double>() + P->lam_off[(size_t)b];
         d.C = P->d_a.as<double>() + P->aoff[(size_t)b];
+        d.Cin = nullptr;
         d.K = P->K;
         d.N = n;
         d.lda = P->M2;
         d.ldb = n;
         d.ldc = P->M2;
         ana.push_back(d);
-        GemmDesc s;
-        s.A = P->d_a.as<double>() + P->aoff[(size_t)b];
-        s.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
-        s.C = P->d_G.as<double>() + b * (int64_t)P->K * P->M2;
-        s.K = n;
-        s.N = n > 0 ? P->K : 0;
-        s.lda = P->M2;
-        s.ldb = P->K;
-        s.ldc = P->M2;
-        syn.push_back(s);
     }
-    // the residual-driven analyses read R instead of X0: a second descriptor set
-    std::vector<GemmDesc> anaR = ana;
-    for (int64_t b = 0; b < 2 * (L + 1); ++b) anaR[(size_t)b].A = P->d_R.as<double>() + b * (int64_t)P->K * P->M2;
-    ana.insert(ana.end(), anaR.begin(), anaR.end());
+    for (int slot = 0; slot < 3; ++slot)
+        for (int64_t b = 0; b < nbat; ++b) {
+            int32_t n = P->nb[(size_t)b];
+            GemmDesc sd{};
+            sd.A = a_in_slot[slot] + P->aoff[(size_t)b];
+            sd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
+            sd.C = P->d_G.as<double>() + b * (int64_t)P->K_al * P->M2;
+            sd.Cin = nullptr;
+            sd.K = n;
+            sd.N = (n > 0) ? P->K_al : 0;
+            sd.lda = P->M2;
+            sd.ldb = P->K;
+            sd.ldc = P->M2;
+            syn.push_back(sd);
+            GemmDesc cd{};
+            cd.A = a_in_slot[slot] + P->aoff[(size_t)b];
+            cd.B = P->d_ic0.as<double>() + P->ic0_off[(size_t)b];
+            cd.C = a_out_slot[slot] + P->aoff[(size_t)b];
+            cd.Cin = P->d_a.as<double>() + P->aoff[(size_t)b];
+            cd.K = n;
+            cd.N = n;
+            cd.lda = P->M2;
+            cd.ldb = n;
+            cd.ldc = P->M2;
+            c0.push_back(cd);
+            GemmDesc ad{};
+            ad.A = P->d_R.as<double>() + b * (int64_t)P->K_al * P->M2;
+            ad.B = P->d_lam.as<double>() + P->lam_off[(size_t)b];
+            ad.C = a_out_slot[slot] + P->aoff[(size_t)b];
+            ad.Cin = nullptr;
+            ad.K = P->K_al;
+            ad.N = (P->K_al > 0) ? n : 0;
+            ad.lda = P->M2;
+            ad.ldb = n;
+            ad.ldc = P->M2;
+            al.push_back(ad);
+        }
     if ((rc = P->d_desc_ana.ensure(sizeof(GemmDesc) * ana.size()))) return rc;
     if ((rc = P->d_desc_syn.ensure(sizeof(GemmDesc) * syn.size()))) return rc;
+    if ((rc = P->d_desc_c0.ensure(sizeof(GemmDesc) * c0.size()))) return rc;
+    if ((rc = P->d_desc_al.ensure(sizeof(GemmDesc) * al.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_ana.p, ana.data(), sizeof(GemmDesc) * ana.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->d_desc_syn.p, syn.data(), sizeof(GemmDesc) * syn.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_desc_c0.p, c0.data(), sizeof(GemmDesc) * c0.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_desc_al.p, al.data(), sizeof(GemmDesc) * al.size(), hipMemcpyHostToDevice));
     return SFB_OK;
 }
 
@@ -766,6 +1169,21 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
     P->niter = niter;
     P->precision = precision;
     P->TQ = (int32_t)std::max<int64_t>(1, std::min<int64_t>(8, 65536 / (16 * (4 * nside + 1))));
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+            P->lds_max = lds;
+    }
+    {
+        const int64_t N4 = 4 * nside, M = 2 * nside;
+        P->logM = 0;
+        while ((1LL << P->logM) < M) ++P->logM;
+        const int64_t budget = std::min<int64_t>(P->lds_max, 98304);
+        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, budget / (2 * M * 16)));
+        // polar: 2 npr twiddles + 2 rings x TQP maps x (npr+1) doubles, npr <= 4 nside - 4
+        const int64_t per = 2 * (N4 + 1) * 8;
+        P->TQP = (int32_t)std::max<int64_t>(2, std::min<int64_t>(16, (budget - 2 * N4 * 16) / per) & ~1LL);
+    }
     hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete P;
@@ -786,7 +1204,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     prof_collect(P);
-    DevBuf* bufs[] = {&P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
+    DevBuf* bufs[] = {&P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
@@ -820,6 +1238,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
 {
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
+    P->ncat = N;
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
     LAUNCH_CHECK();
@@ -872,8 +1291,36 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 {
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
-    dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
     PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
+    const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
+    const int64_t wg_room = (P->lds_max - tbl_bytes) / 8;
+    if (wg_room >= 1024) {
+        const int32_t WG = (int32_t)std::min<int64_t>(8192, wg_room);
+        const int64_t per_pix = std::max<int64_t>(1, (P->ncat + P->npix - 1) / P->npix);
+        const int32_t PT = (int32_t)std::max<int64_t>(32, std::min<int64_t>(1024, WG / (2 * per_pix)));
+        const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(8, P->npix / PT));
+        const int64_t ppc = (P->npix + nchunk - 1) / nchunk;
+        const size_t shm = (size_t)tbl_bytes + 8 * (size_t)WG;
+        dim3 g2((unsigned)P->nmaps, (unsigned)nchunk);
+        if (win_kind == SFB_WIN_DENSE) {
+            HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            hipLaunchKernelGGL(k_scatter_lds<true>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
+                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(),
+                               P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
+                               P->d_delta.as<double>());
+        } else {
+            HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            hipLaunchKernelGGL(k_scatter_lds<false>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
+                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(),
+                               P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
+                               P->d_delta.as<double>());
+        }
+        LAUNCH_CHECK();
+        return SFB_OK;
+    }
+    dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
     if (win_kind == SFB_WIN_DENSE)
         hipLaunchKernelGGL(k_scatter<true>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
                            P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
@@ -886,6 +1333,31 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     return SFB_OK;
 }
 
+// delta (first nq maps) -> X0: FFT for the equatorial ring pairs, direct DFT for the polar ones.
+static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st)
+{
+    const int32_t K = P->K, L = (int32_t)P->lmax;
+    const int64_t nside = P->nside;
+    const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
+    const int64_t M = 2 * nside;
+    size_t shf = sizeof(double2) * 2 * (size_t)P->TQF * (size_t)M;
+    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
+    hipLaunchKernelGGL(k_phi_fft, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
+                       P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                       P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQF, k_eq0, P->logM);
+    LAUNCH_CHECK();
+    if (k_eq0 > 0) {
+        const int64_t npr_max = 4 * (nside - 1);
+        size_t shp = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
+        hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), shp, st,
+                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP);
+        LAUNCH_CHECK();
+    }
+    return SFB_OK;
+}
+
 // delta (first nq maps) -> a, with niter refinements.
 static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
 {
@@ -895,41 +1367,52 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
     {
         PhaseTimer timer(P, SFB_PHASE_PHI, st);
-        hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)K, nblk(nq, P->TQ)), dim3(256), shm, st,
-                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQ);
-        LAUNCH_CHECK();
+        (void)shm;
+        int rc = phi_transform(P, nq, st);
+        if (rc) return rc;
     }
     const unsigned nbat = (unsigned)(2 * (L + 1));
     const GemmDesc* ana = P->d_desc_ana.as<GemmDesc>();
-    const GemmDesc* anaR = ana + nbat;
-    const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>();
     const int32_t maxnb = P->nb[0];
+    const int32_t Kal = P->K_al;
     dim3 ga(nblk(rows, GBM), nblk(maxnb, GBN), nbat);
     {
         PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
         hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, ana, rows, domega, 0.0);
         LAUNCH_CHECK();
     }
-    dim3 gs(nblk(rows, GBM), nblk(K, GBN), nbat);
-    dim3 gr(nblk(nq, 256), (unsigned)K, (unsigned)(L + 1));
+    P->a_final = P->d_a.as<double>();
+    dim3 gs(nblk(rows, GBM), nblk(std::max(Kal, 1), GBN), nbat);
+    dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
     for (int it = 0; it < P->niter; ++it) {
+        const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
+        const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>() + slot * nbat;
+        const GemmDesc* c0 = P->d_desc_c0.as<GemmDesc>() + slot * nbat;
+        const GemmDesc* al = P->d_desc_al.as<GemmDesc>() + slot * nbat;
+        if (Kal > 0) {
+            {
+                PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
+                hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
+                LAUNCH_CHECK();
+            }
+            {
+                PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
+                hipLaunchKernelGGL(k_alias, gr, dim3(256), 0, st, P->d_G.as<double>(), P->d_R.as<double>(),
+                                   P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq);
+                LAUNCH_CHECK();
+            }
+        }
         {
             PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-            hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
+            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, c0, rows, 1.0, 1.0);
             LAUNCH_CHECK();
         }
-        {
-            PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
-            hipLaunchKernelGGL(k_residual, gr, dim3(256), 0, st, P->d_X0.as<double>(), P->d_G.as<double>(),
-                               P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, L, P->M2, nq);
-            LAUNCH_CHECK();
-        }
-        {
+        if (Kal > 0) {
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, anaR, rows, domega, 1.0);
+            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, al, rows, domega, 1.0);
             LAUNCH_CHECK();
         }
+        P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
     }
     return SFB_OK;
 }
@@ -960,7 +1443,7 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     {
         PhaseTimer timer(P, SFB_PHASE_PACK, st);
         hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
-                           P->d_a.as<double>(), P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
+                           P->a_final, P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
                            P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
         LAUNCH_CHECK();
     }
@@ -1057,7 +1540,7 @@ extern "C" int sfb_map2alm(sfb_plan* P, int64_t nmap, const double* maps, double
     int64_t nalm = (P->lmax + 1) * (P->lmax + 2) / 2;
     if ((rc = P->h_out.ensure(sizeof(double) * 2 * (size_t)(nalm * nmap)))) return rc;
     hipLaunchKernelGGL(k_pack_alm, dim3(nblk(nmap, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, P->stream,
-                       P->d_a.as<double>(), P->d_aoff.as<int64_t>(), (int32_t)nmap, (int32_t)P->lmax, P->M2,
+                       P->a_final, P->d_aoff.as<int64_t>(), (int32_t)nmap, (int32_t)P->lmax, P->M2,
                        P->h_out.as<double>());
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpyAsync(alm_out, P->h_out.p, sizeof(double) * 2 * (size_t)(nalm * nmap), hipMemcpyDeviceToHost,
@@ -1111,6 +1594,7 @@ extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, 
     d.A = dA.as<double>();
     d.B = dB.as<double>();
     d.C = dC.as<double>();
+    d.Cin = nullptr;
     d.K = (int32_t)K;
     d.N = (int32_t)N;
     d.lda = rows;
@@ -1139,11 +1623,8 @@ extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, 
     HIP_TRY(hipSetDevice(P->device));
     HIP_TRY(hipMemcpyAsync(P->d_delta.p, maps, sizeof(double) * (size_t)(nmap * P->npix), hipMemcpyHostToDevice,
                            P->stream));
-    size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
-    hipLaunchKernelGGL(k_phi_dft, dim3((unsigned)P->K, nblk(nmap, P->TQ)), dim3(256), shm, P->stream,
-                       P->d_delta.as<double>(), P->npix, (int32_t)nmap, P->M2, P->K, (int32_t)P->lmax,
-                       P->d_rings.as<RingDev>(), P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQ);
-    LAUNCH_CHECK();
+    int rc = phi_transform(P, (int32_t)nmap, P->stream);
+    if (rc) return rc;
     // out[(b * K + k) * 2 nmap + i]
     HIP_TRY(hipMemcpy2DAsync(x_out, sizeof(double) * 2 * (size_t)nmap, P->d_X0.p, sizeof(double) * (size_t)P->M2,
                              sizeof(double) * 2 * (size_t)nmap, (size_t)(2 * (P->lmax + 1) * P->K),
