@@ -168,6 +168,48 @@ __global__ void k_gather(int64_t N, const uint32_t* __restrict__ idx, const doub
     w_s[k] = w ? w[i] : 1.0;
 }
 
+// Scatter tile of pixel p: chunk c = p / ppc, tile t = (p - c ppc) / PT inside it.
+__device__ __forceinline__ uint32_t tile_of(int64_t p, int64_t ppc, int32_t PT, int64_t tpc)
+{
+    const int64_t c = p / ppc;
+    return (uint32_t)(c * tpc + (p - c * ppc) / PT);
+}
+
+// key[k] = scatter tile of the k-th galaxy in r order.
+__global__ void k_tile_keys(int64_t N, const uint32_t* __restrict__ idx_r, const uint32_t* __restrict__ pix,
+                            int64_t ppc, int32_t PT, int64_t tpc, uint32_t* __restrict__ key)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    key[k] = tile_of(pix[idx_r[k]], ppc, PT, tpc);
+}
+
+// posp[idx_pix[k]] = k: position of every galaxy in pixel order.
+__global__ void k_inverse(int64_t N, const uint32_t* __restrict__ idx_pix, uint32_t* __restrict__ posp)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    posp[idx_pix[k]] = (uint32_t)k;
+}
+
+// Galaxies in (tile, r) order with their slot in the tile's pixel order.
+__global__ void k_gather_tile(int64_t N, const uint32_t* __restrict__ idx_t, const uint32_t* __restrict__ key_t,
+                              const uint32_t* __restrict__ posp, const int32_t* __restrict__ pstart,
+                              const double* __restrict__ rtp, const double* __restrict__ w, int64_t ppc, int32_t PT,
+                              int64_t tpc, double* __restrict__ rt, double* __restrict__ wt,
+                              int32_t* __restrict__ lpos)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const uint32_t i = idx_t[k];
+    const uint32_t T = key_t[k];
+    const int64_t c = T / tpc, t = T - c * tpc;
+    const int64_t p0 = c * ppc + t * PT;  // first pixel of the tile
+    rt[k] = rtp[3 * (int64_t)i];
+    wt[k] = w ? w[i] : 1.0;
+    lpos[k] = (int32_t)posp[i] - pstart[p0];
+}
+
 // pstart[p] = first sorted galaxy with pixel >= p, p = 0..npix.
 __global__ void k_pixel_start(int64_t npix, int64_t N, const uint32_t* __restrict__ pix_sorted,
                               int32_t* __restrict__ pstart)
@@ -309,65 +351,82 @@ __device__ __forceinline__ void spline_lds4(const MapInfo& mi, const double4* __
     }
 }
 
-// delta maps with the map's spline table in LDS.  Block (q, chunk): the table of
-// map q, then pixel tiles of PT pixels; the tile's galaxies are evaluated one per
-// thread into wg[], then every pixel sums its own galaxies in order.
+// Orders the LDS accesses of one wavefront: every lane's earlier LDS writes are
+// visible to every lane's later LDS reads.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// delta maps with the map's spline table in LDS.  Block (q, chunk) stages the
+// table of map q; then every wavefront independently takes pixel tiles of PT
+// pixels: it evaluates the tile's galaxies (in r order, one per lane) into its own
+// wg[] slice at their pixel-order slots, and every pixel then sums its own
+// galaxies in the reference order.  No block-wide barrier after the staging, so
+// the 16 waves of a CU hide each other's memory latency.  A tile with more than
+// WG galaxies is summed with per-pixel evaluation instead.
 template <bool DENSE>
 __global__ __launch_bounds__(1024) void k_scatter_lds(int64_t npix, int64_t pix_per_chunk, int32_t PT, int32_t WG,
-                                                     int32_t tbl_cap, const int32_t* __restrict__ pstart,
-                                                     const double* __restrict__ r_s, const double* __restrict__ w_s,
-                                                     const MapInfo* __restrict__ maps, const double4* __restrict__ rec,
-                                                     double inv, const double* __restrict__ win_a,
-                                                     const double* __restrict__ win_b, int64_t lmax,
-                                                     double* __restrict__ delta)
+                                                      int32_t tbl_cap, const int32_t* __restrict__ pstart,
+                                                      const double* __restrict__ r_s, const double* __restrict__ w_s,
+                                                      const double* __restrict__ rt, const double* __restrict__ wt,
+                                                      const int32_t* __restrict__ lpos,
+                                                      const MapInfo* __restrict__ maps,
+                                                      const double4* __restrict__ rec, double inv,
+                                                      const double* __restrict__ win_a,
+                                                      const double* __restrict__ win_b, int64_t lmax,
+                                                      double* __restrict__ delta)
 {
     extern __shared__ double4 sm4[];
     const int q = blockIdx.x;
     const MapInfo mi = maps[q];
     double4* tb = sm4;
-    double* wg = (double*)(sm4 + tbl_cap);
     for (int k = threadIdx.x; k < mi.nk; k += blockDim.x) tb[k] = rec[mi.koff + k];
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    double* wg = (double*)(sm4 + tbl_cap) + (int64_t)wave * WG;
     const int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
     const double wcol = DENSE ? 0.0 : win_b[col];
     const int64_t pb = (int64_t)blockIdx.y * pix_per_chunk;
     const int64_t pe = min(npix, pb + pix_per_chunk);
-    for (int64_t p0 = pb; p0 < pe; p0 += PT) {
+    const int ntile = (int)((pe - pb + PT - 1) / PT);
+    __syncthreads();  // table staged
+    for (int t = wave; t < ntile; t += nwave) {
+        const int64_t p0 = pb + (int64_t)t * PT;
         const int64_t p1 = min(p0 + (int64_t)PT, pe);
-        const int32_t G0 = pstart[p0], G1 = pstart[p1];
-        const int64_t p = p0 + threadIdx.x;
-        const bool own = (int)threadIdx.x < PT && p < p1;
-        double s = 0.0;
-        if (G1 - G0 <= WG) {
-            const int ng = G1 - G0;
-            const int stride = blockDim.x;
-            for (int j0 = threadIdx.x; j0 < ng; j0 += 4 * stride) {
+        const int32_t G0 = pstart[p0], ng = pstart[p1] - G0;
+        const bool coop = ng <= WG;
+        if (coop) {
+            for (int j0 = lane; j0 < ng; j0 += 4 * 64) {
                 double rr[4], ww[4], vv[4];
+                int32_t lp[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int j = min(j0 + u * stride, ng - 1);
-                    rr[u] = r_s[G0 + j];
-                    ww[u] = w_s[G0 + j];
+                    const int j = min(j0 + u * 64, ng - 1);
+                    rr[u] = rt[G0 + j];
+                    ww[u] = wt[G0 + j];
+                    lp[u] = lpos[G0 + j];
                 }
                 spline_lds4(mi, tb, rr, vv);
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (j0 + u * stride < ng) wg[j0 + u * stride] = __dmul_rn(ww[u], vv[u]);
+                    if (j0 + u * 64 < ng) wg[lp[u]] = __dmul_rn(ww[u], vv[u]);
             }
-            __syncthreads();
-            if (own) {
-                const int32_t a = pstart[p] - G0, b = pstart[p + 1] - G0;
-                for (int32_t j = a; j < b; ++j) s = __dadd_rn(s, wg[j]);
-            }
-            __syncthreads();
-        } else if (own) {
-            const int32_t b = pstart[p + 1];
-            for (int32_t g = pstart[p]; g < b; ++g) s = __dadd_rn(s, __dmul_rn(w_s[g], spline_lds(mi, tb, r_s[g])));
+            wave_lds_sync();
         }
-        if (own) {
+        for (int64_t p = p0 + lane; p < p1; p += 64) {
+            const int32_t a = pstart[p], b = pstart[p + 1];
+            double s = 0.0;
+            if (coop) {
+                for (int32_t j = a - G0; j < b - G0; ++j) s = __dadd_rn(s, wg[j]);
+            } else {
+                for (int32_t g = a; g < b; ++g) s = __dadd_rn(s, __dmul_rn(w_s[g], spline_lds(mi, tb, r_s[g])));
+            }
             const double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], wcol);
             delta[(int64_t)q * npix + p] = __dsub_rn(__dmul_rn(s, inv), W);
         }
+        wave_lds_sync();  // the slice is rewritten by the next tile
     }
 }
 
@@ -482,8 +541,9 @@ __global__ __launch_bounds__(256) void k_phi_fft(const double* __restrict__ delt
     const RingDev R = rings[k];
     const int N = R.npr, M = N >> 1;
     const bool eq = (k == K - 1);
-    const double2* twr = tw + R.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     const int nf = 2 * TQ;
+    double2* twr = zs + nf * M;  // (cos, sin)(pi kk / N), kk < 2N, staged in LDS
+    for (int e = threadIdx.x; e < 2 * N; e += blockDim.x) twr[e] = tw[R.tw_off + e];
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
         const int f = e / M, jj = e - f * M;
         const int ql = f >> 1, side = f & 1;
@@ -517,7 +577,7 @@ __global__ __launch_bounds__(256) void k_phi_fft(const double* __restrict__ delt
     // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M
     const int nm = min(L, M);
     for (int e = threadIdx.x; e < TQ * (nm + 1); e += blockDim.x) {
-        const int ql = e / (nm + 1), m = e - ql * (nm + 1);
+        const int m = e / TQ, ql = e - m * TQ;  // maps fastest: contiguous X stores
         const int q = q0 + ql;
         if (q >= nq) continue;
         double xr[2], xi[2];
@@ -566,7 +626,7 @@ __global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ de
     const int nu = min(L, npr / 2) + 1;
     const int G = TQ / 2;
     for (int e = threadIdx.x; e < nu * G; e += blockDim.x) {
-        const int g = e / nu, kf = e - g * nu;
+        const int kf = e / G, g = e - kf * G;  // map pairs fastest: contiguous X stores
         const int qa = 2 * g;
         double a0r = 0, a0i = 0, b0r = 0, b0i = 0, a1r = 0, a1i = 0, b1r = 0, b1i = 0;
         const double* n0 = dN + qa * ld;
@@ -665,6 +725,97 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
                     d.C[o] = val;
                 }
             }
+}
+
+// Tall-skinny FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta Cin[n][i],
+// i < rows.  A block owns 64 rows and up to 16*NTM columns, so A (the large
+// operand) is read once per batch; 4 waves of 16 rows x all columns; the next
+// k-tile is prefetched into registers while the current one is on the MFMAs.
+template <int NTM>
+__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+                                                   double beta)
+{
+    constexpr int BM = 64, BK = 16, BNW = 16 * NTM;
+    const GemmDesc d = descs[blockIdx.z];
+    const int i0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BNW;
+    if (n0 >= d.N) return;
+    const int ntl = min(NTM, (d.N - n0 + 15) / 16);
+    __shared__ double As[BK][BM];
+    __shared__ double Bs[BK][BNW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave * 16;
+    double4_t acc[NTM];
+#pragma unroll
+    for (int t = 0; t < NTM; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    double ra[4], rb[NTM];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e >> 6, ii = e & 63;
+            const int gk = k0 + kk, gi = i0 + ii;
+            ra[u] = (gk < d.K && gi < rows) ? d.A[(int64_t)gk * d.lda + gi] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NTM; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e / BNW, nn = e - kk * BNW;
+            const int gk = k0 + kk, gn = n0 + nn;
+            rb[u] = (gk < d.K && gn < d.N) ? d.B[(int64_t)gk * d.ldb + gn] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[e >> 6][e & 63] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NTM; ++u) {
+            const int e = tid + 256 * u;
+            Bs[e / BNW][e % BNW] = rb[u];
+        }
+        __syncthreads();
+        if (k0 + BK < d.K) fetch(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            const double fb = As[kk][wi + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < NTM; ++t)
+                if (t < ntl) {
+                    const double fa = Bs[kk][t * 16 + (lane & 15)];
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fb, acc[t], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    const int i = i0 + wi + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < NTM; ++t)
+        if (t < ntl) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = n0 + t * 16 + (lane >> 4) + 4 * v;
+                if (n < d.N && i < rows) {
+                    const int64_t o = (int64_t)n * d.ldc + i;
+                    double val = alpha * acc[t][v];
+                    if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
+                    d.C[o] = val;
+                }
+            }
+        }
+}
+
+constexpr int TALL_NTM = 8;  // 128 columns per block
+
+static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
+                               double beta, hipStream_t st)
+{
+    dim3 g((unsigned)((rows + 63) / 64), (unsigned)((maxN + 16 * TALL_NTM - 1) / (16 * TALL_NTM)), nbat);
+    hipLaunchKernelGGL(k_gemm_tall<TALL_NTM>, g, dim3(256), 0, st, descs, rows, alpha, beta);
 }
 
 // ---------------------------------------------------------------------------
@@ -827,6 +978,11 @@ struct sfb_plan {
     DevBuf h_rtp, h_w, h_wa, h_wb, h_out;
     int64_t cat_cap = -1;
     int64_t ncat = 0;
+    // scatter tiling of the current call (LDS path)
+    bool sc_lds = false;
+    int32_t sc_WG = 0, sc_PT = 0;
+    int64_t sc_ppc = 0, sc_nchunk = 0, sc_tpc = 0;
+    DevBuf d_rt, d_wt, d_lpos, d_posp, d_keyt, d_keyt2, d_idxt;
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[SFB_NPHASE];
@@ -1179,7 +1335,9 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         P->logM = 0;
         while ((1LL << P->logM) < M) ++P->logM;
         const int64_t budget = std::min<int64_t>(P->lds_max, 98304);
-        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, budget / (2 * M * 16)));
+        // 2 TQF sequences of M complex values plus the 2 N4 twiddles; TQF <= 4 keeps
+        // several blocks resident per CU
+        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, (budget - 2 * N4 * 16) / (2 * M * 16)));
         // polar: 2 npr twiddles + 2 rings x TQP maps x (npr+1) doubles, npr <= 4 nside - 4
         const int64_t per = 2 * (N4 + 1) * 8;
         P->TQP = (int32_t)std::max<int64_t>(2, std::min<int64_t>(16, (budget - 2 * N4 * 16) / per) & ~1LL);
@@ -1204,7 +1362,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     prof_collect(P);
-    DevBuf* bufs[] = {&P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
+    DevBuf* bufs[] = {&P->d_rt, &P->d_wt, &P->d_lpos, &P->d_posp, &P->d_keyt, &P->d_keyt2, &P->d_idxt, &P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
@@ -1233,12 +1391,32 @@ extern "C" int sfb_plan_set_precision(sfb_plan* P, int32_t precision)
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Sort the catalogue and build the pixel segments.  Leaves r_s, w_s, pstart.
+// Tiling of the LDS scatter for a catalogue of N galaxies: sc_lds is false when
+// the largest spline table does not fit next to a 1024-entry galaxy buffer.
+static void scatter_geometry(sfb_plan* P, int64_t N)
+{
+    const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
+    const int64_t wg_room = (P->lds_max - tbl_bytes) / 8;
+    P->sc_lds = wg_room >= 16 * 256;
+    if (!P->sc_lds) return;
+    // 16 wavefronts, each with a slice of WG galaxy slots; a tile is sized so its
+    // expected galaxy count is ~85% of WG
+    P->sc_WG = (int32_t)std::min<int64_t>(1024, wg_room / 16);
+    const double per_pix = std::max(1.0, (double)N / (double)P->npix);
+    P->sc_PT = (int32_t)std::max<int64_t>(8, std::min<int64_t>(1024, (int64_t)(0.85 * P->sc_WG / per_pix)));
+    P->sc_nchunk = std::max<int64_t>(1, std::min<int64_t>(8, P->npix / P->sc_PT));
+    P->sc_ppc = (P->npix + P->sc_nchunk - 1) / P->sc_nchunk;
+    P->sc_tpc = (P->sc_ppc + P->sc_PT - 1) / P->sc_PT;
+}
+
+// Sort the catalogue and build the pixel segments.  Leaves r_s, w_s, pstart and,
+// for the LDS scatter, the (tile, r)-ordered rt, wt, lpos.
 static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, hipStream_t st)
 {
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
     P->ncat = N;
+    scatter_geometry(P, N);
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
     LAUNCH_CHECK();
@@ -1266,7 +1444,18 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     while (((int64_t)1 << pbits) < P->npix) ++pbits;
     HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp2, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
                                       P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
-    if ((rc = P->d_sorttmp.ensure(std::max(tmp1, tmp2)))) return rc;
+    size_t tmp3 = 0;
+    if (P->sc_lds) {
+        const int64_t ntiles = P->sc_nchunk * P->sc_tpc;
+        unsigned tbits = 1;
+        while (((int64_t)1 << tbits) < ntiles) ++tbits;
+        HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp3, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
+                                          P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, tbits,
+                                          st));
+    }
+    // every temporary size is known before the first sort is enqueued: the buffer
+    // is never reallocated under work in flight
+    if ((rc = P->d_sorttmp.ensure(std::max(std::max(tmp1, tmp2), tmp3)))) return rc;
     tmp1 = P->d_sorttmp.bytes;
     HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
                                       P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
@@ -1283,6 +1472,33 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     hipLaunchKernelGGL(k_pixel_start, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->npix, N,
                        P->d_pixkey2.as<uint32_t>(), P->d_pstart.as<int32_t>());
     LAUNCH_CHECK();
+    if (P->sc_lds) {
+        // 3) stable sort of the r-ordered list by scatter tile: (tile, r, index) order
+        if ((rc = P->d_rt.ensure(8 * n)) || (rc = P->d_wt.ensure(8 * n)) || (rc = P->d_lpos.ensure(4 * n)) ||
+            (rc = P->d_posp.ensure(4 * n)) || (rc = P->d_keyt.ensure(4 * n)) || (rc = P->d_keyt2.ensure(4 * n)) ||
+            (rc = P->d_idxt.ensure(4 * n)))
+            return rc;
+        const int64_t ntiles = P->sc_nchunk * P->sc_tpc;
+        unsigned tbits = 1;
+        while (((int64_t)1 << tbits) < ntiles) ++tbits;
+        // sort 1 left the r order in idx2; it was consumed as the input of sort 2, which
+        // does not modify its input buffers, so idx2 still holds the r order
+        hipLaunchKernelGGL(k_tile_keys, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx2.as<uint32_t>(),
+                           P->d_pix.as<uint32_t>(), P->sc_ppc, P->sc_PT, P->sc_tpc, P->d_keyt.as<uint32_t>());
+        LAUNCH_CHECK();
+        tmp3 = P->d_sorttmp.bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp3, P->d_keyt.as<uint32_t>(), P->d_keyt2.as<uint32_t>(),
+                                          P->d_idx2.as<uint32_t>(), P->d_idxt.as<uint32_t>(), (unsigned)N, 0, tbits,
+                                          st));
+        hipLaunchKernelGGL(k_inverse, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx.as<uint32_t>(),
+                           P->d_posp.as<uint32_t>());
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gather_tile, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idxt.as<uint32_t>(),
+                           P->d_keyt2.as<uint32_t>(), P->d_posp.as<uint32_t>(), P->d_pstart.as<int32_t>(), d_rtp, d_w,
+                           P->sc_ppc, P->sc_PT, P->sc_tpc, P->d_rt.as<double>(), P->d_wt.as<double>(),
+                           P->d_lpos.as<int32_t>());
+        LAUNCH_CHECK();
+    }
     return SFB_OK;
 }
 
@@ -1292,28 +1508,27 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
     PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
-    const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
-    const int64_t wg_room = (P->lds_max - tbl_bytes) / 8;
-    if (wg_room >= 1024) {
-        const int32_t WG = (int32_t)std::min<int64_t>(8192, wg_room);
-        const int64_t per_pix = std::max<int64_t>(1, (P->ncat + P->npix - 1) / P->npix);
-        const int32_t PT = (int32_t)std::max<int64_t>(32, std::min<int64_t>(1024, WG / (2 * per_pix)));
-        const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(8, P->npix / PT));
-        const int64_t ppc = (P->npix + nchunk - 1) / nchunk;
-        const size_t shm = (size_t)tbl_bytes + 8 * (size_t)WG;
-        dim3 g2((unsigned)P->nmaps, (unsigned)nchunk);
+    if (P->sc_lds) {
+        const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
+        const int32_t WG = P->sc_WG, PT = P->sc_PT;
+        const int64_t ppc = P->sc_ppc;
+        const size_t shm = (size_t)tbl_bytes + 16 * 8 * (size_t)WG;
+        dim3 g2((unsigned)P->nmaps, (unsigned)P->sc_nchunk);
+        const double* rt = P->ncat > 0 ? P->d_rt.as<double>() : P->d_rs.as<double>();
+        const double* wt = P->ncat > 0 ? P->d_wt.as<double>() : P->d_ws.as<double>();
+        const int32_t* lp = P->ncat > 0 ? P->d_lpos.as<int32_t>() : P->d_pstart.as<int32_t>();
         if (win_kind == SFB_WIN_DENSE) {
             HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<true>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
             hipLaunchKernelGGL(k_scatter_lds<true>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
-                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(),
+                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(), rt, wt, lp,
                                P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
                                P->d_delta.as<double>());
         } else {
             HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<false>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
             hipLaunchKernelGGL(k_scatter_lds<false>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
-                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(),
+                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(), rt, wt, lp,
                                P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
                                P->d_delta.as<double>());
         }
@@ -1340,7 +1555,7 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st)
     const int64_t nside = P->nside;
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
     const int64_t M = 2 * nside;
-    size_t shf = sizeof(double2) * 2 * (size_t)P->TQF * (size_t)M;
+    size_t shf = sizeof(double2) * (2 * (size_t)P->TQF * (size_t)M + 2 * (size_t)(2 * M));
     HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
     hipLaunchKernelGGL(k_phi_fft, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
                        P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
@@ -1375,14 +1590,12 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     const GemmDesc* ana = P->d_desc_ana.as<GemmDesc>();
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
-    dim3 ga(nblk(rows, GBM), nblk(maxnb, GBN), nbat);
     {
         PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-        hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, ana, rows, domega, 0.0);
+        launch_gemm(ana, nbat, rows, maxnb, domega, 0.0, st);
         LAUNCH_CHECK();
     }
     P->a_final = P->d_a.as<double>();
-    dim3 gs(nblk(rows, GBM), nblk(std::max(Kal, 1), GBN), nbat);
     dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
@@ -1392,7 +1605,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-                hipLaunchKernelGGL(k_gemm_f64, gs, dim3(256), 0, st, syn, rows, 1.0, 0.0);
+                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st);
                 LAUNCH_CHECK();
             }
             {
@@ -1404,12 +1617,12 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         }
         {
             PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, c0, rows, 1.0, 1.0);
+            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st);
             LAUNCH_CHECK();
         }
         if (Kal > 0) {
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            hipLaunchKernelGGL(k_gemm_f64, ga, dim3(256), 0, st, al, rows, domega, 1.0);
+            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -1602,8 +1815,7 @@ extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, 
     d.ldc = rows;
     if (e == hipSuccess) e = hipMemcpy(dD.p, &d, sizeof d, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_gemm_f64, dim3(nblk(rows, GBM), nblk(N, GBN), 1), dim3(256), 0, 0, dD.as<GemmDesc>(),
-                           (int32_t)rows, 1.0, 0.0);
+        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, 0.0, 0);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(C, dC.p, sizeof(double) * (size_t)(N * rows), hipMemcpyDeviceToHost);
