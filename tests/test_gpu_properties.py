@@ -1,0 +1,84 @@
+"""Size-independent properties of the HIP cat2amln at the headline size
+(N = 1e6, kmax = 0.1, nside = 128: 2514 maps), where the oracle is too slow to
+run in full, plus the reference's error behaviour.
+
+* Permutation invariance: galaxies are summed per pixel in (r, input index)
+  order, so shuffling a catalogue with distinct radii gives bit-identical a_nlm.
+* Affine structure: cat2amln(w) = A(S(w) - W) with S linear, so
+  f(2w) - 2 f(w) + f(empty catalogue) = 0 up to rounding.
+* A bounded oracle sample: the first maps of the same call against the oracle.
+* Errors: a NaN weight gives AssertionError (src/cat2anlm.jl:308), a NaN angle is
+  rejected (Healpix.jl's floor(Int, NaN) throws), lmax > 4 nside is refused.
+"""
+import numpy as np
+import pytest
+
+import sfb_amd as S
+from helpers import rel_errors, separable_unit_window, shell_volume, uniform_shell_catalog
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    am = S.AnlmModes(0.1, 500.0, 1500.0, nside=128)
+    win, _ = separable_unit_window(S, am)
+    cat = uniform_shell_catalog(1_000_000, 500.0, 1500.0, seed=0)
+    nbar = cat.shape[1] / shell_volume(500.0, 1500.0)
+    plan = S.get_plan(am)
+    return am, win, cat, nbar, plan
+
+
+def test_permutation_invariance_bitexact(c2):
+    am, win, cat, nbar, plan = c2
+    say("C2 permutation invariance")
+    a = plan.cat2amln(cat, nbar, win)
+    perm = np.random.default_rng(11).permutation(cat.shape[1])
+    b = plan.cat2amln(cat[:, perm], nbar, win)
+    assert np.array_equal(a, b)
+
+
+def test_affine_in_weights(c2):
+    am, win, cat, nbar, plan = c2
+    say("C2 affine structure")
+    w = np.random.default_rng(12).uniform(0.5, 1.5, cat.shape[1])
+    f1 = plan.cat2amln(cat, nbar, win, w)
+    f2 = plan.cat2amln(cat, nbar, win, 2 * w)
+    f0 = plan.cat2amln(np.zeros((3, 0)), nbar, win, [])
+    resid = f2 - 2 * f1 + f0
+    assert np.max(np.abs(resid)) < 1e-11 * np.max(np.abs(f2))
+
+
+def test_c2_sample_against_oracle(c2):
+    am, win, cat, nbar, plan = c2
+    say("C2 oracle sample")
+    a = plan.cat2amln(cat, nbar, win)
+    om = O.OracleModes.__new__(O.OracleModes)
+    om.nside, om.lmax, om.nmax, om.lmax_n = am.nside, am.lmax, am.nmax, am.lmax_n
+    om.knot_off, om.knot_x, om.knot_y, om.knot_ypp = am.knot_off, am.knot_x, am.knot_y, am.knot_ypp
+    q1 = 12
+    ref = O.cat2amln(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), q_range=(0, q1))
+    n_out = sum(l + 1 for l in range(q1))  # maps 0..q1-1 are n = 1, l = 0..q1-1
+    d = np.abs(a[:n_out] - ref[:n_out])
+    assert np.max(d) < 1e-10 * np.max(np.abs(ref[:n_out]))
+
+
+def test_nan_weight_is_assertion_error():
+    am = S.AnlmModes(0.03, 500.0, 1000.0, nside=16)
+    win, _ = separable_unit_window(S, am, nr=100)
+    cat = uniform_shell_catalog(1000, 500.0, 1000.0, seed=2)
+    w = np.ones(1000)
+    w[17] = np.nan
+    with pytest.raises(AssertionError):
+        S.cat2amln(cat, am, 1e-4, win, w)
+    cat[1, 5] = np.nan
+    with pytest.raises(S.SFBError) as e:
+        S.cat2amln(cat, am, 1e-4, win)
+    assert "not finite" in str(e.value)
+    # the plan is still usable after errors
+    assert np.all(np.isfinite(S.cat2amln(cat[:, :4], am, 1e-4, win)))
