@@ -159,6 +159,10 @@ int sfb_map2alm(sfb_plan* p, int64_t nmap, const double* maps, double* alm_out);
  * counterpart: it checks the MFMA kernel that carries the map2alm Legendre sums. */
 int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
 
+/* The same GEMM through the FP32-operand MFMA kernel used for the Jacobi corrections
+ * of SFB_PREC_MIXED (operands rounded to FP32, FP64 accumulation across k-tiles). */
+int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
+
 /* The per-ring phi-transform of nmap maps [nmap][npix]: x_out[((m*2+par)*K + k)*2*nmap + 2q+c],
  * K = 2 nside ring pairs, par 0 = F_north + F_south, 1 = F_north - F_south,
  * F_m = sum_j map_j e^{-i m phi_j} (the ring DFT inside libsharp's map2alm). */

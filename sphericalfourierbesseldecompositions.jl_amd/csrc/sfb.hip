@@ -727,95 +727,137 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ d
             }
 }
 
-// Tall-skinny FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta Cin[n][i],
-// i < rows.  A block owns 64 rows and up to 16*NTM columns, so A (the large
-// operand) is read once per batch; 4 waves of 16 rows x all columns; the next
-// k-tile is prefetched into registers while the current one is on the MFMAs.
-template <int NTM>
-__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
-                                                   double beta)
+// Tall-skinny GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta Cin[n][i],
+// i < rows.  A block owns 64 rows and up to 128 columns, so A (the large operand)
+// is read once per batch; 4 waves of 16 rows x all columns; the next k-tile is
+// prefetched into registers while the current one is on the MFMAs.  The column
+// tile count NT of a block is a template argument (dispatched at run time), so
+// the MFMA accumulators never pass through a data-dependent branch.
+// F32: operands rounded to FP32 on the way into LDS and accumulated in FP32 on
+// v_mfma_f32_16x16x4f32 (used for the Jacobi corrections, K <= ~2 lmax); the
+// epilogue alpha*acc + beta*Cin is FP64 either way.
+// Result-row map: v_mfma_f64_16x16x4f64 leaves row (l/16) + 4v in register v of
+// lane l, v_mfma_f32_16x16x4f32 row 4(l/16) + v (both checked by sfb_gemm_f64/f32).
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <bool F32> struct MfmaT;
+template <> struct MfmaT<false> {
+    typedef double T;
+    typedef double4_t V;
+    static __device__ __forceinline__ V zero() { return (V){0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct MfmaT<true> {
+    typedef float T;
+    typedef float4_t V;
+    static __device__ __forceinline__ V zero() { return (V){0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 8, GT_BNW = 16 * GT_NTM;
+
+template <bool F32, int NT>
+__device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
+                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
+                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
-    constexpr int BM = 64, BK = 16, BNW = 16 * NTM;
-    const GemmDesc d = descs[blockIdx.z];
-    const int i0 = blockIdx.x * BM;
-    const int n0 = blockIdx.y * BNW;
-    if (n0 >= d.N) return;
-    const int ntl = min(NTM, (d.N - n0 + 15) / 16);
-    __shared__ double As[BK][BM];
-    __shared__ double Bs[BK][BNW];
+    typedef MfmaT<F32> M;
+    typedef typename M::T T;
+    constexpr int BNW = 16 * NT;  // active columns of this block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave * 16;
-    double4_t acc[NTM];
+    typename M::V acc[NT];
 #pragma unroll
-    for (int t = 0; t < NTM; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    double ra[4], rb[NTM];
+    for (int t = 0; t < NT; ++t) acc[t] = M::zero();
+    T ra[4], rb[NT];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = tid + 256 * u;
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
-            ra[u] = (gk < d.K && gi < rows) ? d.A[(int64_t)gk * d.lda + gi] : 0.0;
+            ra[u] = (gk < d.K && gi < rows) ? (T)d.A[(int64_t)gk * d.lda + gi] : (T)0;
         }
 #pragma unroll
-        for (int u = 0; u < NTM; ++u) {
+        for (int u = 0; u < NT; ++u) {
             const int e = tid + 256 * u;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
-            rb[u] = (gk < d.K && gn < d.N) ? d.B[(int64_t)gk * d.ldb + gn] : 0.0;
+            rb[u] = (gk < d.K && gn < d.N) ? (T)d.B[(int64_t)gk * d.ldb + gn] : (T)0;
         }
     };
     fetch(0);
-    for (int k0 = 0; k0 < d.K; k0 += BK) {
+    for (int k0 = 0; k0 < d.K; k0 += GT_BK) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = tid + 256 * u;
             As[e >> 6][e & 63] = ra[u];
         }
 #pragma unroll
-        for (int u = 0; u < NTM; ++u) {
+        for (int u = 0; u < NT; ++u) {
             const int e = tid + 256 * u;
             Bs[e / BNW][e % BNW] = rb[u];
         }
         __syncthreads();
-        if (k0 + BK < d.K) fetch(k0 + BK);
+        if (k0 + GT_BK < d.K) fetch(k0 + GT_BK);
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 4) {
+        for (int ks = 0; ks < GT_BK; ks += 4) {
             const int kk = ks + (lane >> 4);
-            const double fb = As[kk][wi + (lane & 15)];
+            const T fb = As[kk][wi + (lane & 15)];
 #pragma unroll
-            for (int t = 0; t < NTM; ++t)
-                if (t < ntl) {
-                    const double fa = Bs[kk][t * 16 + (lane & 15)];
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fb, acc[t], 0, 0, 0);
-                }
+            for (int t = 0; t < NT; ++t) acc[t] = M::mma(Bs[kk][t * 16 + (lane & 15)], fb, acc[t]);
         }
         __syncthreads();
     }
     const int i = i0 + wi + (lane & 15);
 #pragma unroll
-    for (int t = 0; t < NTM; ++t)
-        if (t < ntl) {
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int n = n0 + t * 16 + (lane >> 4) + 4 * v;
-                if (n < d.N && i < rows) {
-                    const int64_t o = (int64_t)n * d.ldc + i;
-                    double val = alpha * acc[t][v];
-                    if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
-                    d.C[o] = val;
-                }
+        for (int v = 0; v < 4; ++v) {
+            const int n = n0 + t * 16 + M::row(lane, v);
+            if (n < d.N && i < rows) {
+                const int64_t o = (int64_t)n * d.ldc + i;
+                double val = alpha * (double)acc[t][v];
+                if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
+                d.C[o] = val;
             }
         }
 }
 
-constexpr int TALL_NTM = 8;  // 128 columns per block
+template <bool F32>
+__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+                                                   double beta)
+{
+    typedef typename MfmaT<F32>::T T;
+    __shared__ T As[GT_BK][GT_BM];
+    __shared__ T Bs[GT_BK][GT_BNW];
+    const GemmDesc d = descs[blockIdx.z];
+    const int i0 = blockIdx.x * GT_BM;
+    const int n0 = blockIdx.y * GT_BNW;
+    if (n0 >= d.N) return;
+    const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
+    switch (ntl) {
+        case 1: gemm_tall_body<F32, 1>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 5: gemm_tall_body<F32, 5>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 6: gemm_tall_body<F32, 6>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 7: gemm_tall_body<F32, 7>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, 8>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+    }
+}
 
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
-                               double beta, hipStream_t st)
+                               double beta, hipStream_t st, bool f32 = false)
 {
-    dim3 g((unsigned)((rows + 63) / 64), (unsigned)((maxN + 16 * TALL_NTM - 1) / (16 * TALL_NTM)), nbat);
-    hipLaunchKernelGGL(k_gemm_tall<TALL_NTM>, g, dim3(256), 0, st, descs, rows, alpha, beta);
+    dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
+    if (f32)
+        hipLaunchKernelGGL(k_gemm_tall<true>, g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else
+        hipLaunchKernelGGL(k_gemm_tall<false>, g, dim3(256), 0, st, descs, rows, alpha, beta);
 }
 
 // ---------------------------------------------------------------------------
@@ -1596,6 +1638,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         LAUNCH_CHECK();
     }
     P->a_final = P->d_a.as<double>();
+    const bool mixed = P->precision == SFB_PREC_MIXED;
     dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
@@ -1605,7 +1648,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st);
+                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, mixed);
                 LAUNCH_CHECK();
             }
             {
@@ -1617,12 +1660,12 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         }
         {
             PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st);
+            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st, mixed);
             LAUNCH_CHECK();
         }
         if (Kal > 0) {
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st);
+            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st, mixed);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -1789,8 +1832,23 @@ extern "C" int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const 
     return SFB_OK;
 }
 
+static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
+                      double* C, bool f32);
+
 extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
                             double* C)
+{
+    return gemm_check(device, rows, K, N, A, B, C, false);
+}
+
+extern "C" int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
+                            double* C)
+{
+    return gemm_check(device, rows, K, N, A, B, C, true);
+}
+
+static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
+                      double* C, bool f32)
 {
     sfb::clear_error();
     if (rows < 1 || K < 1 || N < 1 || rows > (1 << 24) || N > (1 << 24) || !A || !B || !C)
@@ -1815,7 +1873,7 @@ extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, 
     d.ldc = rows;
     if (e == hipSuccess) e = hipMemcpy(dD.p, &d, sizeof d, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, 0.0, 0);
+        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, 0.0, 0, f32);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(C, dC.p, sizeof(double) * (size_t)(N * rows), hipMemcpyDeviceToHost);

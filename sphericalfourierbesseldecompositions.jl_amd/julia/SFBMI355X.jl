@@ -1,0 +1,113 @@
+# SFBMI355X -- drop-in cat2amln for SphericalFourierBesselDecompositions.jl on an
+# MI355X, through the C ABI of libsfb_mi355x.so (include/sfb.h).
+#
+#     import SphericalFourierBesselDecompositions as SFB
+#     import SFBMI355X
+#     anlm = SFBMI355X.cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weights)   # was SFB.cat2amln
+#
+# Same positional signature and return value as SFB.cat2amln
+# (src/cat2anlm.jl:257-310): rθϕ is 3 x N, win_rhat_ln is the dense
+# (npix, lmax+1, nmax) array or the SeparableArray(mask, w_ln) that SFB.win_rhat_ln
+# returns, weight defaults to ones(N).  The basis-function splines are passed
+# straight from amodes.basisfunctions.gnl[n, l+1] (x, y, ypp), so the knots are
+# the caller's own.  A new function, not a method of SFB.cat2amln (no type piracy).
+# Only argument marshalling lives here; every computation runs in the library.
+module SFBMI355X
+
+export cat2amln, cat2nlm
+
+const libsfb = get(ENV, "SFB_MI355X_LIB",
+                   joinpath(@__DIR__, "..", "sfb_amd", "libsfb_mi355x.so"))
+
+const SFB_OK = 0
+const SFB_ERR_NONFINITE = 3
+
+function check(rc)
+    rc == SFB_OK && return nothing
+    msg = unsafe_string(ccall((:sfb_last_error, libsfb), Cstring, ()))
+    rc == SFB_ERR_NONFINITE && throw(AssertionError(msg))
+    error("libsfb_mi355x: " * msg)
+end
+
+mutable struct Plan
+    ptr::Ptr{Cvoid}
+    nlmsize::Int
+    npix::Int
+    function Plan(ptr, nlmsize, npix)
+        p = new(ptr, nlmsize, npix)
+        finalizer(p) do q
+            q.ptr == C_NULL || ccall((:sfb_plan_destroy, libsfb), Cvoid, (Ptr{Cvoid},), q.ptr)
+            q.ptr = C_NULL
+        end
+        return p
+    end
+end
+
+const plans = IdDict{Any,Dict{Tuple{Int,Symbol,Int},Plan}}()
+
+"""
+    plan(amodes; niter=3, precision=:fp64, device=0)
+
+Device plan for `amodes` (spline tables, lambda tables, work buffers), cached per
+amodes, niter, precision and device.
+"""
+function plan(amodes; niter::Integer=3, precision::Symbol=:fp64, device::Integer=0)
+    cache = get!(() -> Dict{Tuple{Int,Symbol,Int},Plan}(), plans, amodes)
+    key = (Int(niter), precision, Int(device))
+    haskey(cache, key) && return cache[key]
+    gnl = amodes.basisfunctions.gnl
+    xs, ys, ypps, offs = Float64[], Float64[], Float64[], Int64[0]
+    for n in 1:amodes.nmax, l in 0:amodes.lmax_n[n]
+        s = gnl[n, l+1]
+        append!(xs, s.x); append!(ys, s.y); append!(ypps, s.ypp)
+        push!(offs, length(xs))
+    end
+    lmax_n = Int64.(amodes.lmax_n)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    prec = precision === :fp64 ? Int32(0) : precision === :mixed ? Int32(1) : error("precision must be :fp64 or :mixed")
+    rc = ccall((:sfb_plan_create, libsfb), Cint,
+               (Ptr{Ptr{Cvoid}}, Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Int32, Int32, Int32),
+               out, amodes.nside, amodes.lmax, amodes.nmax, lmax_n, offs, xs, ys, ypps,
+               Int32(niter), prec, Int32(device))
+    check(rc)
+    nlmsize = sum(l -> (l + 1) * (l + 2) ÷ 2, amodes.lmax_n)
+    return cache[key] = Plan(out[], nlmsize, 12 * amodes.nside^2)
+end
+
+# (kind, a, b): dense W[npix, lmax+1, nmax], or separable mask[npix] and w_ln[lmax+1, nmax]
+function window_arrays(win)
+    if hasfield(typeof(win), :arr1) && hasfield(typeof(win), :arr2)   # SeparableArray
+        return Int32(1), Vector{Float64}(vec(getfield(win, :arr1))), Matrix{Float64}(getfield(win, :arr2))
+    end
+    return Int32(0), Array{Float64}(win), Float64[]
+end
+
+"""
+    cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight=ones(N); niter=3, precision=:fp64, device=0)
+
+Catalogue -> SFB coefficients on the GPU; the reference's cat2amln
+(src/cat2anlm.jl:257-310) with the same arguments and result.
+"""
+function cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight=ones(eltype(rθϕ), size(rθϕ, 2));
+                  niter::Integer=3, precision::Symbol=:fp64, device::Integer=0)
+    p = plan(amodes; niter, precision, device)
+    rtp = Matrix{Float64}(rθϕ)
+    N = size(rtp, 2)
+    w = isempty(weight) ? Float64[] : Vector{Float64}(weight)
+    kind, a, b = window_arrays(win_rhat_ln)
+    out = Vector{Float64}(undef, 2 * p.nlmsize)
+    GC.@preserve rtp w a b out begin
+        rc = ccall((:sfb_cat2amln, libsfb), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Int32, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}),
+                   p.ptr, N, N > 0 ? pointer(rtp) : C_NULL, isempty(w) ? C_NULL : pointer(w),
+                   Float64(nbar), kind, pointer(a), kind == 1 ? pointer(b) : C_NULL, pointer(out))
+    end
+    check(rc)
+    return copy(reinterpret(ComplexF64, out))
+end
+
+const cat2nlm = cat2amln
+
+end # module

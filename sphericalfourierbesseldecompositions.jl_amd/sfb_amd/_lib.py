@@ -48,6 +48,7 @@ SIGNATURES = {
     "sfb_delta_map": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _i32, _P, _P, _i64, _P]),
     "sfb_map2alm": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_gemm_f64": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
+    "sfb_gemm_f32": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
     "sfb_phi_transform": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_plan_profile": (ctypes.c_int, [_P, _i32]),
     "sfb_plan_phase_times": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -22,6 +22,17 @@ def test_gemm_f64_matches_numpy(rows, K, N):
     assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * K
 
 
+@pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (70, 37, 23), (130, 256, 71)])
+def test_gemm_f32_matches_numpy(rows, K, N):
+    rng = np.random.default_rng(rows * 3 + K + N)
+    A = rng.normal(size=(K, rows))
+    B = rng.normal(size=(K, N))
+    C = np.empty((N, rows))
+    L.check(L.lib().sfb_gemm_f32(0, rows, K, N, L.ptr(A), L.ptr(B), L.ptr(C)))
+    ref = B.astype(np.float32).astype(np.float64).T @ A.astype(np.float32).astype(np.float64)
+    assert np.max(np.abs(C - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
 def test_phi_transform_matches_direct_dft():
     nside, lmax = 8, 16
     am = S.AnlmModes.from_nl(2, lmax, 500.0, 1000.0, nside=nside)
