@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--cpu-maps", type=int, default=48, help="maps the CPU baseline sample runs")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one elementwise copy of 512 MiB (torch.neg, FP64) before the steps, as the "
+                         "known-size kernel that scales FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the (n,l) maps would shard "
@@ -118,6 +121,14 @@ def main():
     plan = S.Plan(am, niter=niter, precision="fp64")
     t_plan = time.perf_counter() - t0
     log(f"plan: {t_plan:.2f}s, window: {t_win:.2f}s")
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     def run(prec):
         plan.set_precision(prec)
@@ -161,6 +172,17 @@ def main():
         achieved = w["bytes"] / t_launch / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+
+    # HBM bytes per launch of the dominant kernel from the FETCH_SIZE / WRITE_SIZE
+    # counter passes (scaled by the same counters on the calibration copy)
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f).get(head, {}).get(dom)
+        if tr:
+            roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
+            roof["traffic_unit"] = "B/launch"
+            roof["algorithmic_bytes"] = w["bytes"]
 
     # CPU baseline + accuracy on a bounded sample of maps
     cpu = None
