@@ -153,6 +153,22 @@ def main():
     for prec in precs:
         results[prec] = run(prec)
 
+    # adjoint (sfb_cat2amln_vjp_device, FP64) on the same catalogue: dL/dr, dL/dw for a
+    # random cotangent, inputs resident in HBM
+    plan.set_precision("fp64")
+    d_cot = torch.randn(2 * nlm, dtype=torch.float64, device=dev)
+    d_dr = torch.empty(N, dtype=torch.float64, device=dev)
+    d_dw = torch.empty(N, dtype=torch.float64, device=dev)
+    for _ in range(max(1, args.warmup)):
+        plan.cat2amln_vjp_device(N, d_rtp, None, nbar, d_cot, d_dr, d_dw)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        plan.cat2amln_vjp_device(N, d_rtp, None, nbar, d_cot, d_dr, d_dw)
+    torch.cuda.synchronize()
+    adj_dt = (time.perf_counter() - t) / args.steps
+    log(f"[adjoint fp64] {adj_dt * 1e3:.2f} ms/step")
+
     head = "mixed" if "mixed" in results else "fp64"
     dt, phases, calls, anlm = results[head]
     work = algorithmic_work(am, N, niter)
@@ -231,6 +247,7 @@ def main():
         "errors_vs_oracle": errs,
         "phases_ms": {k: v[0] / max(calls, 1) for k, v in phases.items()},
         "setup_s": {"AnlmModes": t_modes, "plan": t_plan},
+        "adjoint_fp64_ms": adj_dt * 1e3,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -117,6 +117,24 @@ int sfb_cat2amln_device(sfb_plan* p, int64_t N, const double* d_rtp, const doubl
                         int32_t win_kind, const double* d_win_a, const double* d_win_b,
                         double* d_anlm_out, void* stream);
 
+/* ---- adjoint ----------------------------------------------------------------- */
+
+/* Vector-Jacobian product of cat2amln at (rθϕ, w, nbar): given dL_danlm (2*nlmsize
+ * doubles, (dL/dRe a, dL/dIm a) interleaved in a_nlm order), returns dL/dr_i and
+ * dL/dw_i (N each, input order) and dL/dnbar.  dL/dθ and dL/dφ are zero (the
+ * pixel assignment is piecewise constant) and the window does not enter (the
+ * transform is affine in it).  g'(r) is the spline derivative of src/Splines.jl:367-377
+ * (derivative(gnl, n, l, r), src/GNLs.jl:110).  No counterpart in the reference,
+ * which has no derivative of cat2amln; it backs the ChainRulesCore.rrule of the
+ * Julia shim.  Any of dL_dr, dL_dw, dL_dnbar may be NULL. */
+int sfb_cat2amln_vjp(sfb_plan* p, int64_t N, const double* rtp, const double* w, double nbar,
+                     const double* dL_danlm, double* dL_dr, double* dL_dw, double* dL_dnbar);
+
+/* Same with device arrays (d_dr or d_dw may be NULL), enqueued on stream
+ * (NULL = the plan's stream) and synchronised before returning. */
+int sfb_cat2amln_vjp_device(sfb_plan* p, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                            const double* d_dL_danlm, double* d_dr, double* d_dw, void* stream);
+
 /* ---- per-phase timing ------------------------------------------------------ */
 
 #define SFB_PHASE_CATALOG 0    /* ang2pix, the two stable sorts, gathers, pixel segments */

@@ -960,6 +960,198 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_f64(double* p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of cat2amln.  a = J delta with J = P_niter(A S) A (A: analysis with pixel
+// weight dOmega, S: synthesis, P_n(X) = sum_{j<=n} (I - X)^j, the Healpix iteration).
+// In real coordinates A^T = dOmega S D_half and S^T = D_c A / dOmega (D_half halves,
+// D_c doubles the m > 0 entries), hence J^T c = dOmega S P_niter(A S) D_half c: the
+// same Jacobi recursion run on b = D_half c, then one synthesis to pixels.
+// ---------------------------------------------------------------------------
+
+// b = D_half(cotangent) in the a layout: per map q the row l = l_q, m = 0..l, halved
+// for m > 0; Im of m = 0 is dropped (Im a_l0 vanishes identically).  a is zeroed first.
+__global__ void k_unpack_cot(const double* __restrict__ cot, const int64_t* __restrict__ aoff,
+                             const MapInfo* __restrict__ maps, const int64_t* __restrict__ out_base, int32_t nq,
+                             int64_t M2, double* __restrict__ a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (q >= nq) return;
+    const int l = maps[q].l;
+    if (m > l) return;
+    const int par = (l - m) & 1, j = (l - m) >> 1;
+    double* dst = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
+    const double re = cot[2 * (out_base[q] + m)], im = cot[2 * (out_base[q] + m) + 1];
+    dst[0] = m ? 0.5 * re : re;
+    dst[1] = m ? 0.5 * im : 0.0;
+}
+
+// Pixel values from ring-Fourier coefficients (X layout of the phi-transform, E/O
+// form): out[q][p] = scale * sum_{m<=L} c_m Re(G_m e^{i m phi_j}), c_0 = 1, c_m = 2,
+// G = E + O on the north ring and E - O on the south ring of pair k, by complex
+// Horner in z = e^{i phi_j}.
+__global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
+                                                   int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                   double scale, double* __restrict__ out, int32_t TQ)
+{
+    extern __shared__ double2 gs[];  // [TQ][side][L+1]
+    const int k = blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev R = rings[k];
+    const int npr = R.npr;
+    const bool eq = (k == K - 1);
+    const int nside = eq ? 1 : 2;
+    for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
+        const int m = e / TQ, ql = e - m * TQ;
+        const int q = q0 + ql;
+        double er = 0, ei = 0, orr = 0, oi = 0;
+        if (q < nq) {
+            const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+            er = X[be];
+            ei = X[be + 1];
+            if (m + 1 <= L) {
+                const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+                orr = X[bo];
+                oi = X[bo + 1];
+            }
+        }
+        const double c = m ? 2.0 : 1.0;
+        gs[(ql * 2 + 0) * (L + 1) + m] = make_double2(c * (er + orr), c * (ei + oi));
+        gs[(ql * 2 + 1) * (L + 1) + m] = make_double2(c * (er - orr), c * (ei - oi));
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TQ * nside * npr; e += blockDim.x) {
+        const int j = e % npr, t = e / npr;
+        const int side = t % nside, ql = t / nside;
+        const int q = q0 + ql;
+        if (q >= nq) continue;
+        double zs, zc;
+        sincospi((double)(2 * j + R.shift) / (double)npr, &zs, &zc);
+        const double2* g = gs + (ql * 2 + side) * (L + 1);
+        double ar = 0.0, ai = 0.0;
+        for (int m = L; m >= 0; --m) {
+            const double2 gm = g[m];
+            const double nr = fma(ar, zc, fma(-ai, zs, gm.x));
+            const double ni = fma(ar, zs, fma(ai, zc, gm.y));
+            ar = nr;
+            ai = ni;
+        }
+        out[(int64_t)q * npix + (side ? R.start_s : R.start_n) + j] = scale * ar;
+    }
+}
+
+// Spline1D{3} value and derivative (src/Splines.jl:61-78,367-377): the derivative is
+// (c + t(2b + 3at))/h inside the knots and 0 where the value is the constant
+// boundary extrapolation.
+__device__ __forceinline__ void spline_vd(const MapInfo& mi, const double4* __restrict__ tb, double r, double& v,
+                                          double& dv)
+{
+    if (!(r >= mi.xfirst) || r > mi.xlast) {
+        v = (r < mi.xfirst) ? mi.yfirst : (r > mi.xlast ? mi.ylast : r);
+        dv = (r == r) ? 0.0 : r;
+        return;
+    }
+    int i = (int)((r - mi.xfirst) * mi.invdx);
+    i = min(max(i, 0), mi.nk - 2);
+    double4 lo = tb[i];
+    double2 hi = *(const double2*)&tb[i + 1];
+    while (i > 0 && r < lo.x) {
+        --i;
+        lo = tb[i];
+        hi = *(const double2*)&tb[i + 1];
+    }
+    while (i < mi.nk - 2 && r >= hi.x) {
+        ++i;
+        lo = tb[i];
+        hi = *(const double2*)&tb[i + 1];
+    }
+    const double h = hi.x - lo.x;
+    const double t = (r - lo.x) / h;
+    const double b = h * h * lo.z * 0.5;
+    const double a = lo.w;
+    const double c = ((hi.y - a) - b) - lo.y;
+    v = fma(t, fma(t, fma(t, a, b), c), lo.y);
+    dv = fma(t, fma(t, 3.0 * a, 2.0 * b), c) / h;
+}
+
+// part[g][k] = (sum_q g_q(r_k) dbar_q(p_k), sum_q g'_q(r_k) dbar_q(p_k)) over the maps
+// q of group g = blockIdx.y, for the VG_GPT * 1024 galaxies (pixel order) of chunk
+// blockIdx.x.  Each map's spline table is staged in LDS (or read from HBM when it
+// does not fit).
+constexpr int VG_GPT = 8;
+__global__ __launch_bounds__(1024) void k_vjp_gather(int64_t N, int64_t npix, const double* __restrict__ r_s,
+                                                     const uint32_t* __restrict__ pix_s,
+                                                     const MapInfo* __restrict__ maps,
+                                                     const double4* __restrict__ rec, int32_t nq, int32_t MG,
+                                                     bool lds_tables, const double* __restrict__ dbar,
+                                                     double2* __restrict__ part)
+{
+    extern __shared__ double4 vtb[];
+    const int64_t k0 = (int64_t)blockIdx.x * (1024 * VG_GPT) + threadIdx.x;
+    double r[VG_GPT], aw[VG_GPT], ar[VG_GPT];
+    int64_t pp[VG_GPT];
+#pragma unroll
+    for (int u = 0; u < VG_GPT; ++u) {
+        const int64_t k = k0 + 1024 * u;
+        r[u] = (k < N) ? r_s[k] : 0.0;
+        pp[u] = (k < N) ? (int64_t)pix_s[k] : 0;
+        aw[u] = 0.0;
+        ar[u] = 0.0;
+    }
+    const int qb = blockIdx.y * MG, qe = min(nq, qb + MG);
+    for (int q = qb; q < qe; ++q) {
+        const MapInfo mi = maps[q];
+        const double4* tb = rec + mi.koff;
+        if (lds_tables) {
+            __syncthreads();  // the previous table is no longer read
+            for (int e = threadIdx.x; e < mi.nk; e += blockDim.x) vtb[e] = rec[mi.koff + e];
+            __syncthreads();
+            tb = vtb;
+        }
+        const double* dq = dbar + (int64_t)q * npix;
+#pragma unroll
+        for (int u = 0; u < VG_GPT; ++u) {
+            if (k0 + 1024 * u < N) {
+                double v, dv;
+                spline_vd(mi, tb, r[u], v, dv);
+                const double d = dq[pp[u]];
+                aw[u] = fma(v, d, aw[u]);
+                ar[u] = fma(dv, d, ar[u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < VG_GPT; ++u) {
+        const int64_t k = k0 + 1024 * u;
+        if (k < N) part[(int64_t)blockIdx.y * N + k] = make_double2(aw[u], ar[u]);
+    }
+}
+
+// dL/dw_i = inv sum_g part_w, dL/dr_i = inv w_i sum_g part_r, summed over the map
+// groups in order and written at the galaxy's input position.
+__global__ void k_vjp_reduce(int64_t N, int32_t ngroups, const double2* __restrict__ part,
+                             const uint32_t* __restrict__ idx, const double* __restrict__ w_s, double inv,
+                             double* __restrict__ dr, double* __restrict__ dw)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double sw = 0.0, sr = 0.0;
+    for (int g = 0; g < ngroups; ++g) {
+        const double2 v = part[(int64_t)g * N + k];
+        sw += v.x;
+        sr += v.y;
+    }
+    const uint32_t i = idx[k];
+    if (dw) dw[i] = inv * sw;
+    if (dr) dr[i] = inv * w_s[k] * sr;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1011,6 +1203,8 @@ struct sfb_plan {
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
     // work
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
+    DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
+    DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
     double* a_final = nullptr;     // the a buffer holding the last result
@@ -1317,6 +1511,26 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             ad.ldc = P->M2;
             al.push_back(ad);
         }
+    // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over all K ring pairs,
+    // one set per a buffer (d_a, d_a1, d_a2)
+    std::vector<GemmDesc> synf;
+    for (int slot = 0; slot < 3; ++slot)
+        for (int64_t b = 0; b < nbat; ++b) {
+            int32_t n = P->nb[(size_t)b];
+            GemmDesc fd{};
+            fd.A = a_in_slot[slot] + P->aoff[(size_t)b];
+            fd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
+            fd.C = P->d_X0.as<double>() + b * (int64_t)P->K * P->M2;
+            fd.Cin = nullptr;
+            fd.K = n;
+            fd.N = (n > 0) ? P->K : 0;
+            fd.lda = P->M2;
+            fd.ldb = P->K;
+            fd.ldc = P->M2;
+            synf.push_back(fd);
+        }
+    if ((rc = P->d_desc_synf.ensure(sizeof(GemmDesc) * synf.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_synf.p, synf.data(), sizeof(GemmDesc) * synf.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_desc_ana.ensure(sizeof(GemmDesc) * ana.size()))) return rc;
     if ((rc = P->d_desc_syn.ensure(sizeof(GemmDesc) * syn.size()))) return rc;
     if ((rc = P->d_desc_c0.ensure(sizeof(GemmDesc) * c0.size()))) return rc;
@@ -1408,7 +1622,8 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
-                      &P->h_rtp, &P->h_w, &P->h_wa, &P->h_wb, &P->h_out};
+                      &P->h_rtp, &P->h_w, &P->h_wa, &P->h_wb, &P->h_out,
+                      &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -1615,28 +1830,16 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st)
     return SFB_OK;
 }
 
-// delta (first nq maps) -> a, with niter refinements.
-static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
+// niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a
+// (Healpix map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.
+static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int32_t rows = 2 * nq;
     const double domega = 4.0 * kPi / (double)P->npix;
-    size_t shm = sizeof(double) * 2 * (size_t)P->TQ * (size_t)(4 * P->nside + 1);
-    {
-        PhaseTimer timer(P, SFB_PHASE_PHI, st);
-        (void)shm;
-        int rc = phi_transform(P, nq, st);
-        if (rc) return rc;
-    }
     const unsigned nbat = (unsigned)(2 * (L + 1));
-    const GemmDesc* ana = P->d_desc_ana.as<GemmDesc>();
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
-    {
-        PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-        launch_gemm(ana, nbat, rows, maxnb, domega, 0.0, st);
-        LAUNCH_CHECK();
-    }
     P->a_final = P->d_a.as<double>();
     const bool mixed = P->precision == SFB_PREC_MIXED;
     dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
@@ -1671,6 +1874,24 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
     }
     return SFB_OK;
+}
+
+// delta (first nq maps) -> a, with niter refinements.
+static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
+{
+    const int32_t L = (int32_t)P->lmax;
+    const double domega = 4.0 * kPi / (double)P->npix;
+    {
+        PhaseTimer timer(P, SFB_PHASE_PHI, st);
+        int rc = phi_transform(P, nq, st);
+        if (rc) return rc;
+    }
+    {
+        PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
+        launch_gemm(P->d_desc_ana.as<GemmDesc>(), (unsigned)(2 * (L + 1)), 2 * nq, P->nb[0], domega, 0.0, st);
+        LAUNCH_CHECK();
+    }
+    return jacobi_stage(P, nq, st);
 }
 
 static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, const double* wb)
@@ -1761,6 +1982,106 @@ extern "C" int sfb_cat2amln(sfb_plan* P, int64_t N, const double* rtp, const dou
                            P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
     if (rc) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
+    return SFB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (vector-Jacobian product) of cat2amln
+// ---------------------------------------------------------------------------
+extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                                       const double* d_cot, double* d_dr, double* d_dw, void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (N < 0 || (N > 0 && !d_rtp) || !d_cot) return fail(SFB_ERR_ARG, "bad catalogue or cotangent pointer");
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    if (N == 0) return SFB_OK;
+    int rc;
+    if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
+    const int32_t nq = (int32_t)P->nmaps, L = (int32_t)P->lmax;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(P->a_size, 256)), dim3(256), 0, st, P->d_a.as<double>(), P->a_size, 0.0);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_unpack_cot, dim3(nblk(nq, 256), (unsigned)(L + 1)), dim3(256), 0, st, d_cot,
+                       P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(), P->d_outbase.as<int64_t>(), nq, P->M2,
+                       P->d_a.as<double>());
+    LAUNCH_CHECK();
+    if ((rc = jacobi_stage(P, nq, st))) return rc;
+    // dbar = dOmega S a_final on every pixel of every map
+    const int slot = (P->a_final == P->d_a.as<double>()) ? 0 : (P->a_final == P->d_a1.as<double>() ? 1 : 2);
+    const unsigned nbat = (unsigned)(2 * (L + 1));
+    launch_gemm(P->d_desc_synf.as<GemmDesc>() + slot * nbat, nbat, 2 * nq, P->K, 1.0, 0.0, st);
+    LAUNCH_CHECK();
+    const double domega = 4.0 * kPi / (double)P->npix;
+    const int32_t TQS = 8;
+    const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
+    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+    hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)P->K, nblk(nq, TQS)), dim3(256), shs, st, P->d_X0.as<double>(),
+                       P->npix, nq, P->M2, P->K, L, P->d_rings.as<RingDev>(), domega, P->d_delta.as<double>(), TQS);
+    LAUNCH_CHECK();
+    // per-galaxy sums over the maps, in 32 map groups, then the ordered reduction
+    const int32_t MG = (nq + 31) / 32;
+    const int32_t ngroups = (nq + MG - 1) / MG;
+    if ((rc = P->d_part.ensure(sizeof(double2) * (size_t)ngroups * (size_t)N))) return rc;
+    const bool lds_tables = (int64_t)P->tbl_cap * 32 <= P->lds_max;
+    const size_t shg = lds_tables ? (size_t)P->tbl_cap * 32 : 0;
+    if (lds_tables)
+        HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shg));
+    hipLaunchKernelGGL(k_vjp_gather, dim3(nblk(N, 1024 * VG_GPT), (unsigned)ngroups), dim3(1024), shg, st, N,
+                       P->npix, P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(), P->d_maps.as<MapInfo>(),
+                       P->d_rec.as<double4>(), nq, MG, lds_tables, P->d_delta.as<double>(),
+                       P->d_part.as<double2>());
+    LAUNCH_CHECK();
+    const double inv = 1.0 / (nbar * domega);
+    hipLaunchKernelGGL(k_vjp_reduce, dim3(nblk(N, 256)), dim3(256), 0, st, N, ngroups, P->d_part.as<double2>(),
+                       P->d_idx.as<uint32_t>(), P->d_ws.as<double>(), inv, d_dr, d_dw);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(st));
+    return SFB_OK;
+}
+
+extern "C" int sfb_cat2amln_vjp(sfb_plan* P, int64_t N, const double* rtp, const double* w, double nbar,
+                                const double* dL_danlm, double* dL_dr, double* dL_dw, double* dL_dnbar)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (N < 0 || (N > 0 && !rtp) || !dL_danlm) return fail(SFB_ERR_ARG, "bad catalogue or cotangent pointer");
+    if (dL_dnbar) *dL_dnbar = 0.0;
+    if (N == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(P->device));
+    int rc;
+    if ((rc = P->h_rtp.ensure(sizeof(double) * 3 * (size_t)N)) || (rc = P->h_cot.ensure(sizeof(double) * 2 * (size_t)P->nlmsize)) ||
+        (rc = P->h_dr.ensure(sizeof(double) * (size_t)N)) || (rc = P->h_dw.ensure(sizeof(double) * (size_t)N)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(P->h_rtp.p, rtp, sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->h_cot.p, dL_danlm, sizeof(double) * 2 * (size_t)P->nlmsize, hipMemcpyHostToDevice,
+                           P->stream));
+    const double* d_w = nullptr;
+    if (w) {
+        if ((rc = P->h_w.ensure(sizeof(double) * (size_t)N))) return rc;
+        HIP_TRY(hipMemcpyAsync(P->h_w.p, w, sizeof(double) * (size_t)N, hipMemcpyHostToDevice, P->stream));
+        d_w = P->h_w.as<double>();
+    }
+    if ((rc = sfb_cat2amln_vjp_device(P, N, P->h_rtp.as<double>(), d_w, nbar, P->h_cot.as<double>(),
+                                      P->h_dr.as<double>(), P->h_dw.as<double>(), P->stream)))
+        return rc;
+    std::vector<double> dw_host;
+    double* dw_dst = dL_dw;
+    if (!dw_dst && dL_dnbar) {
+        dw_host.resize((size_t)N);
+        dw_dst = dw_host.data();
+    }
+    if (dL_dr)
+        HIP_TRY(hipMemcpyAsync(dL_dr, P->h_dr.p, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, P->stream));
+    if (dw_dst)
+        HIP_TRY(hipMemcpyAsync(dw_dst, P->h_dw.p, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    if (dL_dnbar) {
+        // inv = 1/(nbar dOmega) multiplies every galaxy term: dL/dnbar = -(1/nbar) sum_i w_i dL/dw_i
+        long double acc = 0.0L;
+        for (int64_t i = 0; i < N; ++i) acc += (long double)(w ? w[i] : 1.0) * (long double)dw_dst[i];
+        *dL_dnbar = (double)(-acc / (long double)nbar);
+    }
     return SFB_OK;
 }
 

@@ -12,9 +12,13 @@
 # straight from amodes.basisfunctions.gnl[n, l+1] (x, y, ypp), so the knots are
 # the caller's own.  A new function, not a method of SFB.cat2amln (no type piracy).
 # Only argument marshalling lives here; every computation runs in the library.
+# cat2amln_vjp and the ChainRulesCore.rrule give the adjoint (no counterpart in the
+# reference, which has no derivative of cat2amln).
 module SFBMI355X
 
-export cat2amln, cat2nlm
+import ChainRulesCore
+
+export cat2amln, cat2nlm, cat2amln_vjp
 
 const libsfb = get(ENV, "SFB_MI355X_LIB",
                    joinpath(@__DIR__, "..", "sfb_amd", "libsfb_mi355x.so"))
@@ -109,5 +113,45 @@ function cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight=ones(eltype(rθϕ), s
 end
 
 const cat2nlm = cat2amln
+
+"""
+    cat2amln_vjp(rθϕ, amodes, nbar, dL_danlm, weight=ones(N); niter=3, precision=:fp64, device=0)
+
+(∂L/∂r, ∂L/∂w, ∂L/∂nbar) of cat2amln for the cotangent `dL_danlm` of its output
+(∂L/∂Re a + i ∂L/∂Im a).  θ and ϕ get no gradient (the pixel is piecewise constant);
+g'(r) is the spline derivative of src/Splines.jl:367-377.
+"""
+function cat2amln_vjp(rθϕ, amodes, nbar, dL_danlm, weight=ones(eltype(rθϕ), size(rθϕ, 2));
+                      niter::Integer=3, precision::Symbol=:fp64, device::Integer=0)
+    p = plan(amodes; niter, precision, device)
+    rtp = Matrix{Float64}(rθϕ)
+    N = size(rtp, 2)
+    w = isempty(weight) ? Float64[] : Vector{Float64}(weight)
+    cot = Vector{ComplexF64}(dL_danlm)
+    length(cot) == p.nlmsize || throw(DimensionMismatch("dL_danlm must have getnlmsize(amodes) entries"))
+    dr, dw, dnbar = zeros(N), zeros(N), Ref(0.0)
+    GC.@preserve rtp w cot dr dw begin
+        rc = ccall((:sfb_cat2amln_vjp, libsfb), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{ComplexF64}, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}),
+                   p.ptr, N, N > 0 ? pointer(rtp) : C_NULL, isempty(w) ? C_NULL : pointer(w), Float64(nbar),
+                   pointer(cot), pointer(dr), pointer(dw), dnbar)
+    end
+    check(rc)
+    return dr, dw, dnbar[]
+end
+
+function ChainRulesCore.rrule(::typeof(cat2amln), rθϕ, amodes, nbar, win_rhat_ln,
+                              weight=ones(eltype(rθϕ), size(rθϕ, 2)); kw...)
+    anlm = cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight; kw...)
+    function cat2amln_pullback(ȳ)
+        dr, dw, dnbar = cat2amln_vjp(rθϕ, amodes, nbar, ChainRulesCore.unthunk(ȳ), weight; kw...)
+        drθϕ = zeros(size(rθϕ))
+        drθϕ[1, :] .= dr
+        return (ChainRulesCore.NoTangent(), drθϕ, ChainRulesCore.NoTangent(), dnbar,
+                ChainRulesCore.@not_implemented("cotangent of win_rhat_ln"), isempty(weight) ? ChainRulesCore.NoTangent() : dw)
+    end
+    return anlm, cat2amln_pullback
+end
 
 end # module

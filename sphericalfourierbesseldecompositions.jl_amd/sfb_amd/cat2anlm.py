@@ -110,6 +110,35 @@ class Plan:
                                             dp(d_win_a), dp(d_win_b), dp(d_out),
                                             None if stream is None else ctypes.c_void_p(stream)))
 
+    def cat2amln_vjp(self, rtp, nbar, dL_danlm, weight=None):
+        """Vector-Jacobian product of cat2amln at (rθϕ, weight, nbar).
+
+        dL_danlm is the cotangent of the complex a_nlm (dL/dRe a + i dL/dIm a).
+        Returns (dL/dr, dL/dw, dL/dnbar); dL/dθ and dL/dφ are zero because the
+        pixel assignment is piecewise constant.
+        """
+        N, host, w = self._catalogue(rtp, weight)
+        c = np.asarray(dL_danlm, dtype=np.complex128).reshape(-1)
+        if c.size != self.nlmsize:
+            raise ValueError(f"dL_danlm must have {self.nlmsize} entries")
+        cot = L.f64(np.stack([c.real, c.imag], axis=1).reshape(-1))
+        dr = np.zeros(N)
+        dw = np.zeros(N)
+        dnbar = ctypes.c_double()
+        L.check(L.lib().sfb_cat2amln_vjp(self._h, N, L.ptr(host), L.ptr(w), float(nbar), L.ptr(cot), L.ptr(dr),
+                                         L.ptr(dw), ctypes.byref(dnbar)))
+        return dr, dw, dnbar.value
+
+    def cat2amln_vjp_device(self, N, d_rtp, d_w, nbar, d_cot, d_dr, d_dw, stream=None):
+        """sfb_cat2amln_vjp_device: device pointers (ints) or torch CUDA tensors."""
+        def dp(t):
+            if t is None:
+                return None
+            return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
+        L.check(L.lib().sfb_cat2amln_vjp_device(self._h, int(N), dp(d_rtp), dp(d_w), float(nbar), dp(d_cot),
+                                                dp(d_dr), dp(d_dw),
+                                                None if stream is None else ctypes.c_void_p(stream)))
+
     def delta_map(self, rtp, nbar, win_rhat_ln, weight, q):
         N, host, w = self._catalogue(rtp, weight)
         kind, a, b = self._window(win_rhat_ln)
@@ -137,6 +166,11 @@ def get_plan(amodes, niter=3, precision="fp64", device=0) -> Plan:
         p = Plan(amodes, niter, precision, device)
         _plans[key] = p
     return p
+
+
+def cat2amln_vjp(rθϕ, amodes, nbar, dL_danlm, weight=None, *, niter=3, precision="fp64", device=0):
+    """(dL/dr, dL/dw, dL/dnbar) of cat2amln for the cotangent dL_danlm."""
+    return get_plan(amodes, niter, precision, device).cat2amln_vjp(rθϕ, nbar, dL_danlm, weight)
 
 
 def cat2amln(rθϕ, amodes, nbar, win_rhat_ln, weight=None, *, niter=3, precision="fp64", device=0):
