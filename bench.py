@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--cpu-maps", type=int, default=48, help="maps the CPU baseline sample runs")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mocks", type=int, default=64,
+                    help="batched-mock measurement: catalogues with N_b ~ Poisson(mu_b), mu_b ~ U[1e5, 1e6] "
+                         "(0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 512 MiB (torch.neg, FP64) before the steps, as the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -169,6 +172,32 @@ def main():
     adj_dt = (time.perf_counter() - t) / args.steps
     log(f"[adjoint fp64] {adj_dt * 1e3:.2f} ms/step")
 
+    # batched mocks in the same geometry, generated on the device, one
+    # sfb_cat2amln_batch_device call over all of them (FP64)
+    batch = None
+    if args.mocks > 0:
+        rng = np.random.default_rng(3)
+        N_b = rng.poisson(rng.uniform(1e5, 1e6, args.mocks)).astype(np.int64)
+        tot = int(N_b.sum())
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(2)
+        u = torch.rand(3, tot, dtype=torch.float64, device=dev, generator=gen)
+        d_mock = torch.stack([torch.pow(u[0] * (rmax ** 3 - rmin ** 3) + rmin ** 3, 1.0 / 3.0),
+                              torch.acos(1.0 - 2.0 * u[1]), 2.0 * np.pi * u[2]], dim=1).contiguous()
+        del u
+        nbars = N_b / (4 * np.pi / 3 * (rmax ** 3 - rmin ** 3))
+        d_mout = torch.empty(args.mocks * 2 * nlm, dtype=torch.float64, device=dev)
+        plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_mout)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_mout)
+        torch.cuda.synchronize()
+        bdt = time.perf_counter() - t
+        batch = {"mocks": args.mocks, "galaxies": tot, "ms": bdt * 1e3, "ms_per_mock": bdt * 1e3 / args.mocks,
+                 "Mgal_s": tot / bdt / 1e6, "path": "fp64"}
+        log(f"[batch] {args.mocks} mocks, {tot} galaxies: {bdt * 1e3:.1f} ms ({tot / bdt / 1e6:.1f} Mgal/s)")
+        del d_mock, d_mout
+
     head = "mixed" if "mixed" in results else "fp64"
     dt, phases, calls, anlm = results[head]
     work = algorithmic_work(am, N, niter)
@@ -248,6 +277,7 @@ def main():
         "phases_ms": {k: v[0] / max(calls, 1) for k, v in phases.items()},
         "setup_s": {"AnlmModes": t_modes, "plan": t_plan},
         "adjoint_fp64_ms": adj_dt * 1e3,
+        "batched_mocks": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

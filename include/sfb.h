@@ -117,6 +117,23 @@ int sfb_cat2amln_device(sfb_plan* p, int64_t N, const double* d_rtp, const doubl
                         int32_t win_kind, const double* d_win_a, const double* d_win_b,
                         double* d_anlm_out, void* stream);
 
+/* ---- batched mocks ----------------------------------------------------------- */
+
+/* cat2amln of B catalogues through one plan (typically mock catalogues of one survey
+ * geometry): catalogue b has N_b[b] galaxies, stored after catalogue b-1 in rtp (3 x sum N)
+ * and w (sum N, or NULL for unit weights), and its own nbar_b[b]; the window is shared.
+ * anlm_out holds B x 2*nlmsize doubles, catalogue-major.  Equivalent to B calls of
+ * sfb_cat2amln (the reference loops cat2amln over mocks); N_b and nbar_b are host
+ * arrays in both variants.  An error names the failing catalogue. */
+int sfb_cat2amln_batch(sfb_plan* p, int32_t B, const int64_t* N_b, const double* rtp, const double* w,
+                       const double* nbar_b, int32_t win_kind, const double* win_a, const double* win_b,
+                       double* anlm_out);
+
+/* Same with rtp, w, the window and the output in device memory. */
+int sfb_cat2amln_batch_device(sfb_plan* p, int32_t B, const int64_t* N_b, const double* d_rtp, const double* d_w,
+                              const double* nbar_b, int32_t win_kind, const double* d_win_a,
+                              const double* d_win_b, double* d_anlm_out, void* stream);
+
 /* ---- adjoint ----------------------------------------------------------------- */
 
 /* Vector-Jacobian product of cat2amln at (rθϕ, w, nbar): given dL_danlm (2*nlmsize

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "sfb_internal.h"
@@ -2082,6 +2083,67 @@ extern "C" int sfb_cat2amln_vjp(sfb_plan* P, int64_t N, const double* rtp, const
         for (int64_t i = 0; i < N; ++i) acc += (long double)(w ? w[i] : 1.0) * (long double)dw_dst[i];
         *dL_dnbar = (double)(-acc / (long double)nbar);
     }
+    return SFB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched mocks: B catalogues, concatenated, through one plan (its tables and
+// work buffers are reused; each catalogue is a full cat2amln).
+// ---------------------------------------------------------------------------
+extern "C" int sfb_cat2amln_batch_device(sfb_plan* P, int32_t B, const int64_t* N_b, const double* d_rtp,
+                                         const double* d_w, const double* nbar_b, int32_t win_kind,
+                                         const double* d_win_a, const double* d_win_b, double* d_out, void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (B < 0 || (B > 0 && (!N_b || !nbar_b || !d_out))) return fail(SFB_ERR_ARG, "bad batch arguments");
+    int64_t total = 0;
+    for (int32_t b = 0; b < B; ++b) {
+        if (N_b[b] < 0) return fail(SFB_ERR_ARG, "catalogue %d has N < 0", b);
+        total += N_b[b];
+    }
+    if (total > 0 && !d_rtp) return fail(SFB_ERR_ARG, "rtp is NULL");
+    int64_t off = 0;
+    for (int32_t b = 0; b < B; ++b) {
+        const int64_t n = N_b[b];
+        int rc = sfb_cat2amln_device(P, n, n ? d_rtp + 3 * off : nullptr, (d_w && n) ? d_w + off : nullptr, nbar_b[b],
+                                     win_kind, d_win_a, d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, stream);
+        if (rc) {
+            std::string msg = sfb_last_error();
+            return fail(rc, "catalogue %d: %s", b, msg.c_str());
+        }
+        off += n;
+    }
+    return SFB_OK;
+}
+
+extern "C" int sfb_cat2amln_batch(sfb_plan* P, int32_t B, const int64_t* N_b, const double* rtp, const double* w,
+                                  const double* nbar_b, int32_t win_kind, const double* win_a, const double* win_b,
+                                  double* anlm_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (B < 0 || (B > 0 && (!N_b || !nbar_b || !anlm_out))) return fail(SFB_ERR_ARG, "bad batch arguments");
+    int rc = check_window(P, win_kind, win_a, win_b);
+    if (rc) return rc;
+    int64_t total = 0;
+    for (int32_t b = 0; b < B; ++b) {
+        if (N_b[b] < 0) return fail(SFB_ERR_ARG, "catalogue %d has N < 0", b);
+        total += N_b[b];
+    }
+    if (total > 0 && !rtp) return fail(SFB_ERR_ARG, "rtp is NULL");
+    HIP_TRY(hipSetDevice(P->device));
+    const double *d_rtp, *d_w, *d_wa, *d_wb;
+    if ((rc = stage_inputs(P, total, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
+    const size_t out_doubles = 2 * (size_t)P->nlmsize * (size_t)std::max(B, 1);
+    if ((rc = P->h_out.ensure(sizeof(double) * out_doubles))) return rc;
+    rc = sfb_cat2amln_batch_device(P, B, N_b, d_rtp, d_w, nbar_b, win_kind, d_wa, d_wb, P->h_out.as<double>(),
+                                   P->stream);
+    if (rc) return rc;
+    if (B > 0)
+        HIP_TRY(hipMemcpyAsync(anlm_out, P->h_out.p, sizeof(double) * 2 * (size_t)P->nlmsize * (size_t)B,
+                               hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
     return SFB_OK;
 }
 

@@ -110,6 +110,38 @@ class Plan:
                                             dp(d_win_a), dp(d_win_b), dp(d_out),
                                             None if stream is None else ctypes.c_void_p(stream)))
 
+    def cat2amln_batch(self, catalogues, nbars, win_rhat_ln, weights=None):
+        """cat2amln of every catalogue in `catalogues` (3 x N_b arrays) with its own nbar
+        and the shared window; returns a (B, nlmsize) complex array."""
+        B = len(catalogues)
+        nbars = L.f64(np.broadcast_to(np.asarray(nbars, dtype=np.float64), (B,)))
+        parts = [np.asarray(c, dtype=np.float64).reshape(3, -1) for c in catalogues]
+        N_b = np.array([p.shape[1] for p in parts], dtype=np.int64)
+        host = L.f64(np.concatenate([p.T for p in parts], axis=0)) if N_b.sum() else np.zeros(3)
+        w = None
+        if weights is not None:
+            w = L.f64(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in weights]))
+            if w.size != N_b.sum():
+                raise ValueError("weights must have one entry per galaxy of every catalogue")
+        kind, a, b = self._window(win_rhat_ln)
+        out = np.empty(2 * self.nlmsize * max(B, 1))
+        L.check(L.lib().sfb_cat2amln_batch(self._h, B, L.ptr(N_b), L.ptr(host), L.ptr(w), L.ptr(nbars), kind,
+                                           L.ptr(a), L.ptr(b), L.ptr(out)))
+        out = out[:2 * self.nlmsize * B]
+        return (out[0::2] + 1j * out[1::2]).reshape(B, self.nlmsize)
+
+    def cat2amln_batch_device(self, B, N_b, d_rtp, d_w, nbars, win_kind, d_win_a, d_win_b, d_out, stream=None):
+        """sfb_cat2amln_batch_device; N_b and nbars are host sequences."""
+        def dp(t):
+            if t is None:
+                return None
+            return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
+        N_b = np.ascontiguousarray(N_b, dtype=np.int64)
+        nb = L.f64(np.asarray(nbars, dtype=np.float64))
+        L.check(L.lib().sfb_cat2amln_batch_device(self._h, int(B), L.ptr(N_b), dp(d_rtp), dp(d_w), L.ptr(nb),
+                                                  int(win_kind), dp(d_win_a), dp(d_win_b), dp(d_out),
+                                                  None if stream is None else ctypes.c_void_p(stream)))
+
     def cat2amln_vjp(self, rtp, nbar, dL_danlm, weight=None):
         """Vector-Jacobian product of cat2amln at (rθϕ, weight, nbar).
 

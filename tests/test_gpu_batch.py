@@ -1,0 +1,64 @@
+"""Batched mocks (sfb_cat2amln_batch): B catalogues of one survey geometry through
+one plan.  Every mock of the batch must equal its own cat2amln call bit for bit
+(same kernels, same summation order), including an empty catalogue, and one mock is
+checked against the oracle.  An error in one mock names it."""
+import numpy as np
+import pytest
+
+import sfb_amd as S
+from helpers import separable_unit_window, shell_volume, uniform_shell_catalog
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    am = S.AnlmModes(0.05, 500.0, 1000.0, nside=32)
+    win, _ = separable_unit_window(S, am, nr=200)
+    rng = np.random.default_rng(40)
+    sizes = [int(rng.poisson(mu)) for mu in rng.uniform(1e4, 1e5, 4)] + [0]
+    cats = [uniform_shell_catalog(n, 500.0, 1000.0, seed=100 + b) if n else np.zeros((3, 0))
+            for b, n in enumerate(sizes)]
+    weights = [rng.uniform(0.5, 1.5, c.shape[1]) for c in cats]
+    nbars = [max(c.shape[1], 1) / shell_volume(500.0, 1000.0) for c in cats]
+    return am, win, cats, weights, nbars
+
+
+def test_batch_equals_single_calls(setup):
+    am, win, cats, weights, nbars = setup
+    say("batch of", [c.shape[1] for c in cats])
+    plan = S.get_plan(am)
+    batch = plan.cat2amln_batch(cats, nbars, win, weights)
+    assert batch.shape == (len(cats), S.getnlmsize(am))
+    for b, c in enumerate(cats):
+        one = plan.cat2amln(c, nbars[b], win, weights[b] if c.shape[1] else [])
+        assert np.array_equal(batch[b], one), b
+    # unit weights
+    batch1 = plan.cat2amln_batch(cats[:2], nbars[:2], win)
+    assert np.array_equal(batch1[1], plan.cat2amln(cats[1], nbars[1], win))
+
+
+def test_batch_mock_against_oracle(setup):
+    am, win, cats, weights, nbars = setup
+    say("batch mock vs oracle")
+    batch = S.get_plan(am).cat2amln_batch(cats[:2], nbars[:2], win, weights[:2])
+    om = O.OracleModes.__new__(O.OracleModes)
+    om.nside, om.lmax, om.nmax, om.lmax_n = am.nside, am.lmax, am.nmax, am.lmax_n
+    om.knot_off, om.knot_x, om.knot_y, om.knot_ypp = am.knot_off, am.knot_x, am.knot_y, am.knot_ypp
+    ref = O.cat2amln(om, cats[1], nbars[1], 1, win.mask, win.w_ln.reshape(-1, order="F"), w=weights[1])
+    assert np.max(np.abs(batch[1] - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
+def test_batch_error_names_catalogue(setup):
+    am, win, cats, weights, nbars = setup
+    say("batch error path")
+    bad = [w.copy() for w in weights[:3]]
+    bad[2][5] = np.nan
+    with pytest.raises(AssertionError) as e:
+        S.get_plan(am).cat2amln_batch(cats[:3], nbars[:3], win, bad)
+    assert "catalogue 2" in str(e.value)
