@@ -51,3 +51,36 @@ def rel_errors(a, ref, amodes):
         "max_blk": float(blk.max()), "rms_blk": float(np.sqrt(np.mean(blk ** 2))),
         "max_abs_over_max": float(d.max() / np.abs(ref).max()),
     }
+
+
+def gaussian_shell_catalog(N, seed, rmin=500.0, rmax=1500.0, r0=1000.0, sigma=300.0):
+    """N galaxies in the northern hemisphere (cos(theta) in (0, 1]) with radial density
+    proportional to r^2 exp(-(r - r0)^2 / (2 sigma^2)) on [rmin, rmax] (rejection from
+    uniform-in-volume), weights uniform in [0.5, 1.5] rounded to FP32 values."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    r = np.empty(0)
+    while r.size < N:
+        u = rng.random(2 * (N - r.size) + 1000)
+        rr = np.cbrt(u * (rmax ** 3 - rmin ** 3) + rmin ** 3)
+        keep = rng.random(rr.size) < np.exp(-(rr - r0) ** 2 / (2 * sigma ** 2))
+        r = np.concatenate([r, rr[keep]])
+    r = r[:N]
+    costh = 1.0 - rng.random(N)
+    th = np.arccos(costh)
+    ph = 2 * np.pi * rng.random(N)
+    w = rng.uniform(0.5, 1.5, N).astype(np.float32).astype(np.float64)
+    return np.stack([r, th, ph]), w
+
+
+def map_entries(amodes, q0, q1):
+    """Boolean selector of the a_nlm entries of maps q0..q1-1 (map order)."""
+    sel = np.zeros(int(np.sum((amodes.lmax_n + 1) * (amodes.lmax_n + 2) // 2)), dtype=bool)
+    i = 0
+    q = 0
+    for n in range(1, amodes.nmax + 1):
+        for l in range(int(amodes.lmax_n[n - 1]) + 1):
+            if q0 <= q < q1:
+                sel[i:i + l + 1] = True
+            i += l + 1
+            q += 1
+    return sel
