@@ -208,7 +208,7 @@ def main():
     dom = max((k for k in total if k in work), key=lambda k: total[k])
     w = work[dom]
     t_launch = per_launch[dom] * 1e-3
-    if dom in ("analysis", "synthesis") or (dom == "phi" and w["flops"] / w["bytes"] > 10):
+    if dom in ("analysis", "synthesis"):
         peak = FP64_PEAK_TFLOPS if head == "fp64" else FP32_PEAK_TFLOPS
         achieved = w["flops"] / t_launch / 1e12
         roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",

@@ -6,23 +6,27 @@
 //   rocprim       stable radix sort by r, then stable radix sort by pixel
 //                 -> galaxies grouped by pixel, r-ordered inside a pixel, which is
 //                 the reference's summation order (sortout, cat2anlm.jl:50-71)
-//   k_gather      r, w in that order; k_pixel_start: first galaxy of every pixel
-//   k_scatter     delta[q][p] = (sum_{i in p} w_i g_q(r_i)) * 1/(nbar dOmega) - W_q(p)
+//   k_gather      r, w in that order; k_pixel_start: first galaxy of every pixel;
+//                 k_sell_*: sliced layout (one wavefront lane per pixel)
+//   k_seg_t       per knot group (maps with identical knots) and galaxy: spline
+//                 segment i and t = (r - x_i)/h, correctly rounded
+//   k_scatter_it  delta[q][p] = (sum_{i in p} w_i g_q(r_i)) * 1/(nbar dOmega) - W_q(p)
 //                 with the reference's spline formula and rounding (cat2anlm.jl:284-291)
-//   k_phi_dft     per ring pair (north k, south 4 nside-2-k): F_m = sum_j delta_j e^{-i m phi_j},
+//   k_phi_fft / k_phi_polar
+//                 per ring pair (north k, south 4 nside-2-k): F_m = sum_j delta_j e^{-i m phi_j},
 //                 stored as E = F_N + F_S and O = F_N - F_S for m = 0..lmax
 //   Legendre      a_{l m} = dOmega sum_k lambda_lm(theta_k) (E or O by parity of l+m):
 //                 one batched FP64 MFMA GEMM per (m, parity) -- rows = (map, re/im),
 //                 K = ring pairs, columns = l
-//   niter times   synthesis G = a Lambda^T (same GEMM), k_residual R = F - DFT(alm2map(a))
-//                 computed in ring-Fourier space with the exact aliasing of short rings,
-//                 analysis a += dOmega R Lambda  (Healpix map2alm! niter loop,
-//                 src/healpix_helpers.jl:50-56)
+//   niter times   synthesis G = a Lambda^T on the aliasing rings, k_alias: minus the
+//                 DFT of the synthesised short rings' aliases, a <- b + a (I - C0) +
+//                 dOmega R Lambda  (Healpix map2alm! niter loop, src/healpix_helpers.jl:50-56)
 //   k_pack        anlm[getidx(n,l,m)] = a^{(n,l)}_{l m}, m = 0..l, plus a non-finite flag
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -169,48 +173,6 @@ __global__ void k_gather(int64_t N, const uint32_t* __restrict__ idx, const doub
     w_s[k] = w ? w[i] : 1.0;
 }
 
-// Scatter tile of pixel p: chunk c = p / ppc, tile t = (p - c ppc) / PT inside it.
-__device__ __forceinline__ uint32_t tile_of(int64_t p, int64_t ppc, int32_t PT, int64_t tpc)
-{
-    const int64_t c = p / ppc;
-    return (uint32_t)(c * tpc + (p - c * ppc) / PT);
-}
-
-// key[k] = scatter tile of the k-th galaxy in r order.
-__global__ void k_tile_keys(int64_t N, const uint32_t* __restrict__ idx_r, const uint32_t* __restrict__ pix,
-                            int64_t ppc, int32_t PT, int64_t tpc, uint32_t* __restrict__ key)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    key[k] = tile_of(pix[idx_r[k]], ppc, PT, tpc);
-}
-
-// posp[idx_pix[k]] = k: position of every galaxy in pixel order.
-__global__ void k_inverse(int64_t N, const uint32_t* __restrict__ idx_pix, uint32_t* __restrict__ posp)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    posp[idx_pix[k]] = (uint32_t)k;
-}
-
-// Galaxies in (tile, r) order with their slot in the tile's pixel order.
-__global__ void k_gather_tile(int64_t N, const uint32_t* __restrict__ idx_t, const uint32_t* __restrict__ key_t,
-                              const uint32_t* __restrict__ posp, const int32_t* __restrict__ pstart,
-                              const double* __restrict__ rtp, const double* __restrict__ w, int64_t ppc, int32_t PT,
-                              int64_t tpc, double* __restrict__ rt, double* __restrict__ wt,
-                              int32_t* __restrict__ lpos)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const uint32_t i = idx_t[k];
-    const uint32_t T = key_t[k];
-    const int64_t c = T / tpc, t = T - c * tpc;
-    const int64_t p0 = c * ppc + t * PT;  // first pixel of the tile
-    rt[k] = rtp[3 * (int64_t)i];
-    wt[k] = w ? w[i] : 1.0;
-    lpos[k] = (int32_t)posp[i] - pstart[p0];
-}
-
 // pstart[p] = first sorted galaxy with pixel >= p, p = 0..npix.
 __global__ void k_pixel_start(int64_t npix, int64_t N, const uint32_t* __restrict__ pix_sorted,
                               int32_t* __restrict__ pstart)
@@ -272,226 +234,222 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __
     delta[(int64_t)q * npix + p] = __dsub_rn(v, W);
 }
 
-// g_nl(r) from a table staged in LDS: tb[k] = (x_k, y_k, ypp_k, a_k) with
-// a_k = h^2 (ypp_{k+1} - ypp_k)/6 precomputed on the host; b, c, d and t are
-// formed here with the reference's expressions (Splines.jl:327-357).
-__device__ __forceinline__ double spline_lds(const MapInfo& mi, const double4* __restrict__ tb, double r)
+// ---------------------------------------------------------------------------
+// Pixel-sliced catalogue layout for the scatter (sliced ELLPACK): pixels are
+// ordered by galaxy count (descending) inside windows of SELL_SIGMA pixels and
+// cut into slices of 64, one wavefront lane per pixel; slice s stores galaxy j
+// of lane l at soff[s] + 64 j + l, so every lane walks its own pixel's galaxies in
+// (r, input index) order with coalesced loads.
+// ---------------------------------------------------------------------------
+constexpr int SELL_SIGMA = 1024;
+
+__global__ void k_sell_keys(int64_t npix, const int32_t* __restrict__ pstart, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ val)
 {
-    if (r < mi.xfirst) return mi.yfirst;
-    if (r > mi.xlast) return mi.ylast;
-    if (!(r == r)) return r;
-    int i = (int)(__dmul_rn(__dsub_rn(r, mi.xfirst), mi.invdx));
-    i = min(max(i, 0), mi.nk - 2);
-    double4 lo = tb[i];
-    double2 hi = *(const double2*)&tb[i + 1];
-    while (i > 0 && r < lo.x) {
-        --i;
-        lo = tb[i];
-        hi = *(const double2*)&tb[i + 1];
-    }
-    while (i < mi.nk - 2 && r >= hi.x) {
-        ++i;
-        lo = tb[i];
-        hi = *(const double2*)&tb[i + 1];
-    }
-    const double h = __dsub_rn(hi.x, lo.x);
-    const double t = __ddiv_rn(__dsub_rn(r, lo.x), h);
-    const double h2 = __dmul_rn(h, h);
-    const double b = __dmul_rn(__dmul_rn(h2, lo.z), 0.5);
-    const double a = lo.w;
-    const double d = lo.y;
-    const double c = __dsub_rn(__dsub_rn(__dsub_rn(hi.y, a), b), d);
-    return fma(t, fma(t, fma(t, a, b), c), d);
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int32_t cnt = pstart[p + 1] - pstart[p];
+    key[p] = ((uint32_t)(p / SELL_SIGMA) << 16) | (uint32_t)(0xffff - min(cnt, 0xffff));
+    val[p] = (uint32_t)p;
 }
 
-// Four independent evaluations of spline_lds, written out so the LDS reads and
-// the divisions of the four galaxies overlap.
-__device__ __forceinline__ void spline_lds4(const MapInfo& mi, const double4* __restrict__ tb, const double r[4],
-                                            double out[4])
+// slice widths 64 * (largest count among the slice's pixels) and the inverse
+// pixel permutation; one thread per slice
+__global__ void k_sell_slices(int64_t npix, int32_t nslice, const uint32_t* __restrict__ perm,
+                              const int32_t* __restrict__ pstart, int64_t* __restrict__ width,
+                              uint32_t* __restrict__ invperm)
 {
-    int i[4];
-    double4 lo[4];
-    double2 hi[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        int v = (int)(__dmul_rn(__dsub_rn(r[u], mi.xfirst), mi.invdx));
-        i[u] = min(max(v, 0), mi.nk - 2);
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslice) return;
+    const int64_t e0 = 64 * (int64_t)s, e1 = min(npix, e0 + 64);
+    int32_t m = 0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const uint32_t p = perm[e];
+        invperm[p] = (uint32_t)e;
+        m = max(m, pstart[p + 1] - pstart[p]);
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        lo[u] = tb[i[u]];
-        hi[u] = *(const double2*)&tb[i[u] + 1];
+    width[s] = 64 * (int64_t)m;
+}
+
+// exclusive prefix sum of n int64 values in one block of 1024 threads; out[n] = total
+__global__ __launch_bounds__(1024) void k_scan_i64(int32_t n, const int64_t* __restrict__ in, int64_t* __restrict__ out)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    int64_t s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        while (i[u] > 0 && r[u] < lo[u].x) {
-            --i[u];
-            lo[u] = tb[i[u]];
-            hi[u] = *(const double2*)&tb[i[u] + 1];
+    int64_t run = part[t] - s;
+    for (int i = b; i < e; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == 1023) out[n] = part[1023];
+}
+
+// galaxy k (pixel order) -> its SELL position
+__global__ void k_sell_fill(int64_t N, const uint32_t* __restrict__ pix_s, const int32_t* __restrict__ pstart,
+                            const uint32_t* __restrict__ invperm, const int64_t* __restrict__ soff,
+                            const double* __restrict__ r_s, const double* __restrict__ w_s, double* __restrict__ sr,
+                            double* __restrict__ sw)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const uint32_t p = pix_s[k];
+    const int64_t j = k - pstart[p];
+    const uint32_t e = invperm[p];
+    const int64_t dst = soff[e >> 6] + 64 * j + (e & 63);
+    sr[dst] = r_s[k];
+    sw[dst] = w_s[k];
+}
+
+// ---------------------------------------------------------------------------
+// Shared segment search.  Maps whose knot arrays are bit-identical (one "group",
+// ~13 maps at kmax 0.1) share, for every galaxy, the segment i and the parameter
+// t = (r - x_i)/h of the reference's spline evaluation.  k_seg_t computes them once
+// per group and galaxy (in the sliced layout); the scatter then evaluates every map
+// of the group from its 32-byte records (d, c, b, a) with three fma.
+// Out-of-range radii get the pseudo-records nk-1 = (y_first, 0, 0, 0) and
+// nk = (y_last, 0, 0, 0) with t = -0, which return y_first / y_last exactly (the
+// 'boundary' extrapolation, Splines.jl:72-78); a NaN radius gives t = NaN.
+// ---------------------------------------------------------------------------
+struct GroupInfo {
+    int64_t xoff;   // knots of the group in gx / 1/h in grh
+    int32_t nk, pad;
+    double xfirst, xlast, invdx;
+};
+
+// t = RN((r - x_i)/h) as q0 = RN(n y), t = RN(q0 + RN(n - h q0) y), y = RN(1/h):
+// Markstein's final correction step, which returns the correctly rounded quotient
+// (checked against IEEE division on every distinct segment of the tables).
+__global__ void k_seg_t(int64_t T, int32_t ngroup, const double* __restrict__ sr, const GroupInfo* __restrict__ groups,
+                        const double* __restrict__ gx, const double* __restrict__ grh, int32_t* __restrict__ it_i,
+                        double* __restrict__ it_t)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (e >= T) return;
+    const GroupInfo G = groups[g];
+    const double r = sr[e];
+    int i;
+    double t;
+    if (r >= G.xfirst && r <= G.xlast) {
+        const double* x = gx + G.xoff;
+        i = min(max((int)((r - G.xfirst) * G.invdx), 0), G.nk - 2);
+        double x0 = x[i], x1 = x[i + 1];
+        while (i > 0 && r < x0) {
+            --i;
+            x1 = x0;
+            x0 = x[i];
         }
-        while (i[u] < mi.nk - 2 && r[u] >= hi[u].x) {
-            ++i[u];
-            lo[u] = tb[i[u]];
-            hi[u] = *(const double2*)&tb[i[u] + 1];
+        while (i < G.nk - 2 && r >= x1) {
+            ++i;
+            x0 = x1;
+            x1 = x[i + 1];
         }
+        const double h = x1 - x0, n = r - x0, y = grh[G.xoff + i];
+        const double q0 = n * y;
+        t = fma(fma(-h, q0, n), y, q0);
+    } else if (r < G.xfirst) {
+        i = G.nk - 1;
+        t = -0.0;
+    } else if (r > G.xlast) {
+        i = G.nk;
+        t = -0.0;
+    } else {  // NaN
+        i = 0;
+        t = r;
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const double h = __dsub_rn(hi[u].x, lo[u].x);
-        const double t = __ddiv_rn(__dsub_rn(r[u], lo[u].x), h);
-        const double h2 = __dmul_rn(h, h);
-        const double b = __dmul_rn(__dmul_rn(h2, lo[u].z), 0.5);
-        const double a = lo[u].w;
-        const double d = lo[u].y;
-        const double c = __dsub_rn(__dsub_rn(__dsub_rn(hi[u].y, a), b), d);
-        double v = fma(t, fma(t, fma(t, a, b), c), d);
-        // boundary extrapolation and NaN, as in spline_lds
-        v = (r[u] < mi.xfirst) ? mi.yfirst : v;
-        v = (r[u] > mi.xlast) ? mi.ylast : v;
-        out[u] = (r[u] == r[u]) ? v : r[u];
-    }
+    it_i[(int64_t)g * T + e] = i;
+    it_t[(int64_t)g * T + e] = t;
 }
 
-// Orders the LDS accesses of one wavefront: every lane's earlier LDS writes are
-// visible to every lane's later LDS reads.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// One block of the scatter: maps qa and qb (same knot group; qb < 0 if none) on
+// the pixel slices of chunk c.
+struct ScatterBlock {
+    int32_t qa, qb, chunk, pad;
+};
 
-// delta maps with the map's spline table in LDS.  Block (q, chunk) stages the
-// table of map q; then every wavefront independently takes pixel tiles of PT
-// pixels: it evaluates the tile's galaxies (in r order, one per lane) into its own
-// wg[] slice at their pixel-order slots, and every pixel then sums its own
-// galaxies in the reference order.  No block-wide barrier after the staging, so
-// the 16 waves of a CU hide each other's memory latency.  A tile with more than
-// WG galaxies is summed with per-pixel evaluation instead.
-template <bool DENSE>
-__global__ __launch_bounds__(1024) void k_scatter_lds(int64_t npix, int64_t pix_per_chunk, int32_t PT, int32_t WG,
-                                                      int32_t tbl_cap, const int32_t* __restrict__ pstart,
-                                                      const double* __restrict__ r_s, const double* __restrict__ w_s,
-                                                      const double* __restrict__ rt, const double* __restrict__ wt,
-                                                      const int32_t* __restrict__ lpos,
-                                                      const MapInfo* __restrict__ maps,
-                                                      const double4* __restrict__ rec, double inv,
-                                                      const double* __restrict__ win_a,
-                                                      const double* __restrict__ win_b, int64_t lmax,
-                                                      double* __restrict__ delta)
+// delta maps from the sliced layout and the group's (i, t).  Block list order is
+// XCD-aware: workgroup b runs on XCD b mod 8, and blk[x * per_xcd + k] is the k-th
+// block of XCD x; each XCD walks its own pixel chunks group by group, so the
+// group's (i, t), the weights of those chunks and the group's records stay in
+// that XCD's L2.  Two maps per block share every (i, t, w) load.
+template <bool DENSE, bool TWO>
+__global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
+                                                     const ScatterBlock* __restrict__ blk, int32_t per_xcd,
+                                                     int32_t tstride, const int32_t* __restrict__ pstart,
+                                                     const uint32_t* __restrict__ perm,
+                                                     const int64_t* __restrict__ soff, const double* __restrict__ sw,
+                                                     const int32_t* __restrict__ it_i, const double* __restrict__ it_t,
+                                                     const MapInfo* __restrict__ maps,
+                                                     const double4* __restrict__ rec4, double inv,
+                                                     const double* __restrict__ win_a,
+                                                     const double* __restrict__ win_b, int64_t lmax,
+                                                     double* __restrict__ delta)
 {
     extern __shared__ double4 sm4[];
-    const int q = blockIdx.x;
-    const MapInfo mi = maps[q];
-    double4* tb = sm4;
-    for (int k = threadIdx.x; k < mi.nk; k += blockDim.x) tb[k] = rec[mi.koff + k];
+    const ScatterBlock B = blk[(int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3)];
+    const bool two = TWO && B.qb >= 0;
+    const MapInfo ma = maps[B.qa];
+    const MapInfo mb = maps[two ? B.qb : B.qa];
+    for (int k = threadIdx.x; k <= ma.nk; k += blockDim.x) sm4[k] = rec4[ma.koff + B.qa + k];
+    if (two)
+        for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) sm4[tstride + k] = rec4[mb.koff + B.qb + k];
+    const int32_t* gi = it_i + (int64_t)ma.pad * T;
+    const double* gt = it_t + (int64_t)ma.pad * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    double* wg = (double*)(sm4 + tbl_cap) + (int64_t)wave * WG;
-    const int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
-    const double wcol = DENSE ? 0.0 : win_b[col];
-    const int64_t pb = (int64_t)blockIdx.y * pix_per_chunk;
-    const int64_t pe = min(npix, pb + pix_per_chunk);
-    const int ntile = (int)((pe - pb + PT - 1) / PT);
-    __syncthreads();  // table staged
-    for (int t = wave; t < ntile; t += nwave) {
-        const int64_t p0 = pb + (int64_t)t * PT;
-        const int64_t p1 = min(p0 + (int64_t)PT, pe);
-        const int32_t G0 = pstart[p0], ng = pstart[p1] - G0;
-        const bool coop = ng <= WG;
-        if (coop) {
-            for (int j0 = lane; j0 < ng; j0 += 4 * 64) {
-                double rr[4], ww[4], vv[4];
-                int32_t lp[4];
+    const int64_t cola = ma.l + (lmax + 1) * (int64_t)(ma.n - 1);
+    const int64_t colb = mb.l + (lmax + 1) * (int64_t)(mb.n - 1);
+    const double wcola = DENSE ? 0.0 : win_b[cola];
+    const double wcolb = DENSE ? 0.0 : win_b[colb];
+    const int s0 = B.chunk * spc, s1 = min(nslice, s0 + spc);
+    __syncthreads();  // records staged
+    for (int s = s0 + wave; s < s1; s += nwave) {
+        const int64_t e = (int64_t)s * 64 + lane;
+        const bool valid = e < npix;
+        const uint32_t p = valid ? perm[e] : 0u;
+        const int32_t cnt = valid ? pstart[p + 1] - pstart[p] : 0;
+        const int64_t base = soff[s] + lane;
+        double acca = 0.0, accb = 0.0;
+        for (int j0 = 0; j0 < cnt; j0 += 4) {
+            double tt[4], ww[4];
+            int ii[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = min(j0 + u * 64, ng - 1);
-                    rr[u] = rt[G0 + j];
-                    ww[u] = wt[G0 + j];
-                    lp[u] = lpos[G0 + j];
+            for (int u = 0; u < 4; ++u) {
+                const int64_t o = base + 64 * (int64_t)min(j0 + u, cnt - 1);
+                ii[u] = gi[o];
+                tt[u] = gt[o];
+                ww[u] = sw[o];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double4 c = sm4[ii[u]];  // (d, c, b, a)
+                const double v = fma(tt[u], fma(tt[u], fma(tt[u], c.w, c.z), c.y), c.x);
+                if (j0 + u < cnt) acca = __dadd_rn(acca, __dmul_rn(ww[u], v));
+                if (two) {
+                    const double4 c2 = sm4[tstride + ii[u]];
+                    const double v2 = fma(tt[u], fma(tt[u], fma(tt[u], c2.w, c2.z), c2.y), c2.x);
+                    if (j0 + u < cnt) accb = __dadd_rn(accb, __dmul_rn(ww[u], v2));
                 }
-                spline_lds4(mi, tb, rr, vv);
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (j0 + u * 64 < ng) wg[lp[u]] = __dmul_rn(ww[u], vv[u]);
             }
-            wave_lds_sync();
         }
-        for (int64_t p = p0 + lane; p < p1; p += 64) {
-            const int32_t a = pstart[p], b = pstart[p + 1];
-            double s = 0.0;
-            if (coop) {
-                for (int32_t j = a - G0; j < b - G0; ++j) s = __dadd_rn(s, wg[j]);
-            } else {
-                for (int32_t g = a; g < b; ++g) s = __dadd_rn(s, __dmul_rn(w_s[g], spline_lds(mi, tb, r_s[g])));
+        if (valid) {
+            const double Wa = DENSE ? win_a[p + npix * cola] : __dmul_rn(win_a[p], wcola);
+            delta[(int64_t)B.qa * npix + p] = __dsub_rn(__dmul_rn(acca, inv), Wa);
+            if (two) {
+                const double Wb = DENSE ? win_a[p + npix * colb] : __dmul_rn(win_a[p], wcolb);
+                delta[(int64_t)B.qb * npix + p] = __dsub_rn(__dmul_rn(accb, inv), Wb);
             }
-            const double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], wcol);
-            delta[(int64_t)q * npix + p] = __dsub_rn(__dmul_rn(s, inv), W);
-        }
-        wave_lds_sync();  // the slice is rewritten by the next tile
-    }
-}
-
-// ---------------------------------------------------------------------------
-// phi-transform of one ring pair for TQ maps.  X[(m*2+par)*K + k][i], i = 2q+c.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_phi_dft(const double* __restrict__ delta, int64_t npix, int32_t nq,
-                                                 int64_t M2, int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                 const double2* __restrict__ tw, double* __restrict__ X, int32_t TQ)
-{
-    extern __shared__ double sm[];
-    const int k = blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
-    const RingDev R = rings[k];
-    const int npr = R.npr;
-    const int ld = npr + 1;
-    const bool eq = (k == K - 1);
-    double* dN = sm;
-    double* dS = sm + TQ * ld;
-    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
-        int ql = e / npr, j = e - ql * npr;
-        int q = q0 + ql;
-        double vn = 0.0, vs = 0.0;
-        if (q < nq) {
-            vn = delta[(int64_t)q * npix + R.start_n + j];
-            if (!eq) vs = delta[(int64_t)q * npix + R.start_s + j];
-        }
-        dN[ql * ld + j] = vn;
-        dS[ql * ld + j] = vs;
-    }
-    __syncthreads();
-    const double2* twr = tw + R.tw_off;
-    const int n2 = 2 * npr;
-    for (int e = threadIdx.x; e < (L + 1) * TQ; e += blockDim.x) {
-        int m = e / TQ, ql = e - m * TQ;
-        int q = q0 + ql;
-        if (q >= nq) continue;
-        int kk = (int)(((int64_t)m * R.shift) % n2);
-        int dk = (int)(((int64_t)2 * m) % n2);
-        double fnr = 0, fni = 0, fsr = 0, fsi = 0;
-        const double* an = dN + ql * ld;
-        const double* as = dS + ql * ld;
-        for (int j = 0; j < npr; ++j) {
-            double2 c = twr[kk];
-            double vn = an[j], vs = as[j];
-            fnr = fma(vn, c.x, fnr);
-            fni = fma(-vn, c.y, fni);
-            fsr = fma(vs, c.x, fsr);
-            fsi = fma(-vs, c.y, fsi);
-            kk += dk;
-            if (kk >= n2) kk -= n2;
-        }
-        int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
-        int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
-        if (eq) {
-            X[be] = fnr;
-            X[be + 1] = fni;
-            X[bo] = fnr;
-            X[bo + 1] = fni;
-        } else {
-            X[be] = fnr + fsr;
-            X[be + 1] = fni + fsi;
-            X[bo] = fnr - fsr;
-            X[bo + 1] = fni - fsi;
         }
     }
 }
@@ -656,76 +614,6 @@ __global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ de
         if (q0 + qa + 1 < nq)
             store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa + 1, a1r, a1i, b1r, b1i);
     }
-}
-
-// ---------------------------------------------------------------------------
-// Batched FP64 GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta C[n][i]
-// for rows i < rows.  Block tile 64(i) x 64(n), k-tile 16, 4 waves of 32 x 32.
-// v_mfma_f64_16x16x4f64 operand layout: lane l holds A[row l%16][k l/16],
-// B[k l/16][col l%16]; result register v of lane l is D[l/16 + 4v][l%16].
-// The MFMA "rows" are the output columns n so the stores run along i.
-// ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16;
-
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
-                                                  double beta)
-{
-    const GemmDesc d = descs[blockIdx.z];
-    const int i0 = blockIdx.x * GBM;
-    const int n0 = blockIdx.y * GBN;
-    if (n0 >= d.N) return;
-    __shared__ double As[GBK][GBM + 1];
-    __shared__ double Bs[GBK][GBN + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave & 1) * 32;
-    const int wn = (wave >> 1) * 32;
-    double4_t acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < d.K; k0 += GBK) {
-        for (int e = tid; e < GBK * GBM; e += 256) {
-            int kk = e / GBM, ii = e - kk * GBM;
-            int gk = k0 + kk, gi = i0 + ii;
-            As[kk][ii] = (gk < d.K && gi < rows) ? d.A[(int64_t)gk * d.lda + gi] : 0.0;
-        }
-        for (int e = tid; e < GBK * GBN; e += 256) {
-            int kk = e / GBN, nn = e - kk * GBN;
-            int gk = k0 + kk, gn = n0 + nn;
-            Bs[kk][nn] = (gk < d.K && gn < d.N) ? d.B[(int64_t)gk * d.ldb + gn] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GBK; ks += 4) {
-            const int kk = ks + (lane >> 4);
-            double fa[2], fb[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                fa[t] = Bs[kk][wn + t * 16 + (lane & 15)];
-                fb[t] = As[kk][wi + t * 16 + (lane & 15)];
-            }
-#pragma unroll
-            for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-                    acc[tn][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[tn], fb[ti], acc[tn][ti], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int n = n0 + wn + tn * 16 + (lane >> 4) + 4 * v;
-                int i = i0 + wi + ti * 16 + (lane & 15);
-                if (n < d.N && i < rows) {
-                    const int64_t o = (int64_t)n * d.ldc + i;
-                    double val = alpha * acc[tn][ti][v];
-                    if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
-                    d.C[o] = val;
-                }
-            }
 }
 
 // Tall-skinny GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta Cin[n][i],
@@ -1198,7 +1086,7 @@ struct sfb_plan {
     std::vector<int32_t> nb;  // per (m, par): number of l
     int64_t a_size = 0, x_size = 0, lam_size = 0;
     // tables
-    DevBuf d_rec;  // per knot (x, y, ypp, a) for the LDS scatter
+    DevBuf d_rec;   // per knot (x, y, ypp, a): the adjoint's spline value and derivative
     int32_t tbl_cap = 0;
     int64_t lds_max = 65536;
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
@@ -1215,11 +1103,16 @@ struct sfb_plan {
     DevBuf h_rtp, h_w, h_wa, h_wb, h_out;
     int64_t cat_cap = -1;
     int64_t ncat = 0;
-    // scatter tiling of the current call (LDS path)
-    bool sc_lds = false;
-    int32_t sc_WG = 0, sc_PT = 0;
-    int64_t sc_ppc = 0, sc_nchunk = 0, sc_tpc = 0;
-    DevBuf d_rt, d_wt, d_lpos, d_posp, d_keyt, d_keyt2, d_idxt;
+    // sliced (SELL) catalogue layout; sell_ok when a map's records fit in LDS
+    bool sell_ok = false, sell_empty = false;
+    int32_t sell_nslice = 0;
+    int64_t sell_total = 0;
+    DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
+    // knot groups (maps with bit-identical knots) and the shared segment search
+    int32_t ngroup = 0, nk_max_group = 0;
+    DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk;
+    int32_t sc_nchunk_it = 0, sc_per_xcd = 0;
+    bool sc_two = false;
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[SFB_NPHASE];
@@ -1338,6 +1231,107 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     int rc;
     if ((rc = P->d_rec.ensure(sizeof(double) * rec.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_rec.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+    // knot groups: maps with bit-identical knot arrays, in order of first appearance
+    {
+        std::vector<GroupInfo> groups;
+        std::vector<double> gx, grh, rec4;
+        std::vector<int32_t> gid((size_t)P->nmaps);
+        rec4.reserve((size_t)(knot_off[P->nmaps] + P->nmaps) * 4);
+        for (int64_t qq = 0; qq < P->nmaps; ++qq) {
+            const int64_t o = knot_off[qq], nk = knot_off[qq + 1] - knot_off[qq];
+            int32_t g = -1;
+            for (size_t gg = 0; gg < groups.size(); ++gg)
+                if (groups[gg].nk == nk &&
+                    std::memcmp(gx.data() + groups[gg].xoff, kx + o, sizeof(double) * (size_t)nk) == 0) {
+                    g = (int32_t)gg;
+                    break;
+                }
+            if (g < 0) {
+                GroupInfo G{};
+                G.xoff = (int64_t)gx.size();
+                G.nk = (int32_t)nk;
+                G.xfirst = kx[o];
+                G.xlast = kx[o + nk - 1];
+                G.invdx = (double)(nk - 1) / (kx[o + nk - 1] - kx[o]);
+                for (int64_t i = 0; i < nk; ++i) {
+                    gx.push_back(kx[o + i]);
+                    grh.push_back(i + 1 < nk ? 1.0 / (kx[o + i + 1] - kx[o + i]) : 0.0);
+                }
+                g = (int32_t)groups.size();
+                groups.push_back(G);
+            }
+            gid[(size_t)qq] = g;
+            P->maps[(size_t)qq].pad = g;
+            // (d, c, b, a) per segment with the reference's expressions, then the
+            // two boundary pseudo-records
+            for (int64_t i = 0; i + 1 < nk; ++i) {
+                const double h = kx[o + i + 1] - kx[o + i];
+                const double h2 = h * h;
+                const double b = h2 * kypp[o + i] / 2.0;
+                const double a = h2 * (kypp[o + i + 1] - kypp[o + i]) / 6.0;
+                const double d = ky[o + i];
+                const double c = ky[o + i + 1] - a - b - d;
+                rec4.push_back(d);
+                rec4.push_back(c);
+                rec4.push_back(b);
+                rec4.push_back(a);
+            }
+            for (int side = 0; side < 2; ++side) {
+                rec4.push_back(side ? ky[o + nk - 1] : ky[o]);
+                rec4.push_back(0.0);
+                rec4.push_back(0.0);
+                rec4.push_back(0.0);
+            }
+        }
+        // blocks visit the maps group by group, so a group's (i, t) stays in cache
+        std::vector<int32_t> qorder((size_t)P->nmaps);
+        for (int64_t qq = 0; qq < P->nmaps; ++qq) qorder[(size_t)qq] = (int32_t)qq;
+        std::stable_sort(qorder.begin(), qorder.end(),
+                         [&](int32_t x, int32_t y) { return gid[(size_t)x] < gid[(size_t)y]; });
+        P->ngroup = (int32_t)groups.size();
+        if ((rc = P->d_groups.ensure(sizeof(GroupInfo) * groups.size())) ||
+            (rc = P->d_gx.ensure(sizeof(double) * gx.size())) || (rc = P->d_grh.ensure(sizeof(double) * grh.size())) ||
+            (rc = P->d_rec4.ensure(sizeof(double) * rec4.size())) ||
+            (rc = P->d_qorder.ensure(sizeof(int32_t) * qorder.size())))
+            return rc;
+        HIP_TRY(hipMemcpy(P->d_groups.p, groups.data(), sizeof(GroupInfo) * groups.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_gx.p, gx.data(), sizeof(double) * gx.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_grh.p, grh.data(), sizeof(double) * grh.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_rec4.p, rec4.data(), sizeof(double) * rec4.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_qorder.p, qorder.data(), sizeof(int32_t) * qorder.size(), hipMemcpyHostToDevice));
+        // scatter block list: per XCD, its pixel chunks, group by group, maps in
+        // pairs of the same group when two record tables fit in LDS
+        P->sell_ok = (int64_t)(P->tbl_cap + 1) * 32 <= P->lds_max;
+        P->sc_two = (int64_t)2 * (P->tbl_cap + 1) * 32 <= P->lds_max;
+        const int64_t nslice = (P->npix + 63) / 64;
+        int32_t nchunk = 8;
+        while (nchunk < 128 && nslice / (nchunk * 2) >= 32) nchunk *= 2;
+        P->sc_nchunk_it = nchunk;
+        std::vector<std::vector<int32_t>> members(groups.size());
+        for (int32_t qq : qorder) members[(size_t)gid[(size_t)qq]].push_back(qq);
+        std::vector<ScatterBlock> per[8];
+        for (int x = 0; x < 8; ++x)
+            for (size_t g = 0; g < members.size(); ++g)
+                for (int32_t c = x; c < nchunk; c += 8) {
+                    const auto& mm = members[g];
+                    const size_t step = P->sc_two ? 2 : 1;
+                    for (size_t k = 0; k < mm.size(); k += step) {
+                        ScatterBlock b{};
+                        b.qa = mm[k];
+                        b.qb = (P->sc_two && k + 1 < mm.size()) ? mm[k + 1] : -1;
+                        b.chunk = c;
+                        per[x].push_back(b);
+                    }
+                }
+        P->sc_per_xcd = (int32_t)per[0].size();
+        std::vector<ScatterBlock> all;
+        for (int x = 0; x < 8; ++x) {
+            if ((int32_t)per[x].size() != P->sc_per_xcd) return fail(SFB_ERR_ARG, "internal: uneven scatter blocks");
+            all.insert(all.end(), per[x].begin(), per[x].end());
+        }
+        if ((rc = P->d_sblk.ensure(sizeof(ScatterBlock) * all.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_sblk.p, all.data(), sizeof(ScatterBlock) * all.size(), hipMemcpyHostToDevice));
+    }
     if ((rc = P->d_maps.ensure(sizeof(MapInfo) * P->maps.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_maps.p, P->maps.data(), sizeof(MapInfo) * P->maps.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_kx.ensure(sizeof(double) * (size_t)knot_off[P->nmaps]))) return rc;
@@ -1619,12 +1613,15 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     prof_collect(P);
-    DevBuf* bufs[] = {&P->d_rt, &P->d_wt, &P->d_lpos, &P->d_posp, &P->d_keyt, &P->d_keyt2, &P->d_idxt, &P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
+    DevBuf* bufs[] = {&P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
                       &P->h_rtp, &P->h_w, &P->h_wa, &P->h_wb, &P->h_out,
-                      &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw};
+                      &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw,
+                      &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
+                      &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
+                      &P->d_it_i, &P->d_it_t, &P->d_sblk};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -1649,32 +1646,13 @@ extern "C" int sfb_plan_set_precision(sfb_plan* P, int32_t precision)
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Tiling of the LDS scatter for a catalogue of N galaxies: sc_lds is false when
-// the largest spline table does not fit next to a 1024-entry galaxy buffer.
-static void scatter_geometry(sfb_plan* P, int64_t N)
-{
-    const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
-    const int64_t wg_room = (P->lds_max - tbl_bytes) / 8;
-    P->sc_lds = wg_room >= 16 * 256;
-    if (!P->sc_lds) return;
-    // 16 wavefronts, each with a slice of WG galaxy slots; a tile is sized so its
-    // expected galaxy count is ~85% of WG
-    P->sc_WG = (int32_t)std::min<int64_t>(1024, wg_room / 16);
-    const double per_pix = std::max(1.0, (double)N / (double)P->npix);
-    P->sc_PT = (int32_t)std::max<int64_t>(8, std::min<int64_t>(1024, (int64_t)(0.85 * P->sc_WG / per_pix)));
-    P->sc_nchunk = std::max<int64_t>(1, std::min<int64_t>(8, P->npix / P->sc_PT));
-    P->sc_ppc = (P->npix + P->sc_nchunk - 1) / P->sc_nchunk;
-    P->sc_tpc = (P->sc_ppc + P->sc_PT - 1) / P->sc_PT;
-}
-
 // Sort the catalogue and build the pixel segments.  Leaves r_s, w_s, pstart and,
-// for the LDS scatter, the (tile, r)-ordered rt, wt, lpos.
+// when the records fit in LDS, the sliced layout sr, sw (soff, perm).
 static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, hipStream_t st)
 {
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
     P->ncat = N;
-    scatter_geometry(P, N);
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
     LAUNCH_CHECK();
@@ -1683,8 +1661,10 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
                            P->npix + 1, 0);
         LAUNCH_CHECK();
         if ((rc = P->d_rs.ensure(8)) || (rc = P->d_ws.ensure(8))) return rc;
+        P->sell_empty = true;  // the scatter takes the per-pixel kernel: delta = -W
         return SFB_OK;
     }
+    P->sell_empty = false;
     size_t n = (size_t)N;
     if ((rc = P->d_keyr.ensure(8 * n)) || (rc = P->d_keyr2.ensure(8 * n)) || (rc = P->d_idx.ensure(4 * n)) ||
         (rc = P->d_idx2.ensure(4 * n)) || (rc = P->d_pix.ensure(4 * n)) || (rc = P->d_pixkey.ensure(4 * n)) ||
@@ -1702,18 +1682,27 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     while (((int64_t)1 << pbits) < P->npix) ++pbits;
     HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp2, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
                                       P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
-    size_t tmp3 = 0;
-    if (P->sc_lds) {
-        const int64_t ntiles = P->sc_nchunk * P->sc_tpc;
-        unsigned tbits = 1;
-        while (((int64_t)1 << tbits) < ntiles) ++tbits;
-        HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp3, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
-                                          P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, tbits,
-                                          st));
+    // the sliced layout and the shared segment search need a map's records in LDS;
+    // otherwise the scatter evaluates per pixel from global memory
+    const bool sell = P->sell_ok;
+    P->sell_nslice = (int32_t)((P->npix + 63) / 64);
+    unsigned wbits = 1;
+    while (((int64_t)1 << wbits) <= P->npix / SELL_SIGMA) ++wbits;
+    size_t tmp4 = 0;
+    if (sell) {
+        if ((rc = P->d_skey.ensure(4 * (size_t)P->npix)) || (rc = P->d_skey2.ensure(4 * (size_t)P->npix)) ||
+            (rc = P->d_sval.ensure(4 * (size_t)P->npix)) || (rc = P->d_perm.ensure(4 * (size_t)P->npix)) ||
+            (rc = P->d_invperm.ensure(4 * (size_t)P->npix)) ||
+            (rc = P->d_swidth.ensure(8 * (size_t)P->sell_nslice)) ||
+            (rc = P->d_soff.ensure(8 * (size_t)(P->sell_nslice + 1))))
+            return rc;
+        HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp4, P->d_skey.as<uint32_t>(), P->d_skey2.as<uint32_t>(),
+                                          P->d_sval.as<uint32_t>(), P->d_perm.as<uint32_t>(), (unsigned)P->npix, 0,
+                                          16 + wbits, st));
     }
     // every temporary size is known before the first sort is enqueued: the buffer
     // is never reallocated under work in flight
-    if ((rc = P->d_sorttmp.ensure(std::max(std::max(tmp1, tmp2), tmp3)))) return rc;
+    if ((rc = P->d_sorttmp.ensure(std::max(std::max(tmp1, tmp2), tmp4)))) return rc;
     tmp1 = P->d_sorttmp.bytes;
     HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
                                       P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
@@ -1730,31 +1719,33 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     hipLaunchKernelGGL(k_pixel_start, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->npix, N,
                        P->d_pixkey2.as<uint32_t>(), P->d_pstart.as<int32_t>());
     LAUNCH_CHECK();
-    if (P->sc_lds) {
-        // 3) stable sort of the r-ordered list by scatter tile: (tile, r, index) order
-        if ((rc = P->d_rt.ensure(8 * n)) || (rc = P->d_wt.ensure(8 * n)) || (rc = P->d_lpos.ensure(4 * n)) ||
-            (rc = P->d_posp.ensure(4 * n)) || (rc = P->d_keyt.ensure(4 * n)) || (rc = P->d_keyt2.ensure(4 * n)) ||
-            (rc = P->d_idxt.ensure(4 * n)))
+    if (sell) {
+        // 3) pixel slices: pixels by count inside windows, slice widths, offsets
+        hipLaunchKernelGGL(k_sell_keys, dim3(nblk(P->npix, 256)), dim3(256), 0, st, P->npix,
+                           P->d_pstart.as<int32_t>(), P->d_skey.as<uint32_t>(), P->d_sval.as<uint32_t>());
+        LAUNCH_CHECK();
+        tmp4 = P->d_sorttmp.bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp4, P->d_skey.as<uint32_t>(), P->d_skey2.as<uint32_t>(),
+                                          P->d_sval.as<uint32_t>(), P->d_perm.as<uint32_t>(), (unsigned)P->npix, 0,
+                                          16 + wbits, st));
+        hipLaunchKernelGGL(k_sell_slices, dim3(nblk(P->sell_nslice, 256)), dim3(256), 0, st, P->npix, P->sell_nslice,
+                           P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_swidth.as<int64_t>(),
+                           P->d_invperm.as<uint32_t>());
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, P->sell_nslice, P->d_swidth.as<int64_t>(),
+                           P->d_soff.as<int64_t>());
+        LAUNCH_CHECK();
+        int64_t total = 0;
+        HIP_TRY(hipMemcpyAsync(&total, P->d_soff.as<int64_t>() + P->sell_nslice, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        P->sell_total = total;
+        if ((rc = P->d_sr.ensure(8 * (size_t)std::max<int64_t>(total, 1))) ||
+            (rc = P->d_sw.ensure(8 * (size_t)std::max<int64_t>(total, 1))))
             return rc;
-        const int64_t ntiles = P->sc_nchunk * P->sc_tpc;
-        unsigned tbits = 1;
-        while (((int64_t)1 << tbits) < ntiles) ++tbits;
-        // sort 1 left the r order in idx2; it was consumed as the input of sort 2, which
-        // does not modify its input buffers, so idx2 still holds the r order
-        hipLaunchKernelGGL(k_tile_keys, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx2.as<uint32_t>(),
-                           P->d_pix.as<uint32_t>(), P->sc_ppc, P->sc_PT, P->sc_tpc, P->d_keyt.as<uint32_t>());
-        LAUNCH_CHECK();
-        tmp3 = P->d_sorttmp.bytes;
-        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp3, P->d_keyt.as<uint32_t>(), P->d_keyt2.as<uint32_t>(),
-                                          P->d_idx2.as<uint32_t>(), P->d_idxt.as<uint32_t>(), (unsigned)N, 0, tbits,
-                                          st));
-        hipLaunchKernelGGL(k_inverse, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx.as<uint32_t>(),
-                           P->d_posp.as<uint32_t>());
-        LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gather_tile, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idxt.as<uint32_t>(),
-                           P->d_keyt2.as<uint32_t>(), P->d_posp.as<uint32_t>(), P->d_pstart.as<int32_t>(), d_rtp, d_w,
-                           P->sc_ppc, P->sc_PT, P->sc_tpc, P->d_rt.as<double>(), P->d_wt.as<double>(),
-                           P->d_lpos.as<int32_t>());
+        hipLaunchKernelGGL(k_sell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_pixkey2.as<uint32_t>(),
+                           P->d_pstart.as<int32_t>(), P->d_invperm.as<uint32_t>(), P->d_soff.as<int64_t>(),
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_sr.as<double>(), P->d_sw.as<double>());
         LAUNCH_CHECK();
     }
     return SFB_OK;
@@ -1766,30 +1757,43 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
     PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
-    if (P->sc_lds) {
-        const int64_t tbl_bytes = (int64_t)P->tbl_cap * 32;
-        const int32_t WG = P->sc_WG, PT = P->sc_PT;
-        const int64_t ppc = P->sc_ppc;
-        const size_t shm = (size_t)tbl_bytes + 16 * 8 * (size_t)WG;
-        dim3 g2((unsigned)P->nmaps, (unsigned)P->sc_nchunk);
-        const double* rt = P->ncat > 0 ? P->d_rt.as<double>() : P->d_rs.as<double>();
-        const double* wt = P->ncat > 0 ? P->d_wt.as<double>() : P->d_ws.as<double>();
-        const int32_t* lp = P->ncat > 0 ? P->d_lpos.as<int32_t>() : P->d_pstart.as<int32_t>();
+    if (P->sell_ok && !P->sell_empty) {
+        // shared segment search per knot group, then three fma per map and galaxy
+        const int64_t T = std::max<int64_t>(P->sell_total, 1);
+        int rc;
+        if ((rc = P->d_it_i.ensure(sizeof(int32_t) * (size_t)(T * P->ngroup))) ||
+            (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
+            return rc;
+        hipLaunchKernelGGL(k_seg_t, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T, P->ngroup,
+                           P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
+                           P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<double>());
+        LAUNCH_CHECK();
+        const int32_t tstride = P->tbl_cap + 1;
+        const size_t shm = (size_t)tstride * 32 * (P->sc_two ? 2 : 1);
+        const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
+        const dim3 g1((unsigned)(8 * P->sc_per_xcd));
+#define SFB_LAUNCH_IT(D, TW)                                                                                      \
+    do {                                                                                                          \
+        HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_it<D, TW>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                    (int)shm));                                                                   \
+        hipLaunchKernelGGL((k_scatter_it<D, TW>), g1, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,        \
+                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_pstart.as<int32_t>(),       \
+                           P->d_perm.as<uint32_t>(), P->d_soff.as<int64_t>(), P->d_sw.as<double>(),               \
+                           P->d_it_i.as<int32_t>(), P->d_it_t.as<double>(), P->d_maps.as<MapInfo>(),              \
+                           P->d_rec4.as<double4>(), inv, d_wa, d_wb, P->lmax, P->d_delta.as<double>());           \
+    } while (0)
         if (win_kind == SFB_WIN_DENSE) {
-            HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-            hipLaunchKernelGGL(k_scatter_lds<true>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
-                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(), rt, wt, lp,
-                               P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
-                               P->d_delta.as<double>());
+            if (P->sc_two)
+                SFB_LAUNCH_IT(true, true);
+            else
+                SFB_LAUNCH_IT(true, false);
         } else {
-            HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_lds<false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-            hipLaunchKernelGGL(k_scatter_lds<false>, g2, dim3(1024), shm, st, P->npix, ppc, PT, WG, P->tbl_cap,
-                               P->d_pstart.as<int32_t>(), P->d_rs.as<double>(), P->d_ws.as<double>(), rt, wt, lp,
-                               P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), inv, d_wa, d_wb, P->lmax,
-                               P->d_delta.as<double>());
+            if (P->sc_two)
+                SFB_LAUNCH_IT(false, true);
+            else
+                SFB_LAUNCH_IT(false, false);
         }
+#undef SFB_LAUNCH_IT
         LAUNCH_CHECK();
         return SFB_OK;
     }

@@ -72,6 +72,27 @@ def test_delta_map_bitexact(c1):
         assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
 
 
+def test_delta_map_crowded_pixels_bitexact():
+    """Pixels far above the mean count (70000 galaxies in one pixel, 3000 in another),
+    past the count clamp of the scatter's pixel ordering, plus out-of-range radii."""
+    am = S.AnlmModes(0.03, 500.0, 1000.0, nside=8)
+    win, _ = separable_unit_window(S, am, nr=100)
+    rng = np.random.default_rng(21)
+    base = uniform_shell_catalog(5000, 500.0, 1000.0, seed=22)
+    crowd = np.stack([rng.uniform(500.0, 1000.0, 70000), np.full(70000, 0.3), np.full(70000, 1.1)])
+    crowd2 = np.stack([rng.uniform(400.0, 1100.0, 3000), np.full(3000, 2.0), np.full(3000, 4.0)])
+    cat = np.concatenate([base, crowd, crowd2], axis=1)
+    w = rng.uniform(0.5, 1.5, cat.shape[1])
+    nbar = cat.shape[1] / shell_volume(500.0, 1000.0)
+    plan = S.get_plan(am)
+    om = _oracle_modes_from(am)
+    for q in (0, am.nmaps // 2, am.nmaps - 1):
+        say("crowded delta map", q)
+        g = plan.delta_map(cat, nbar, win, w, q)
+        c = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), w, q)
+        assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
+
+
 def test_delta_map_weights_dense_bitexact(c1):
     am, win, cat, nbar = c1
     rng = np.random.default_rng(5)
