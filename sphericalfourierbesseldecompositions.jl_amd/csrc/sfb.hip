@@ -616,6 +616,110 @@ __global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ de
     }
 }
 
+// Polar ring pairs (npr = 4i, i < nside): one radix-4 decimation-in-time step,
+// X_kf = sum_{s<4} W^{kf s} Y_s[kf mod i], W = e^{-2 pi i/npr}, with Y_s the length-i
+// DFT of the decimated samples delta[4j + s], computed directly for u <= i/2
+// (real input: Y_s[i - u] = conj Y_s[u]).  A block takes ring pair k and TQ maps
+// (2 TQ sequences, north and south); each Y work item accumulates one sequence x
+// 2 frequencies x 4 phases (16 fma per 64 bytes of LDS).  Y overwrites the staged
+// samples (row stride npr + 8 doubles holds 4 (i/2 + 1) complex values).  The
+// LDS footprint stays ~35 KB so that four blocks share a CU.
+__global__ __launch_bounds__(256) void k_phi_polar4(const double* __restrict__ delta, int64_t npix, int32_t nq,
+                                                    int64_t M2, int32_t K, int32_t L,
+                                                    const RingDev* __restrict__ rings, const double2* __restrict__ tw,
+                                                    double* __restrict__ X, int32_t TQ)
+{
+    extern __shared__ double2 ps[];
+    const int k = blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev R = rings[k];
+    const int npr = R.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
+    const int ld = npr + 8;                       // doubles per sequence row
+    const double2* twl = tw + R.tw_off;           // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
+    double2* twi = ps;                            // (cos, sin)(2 pi v/i), v < i
+    double* D = (double*)(ps + ((ni + 1) & ~1));  // [2 TQ][ld]: samples, then Y (32-byte aligned rows)
+    for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];  // 2 pi v/i = pi (8v)/npr
+    for (int e = threadIdx.x; e < 2 * TQ * npr; e += blockDim.x) {
+        const int f = e / npr, j = e - f * npr;
+        const int q = q0 + (f >> 1);
+        double v = 0.0;
+        if (q < nq) v = delta[(int64_t)q * npix + ((f & 1) ? R.start_s : R.start_n) + j];
+        D[f * ld + j] = v;
+    }
+    __syncthreads();
+    // Y_s[u] of sequence f and frequencies (2 up, 2 up + 1) for work item e = threadIdx.x;
+    // the host keeps 2 TQ * ceil(nu / 2) <= blockDim.x
+    const int nup = (nu + 1) / 2;
+    const int e = threadIdx.x;
+    const bool active = e < 2 * TQ * nup;
+    const int up = active ? e % nup : 0, f = active ? e / nup : 0;
+    double a[2][4][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int ss = 0; ss < 4; ++ss) a[u][ss][0] = a[u][ss][1] = 0.0;
+    if (active) {
+        const int u0 = 2 * up, u1 = u0 + 1;
+        const double* d0 = D + f * ld;
+        int i0 = 0, i1 = 0;
+        for (int j = 0; j < ni; ++j) {
+            const double2 w0 = twi[i0], w1 = twi[i1];
+            const double4 x = *(const double4*)(d0 + 4 * j);
+            const double xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int ss = 0; ss < 4; ++ss) {
+                a[0][ss][0] = fma(xs[ss], w0.x, a[0][ss][0]);
+                a[0][ss][1] = fma(-xs[ss], w0.y, a[0][ss][1]);
+                a[1][ss][0] = fma(xs[ss], w1.x, a[1][ss][0]);
+                a[1][ss][1] = fma(-xs[ss], w1.y, a[1][ss][1]);
+            }
+            i0 += u0;
+            if (i0 >= ni) i0 -= ni;
+            i1 += u1;
+            if (i1 >= ni) i1 -= ni;
+        }
+    }
+    __syncthreads();  // every sample read; Y overwrites them
+    if (active) {
+        double2* Y = (double2*)(D + f * ld);  // Y[s * nu + u]
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int uu = 2 * up + u;
+            if (uu < nu)
+#pragma unroll
+                for (int ss = 0; ss < 4; ++ss) Y[ss * nu + uu] = make_double2(a[u][ss][0], a[u][ss][1]);
+        }
+    }
+    __syncthreads();
+    // X_kf for both rings of every map, then the folded stores (maps fastest)
+    const int nkf = min(L, npr / 2) + 1;
+    for (int e2 = threadIdx.x; e2 < TQ * nkf; e2 += blockDim.x) {
+        const int ql = e2 % TQ, kf = e2 / TQ;
+        const int q = q0 + ql;
+        if (q >= nq) continue;
+        int u = kf % ni;
+        const bool cj = u > ni / 2;
+        if (cj) u = ni - u;
+        double xr[2], xi[2];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double2* Y = (const double2*)(D + (2 * ql + side) * ld);
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int ss = 0; ss < 4; ++ss) {
+                double2 y = Y[ss * nu + u];
+                if (cj) y.y = -y.y;
+                const double2 w = twl[(int)(((int64_t)2 * kf * ss) % n2)];  // W^{kf s} = (c, -s)
+                sr += y.x * w.x + y.y * w.y;
+                si += y.y * w.x - y.x * w.y;
+            }
+            xr[side] = sr;
+            xi[side] = si;
+        }
+        store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q, xr[0], xi[0], xr[1], xi[1]);
+    }
+}
+
 // Tall-skinny GEMM on MFMA: C[n][i] = alpha sum_k A[k][i] B[k][n] + beta Cin[n][i],
 // i < rows.  A block owns 64 rows and up to 128 columns, so A (the large operand)
 // is read once per batch; 4 waves of 16 rows x all columns; the next k-tile is
@@ -1825,11 +1929,22 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st)
     LAUNCH_CHECK();
     if (k_eq0 > 0) {
         const int64_t npr_max = 4 * (nside - 1);
-        size_t shp = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), shp, st,
+        const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
+        const int32_t TQ4 = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, 256 / (2 * nup_max)));
+        size_t shp = sizeof(double2) * (size_t)(npr_max / 4 + 1) + sizeof(double) * 2 * (size_t)TQ4 * (size_t)(npr_max + 8);
+        if (2 * nup_max > 256 || (int64_t)shp > P->lds_max) {
+            size_t sho = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
+            HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sho));
+            hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), sho, st,
+                               P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                               P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP);
+            LAUNCH_CHECK();
+            return SFB_OK;
+        }
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
+        hipLaunchKernelGGL(k_phi_polar4, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), TQ4);
         LAUNCH_CHECK();
     }
     return SFB_OK;
