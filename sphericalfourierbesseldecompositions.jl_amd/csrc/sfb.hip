@@ -210,13 +210,13 @@ __device__ __forceinline__ double spline_dev(const MapInfo& mi, const double* __
 }
 
 // delta[q][p] for every map q (blockIdx.y) and pixel p.
-template <bool DENSE>
+template <bool DENSE, typename R = double>
 __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __restrict__ pstart,
                                                  const double* __restrict__ r_s, const double* __restrict__ w_s,
                                                  const MapInfo* __restrict__ maps, const double* __restrict__ kx,
                                                  const double* __restrict__ seg, double inv,
                                                  const double* __restrict__ win_a, const double* __restrict__ win_b,
-                                                 int64_t lmax, int32_t q0, double* __restrict__ delta)
+                                                 int64_t lmax, int32_t q0, R* __restrict__ delta)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int32_t q = q0 + (int32_t)blockIdx.y;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __
     double v = __dmul_rn(s, inv);
     int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
     double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], win_b[col]);
-    delta[(int64_t)q * npix + p] = __dsub_rn(v, W);
+    delta[(int64_t)q * npix + p] = (R)__dsub_rn(v, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -331,9 +331,11 @@ struct GroupInfo {
 // t = RN((r - x_i)/h) as q0 = RN(n y), t = RN(q0 + RN(n - h q0) y), y = RN(1/h):
 // Markstein's final correction step, which returns the correctly rounded quotient
 // (checked against IEEE division on every distinct segment of the tables).
+// TT = float rounds t once for the FP32 evaluation of the mixed path.
+template <typename TT>
 __global__ void k_seg_t(int64_t T, int32_t ngroup, const double* __restrict__ sr, const GroupInfo* __restrict__ groups,
                         const double* __restrict__ gx, const double* __restrict__ grh, int32_t* __restrict__ it_i,
-                        double* __restrict__ it_t)
+                        TT* __restrict__ it_t)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = blockIdx.y;
@@ -370,7 +372,7 @@ __global__ void k_seg_t(int64_t T, int32_t ngroup, const double* __restrict__ sr
         t = r;
     }
     it_i[(int64_t)g * T + e] = i;
-    it_t[(int64_t)g * T + e] = t;
+    it_t[(int64_t)g * T + e] = (TT)t;
 }
 
 // One block of the scatter: maps qa and qb (same knot group; qb < 0 if none) on
@@ -384,20 +386,28 @@ struct ScatterBlock {
 // block of XCD x; each XCD walks its own pixel chunks group by group, so the
 // group's (i, t), the weights of those chunks and the group's records stay in
 // that XCD's L2.  Two maps per block share every (i, t, w) load.
-template <bool DENSE, bool TWO>
+// R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
+// w and the records rounded to FP32, FP32 Horner and pixel sums, FP32 delta.
+template <typename R> struct Vec4T;
+template <> struct Vec4T<double> { typedef double4 T; };
+template <> struct Vec4T<float> { typedef float4 T; };
+
+template <bool DENSE, bool TWO, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
                                                      int32_t tstride, const int32_t* __restrict__ pstart,
                                                      const uint32_t* __restrict__ perm,
-                                                     const int64_t* __restrict__ soff, const double* __restrict__ sw,
-                                                     const int32_t* __restrict__ it_i, const double* __restrict__ it_t,
+                                                     const int64_t* __restrict__ soff, const R* __restrict__ sw,
+                                                     const int32_t* __restrict__ it_i, const R* __restrict__ it_t,
                                                      const MapInfo* __restrict__ maps,
-                                                     const double4* __restrict__ rec4, double inv,
+                                                     const typename Vec4T<R>::T* __restrict__ rec4, double inv,
                                                      const double* __restrict__ win_a,
                                                      const double* __restrict__ win_b, int64_t lmax,
-                                                     double* __restrict__ delta)
+                                                     R* __restrict__ delta)
 {
-    extern __shared__ double4 sm4[];
+    typedef typename Vec4T<R>::T R4;
+    extern __shared__ double4 sm_raw[];
+    R4* sm4 = (R4*)sm_raw;
     const ScatterBlock B = blk[(int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3)];
     const bool two = TWO && B.qb >= 0;
     const MapInfo ma = maps[B.qa];
@@ -406,7 +416,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     if (two)
         for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) sm4[tstride + k] = rec4[mb.koff + B.qb + k];
     const int32_t* gi = it_i + (int64_t)ma.pad * T;
-    const double* gt = it_t + (int64_t)ma.pad * T;
+    const R* gt = it_t + (int64_t)ma.pad * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int64_t cola = ma.l + (lmax + 1) * (int64_t)(ma.n - 1);
     const int64_t colb = mb.l + (lmax + 1) * (int64_t)(mb.n - 1);
@@ -420,9 +430,9 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         const uint32_t p = valid ? perm[e] : 0u;
         const int32_t cnt = valid ? pstart[p + 1] - pstart[p] : 0;
         const int64_t base = soff[s] + lane;
-        double acca = 0.0, accb = 0.0;
+        R acca = 0, accb = 0;
         for (int j0 = 0; j0 < cnt; j0 += 4) {
-            double tt[4], ww[4];
+            R tt[4], ww[4];
             int ii[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -433,22 +443,22 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double4 c = sm4[ii[u]];  // (d, c, b, a)
-                const double v = fma(tt[u], fma(tt[u], fma(tt[u], c.w, c.z), c.y), c.x);
-                if (j0 + u < cnt) acca = __dadd_rn(acca, __dmul_rn(ww[u], v));
+                const R4 c = sm4[ii[u]];  // (d, c, b, a)
+                const R v = fma(tt[u], fma(tt[u], fma(tt[u], c.w, c.z), c.y), c.x);
+                if (j0 + u < cnt) acca = acca + ww[u] * v;
                 if (two) {
-                    const double4 c2 = sm4[tstride + ii[u]];
-                    const double v2 = fma(tt[u], fma(tt[u], fma(tt[u], c2.w, c2.z), c2.y), c2.x);
-                    if (j0 + u < cnt) accb = __dadd_rn(accb, __dmul_rn(ww[u], v2));
+                    const R4 c2 = sm4[tstride + ii[u]];
+                    const R v2 = fma(tt[u], fma(tt[u], fma(tt[u], c2.w, c2.z), c2.y), c2.x);
+                    if (j0 + u < cnt) accb = accb + ww[u] * v2;
                 }
             }
         }
         if (valid) {
             const double Wa = DENSE ? win_a[p + npix * cola] : __dmul_rn(win_a[p], wcola);
-            delta[(int64_t)B.qa * npix + p] = __dsub_rn(__dmul_rn(acca, inv), Wa);
+            delta[(int64_t)B.qa * npix + p] = (R)__dsub_rn(__dmul_rn((double)acca, inv), Wa);
             if (two) {
                 const double Wb = DENSE ? win_a[p + npix * colb] : __dmul_rn(win_a[p], wcolb);
-                delta[(int64_t)B.qb * npix + p] = __dsub_rn(__dmul_rn(accb, inv), Wb);
+                delta[(int64_t)B.qb * npix + p] = (R)__dsub_rn(__dmul_rn((double)accb, inv), Wb);
             }
         }
     }
@@ -457,19 +467,21 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
 // Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
 // every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
 // each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
-__device__ __forceinline__ void store_folded(double* __restrict__ X, int64_t M2, int32_t K, int32_t k, bool eq,
+template <typename R>
+__device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int32_t K, int32_t k, bool eq,
                                              int npr, int shift, const double2* __restrict__ twr, int L, int kf,
-                                             int q, double xnr, double xni, double xsr, double xsi)
+                                             int q, R xnr, R xni, R xsr, R xsi)
 {
     const int n2 = 2 * npr;
     for (int fam = 0; fam < 2; ++fam) {
         if (fam == 1 && (kf == 0 || 2 * kf == npr)) break;
         const double cj = fam ? -1.0 : 1.0;
         for (int m = fam ? npr - kf : kf; m <= L; m += npr) {
-            const double2 ph = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
-            const double anr = xnr, ani = cj * xni, asr = xsr, asi = cj * xsi;
-            const double fnr = anr * ph.x + ani * ph.y, fni = ani * ph.x - anr * ph.y;
-            const double fsr = asr * ph.x + asi * ph.y, fsi = asi * ph.x - asr * ph.y;
+            const double2 ph64 = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
+            const R phx = (R)ph64.x, phy = (R)ph64.y;
+            const R anr = xnr, ani = (R)cj * xni, asr = xsr, asi = (R)cj * xsi;
+            const R fnr = anr * phx + ani * phy, fni = ani * phx - anr * phy;
+            const R fsr = asr * phx + asi * phy, fsi = asi * phx - asr * phy;
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
             if (eq) {
@@ -487,49 +499,94 @@ __device__ __forceinline__ void store_folded(double* __restrict__ X, int64_t M2,
     }
 }
 
+template <typename R> struct Vec2T;
+template <> struct Vec2T<double> { typedef double2 T; };
+template <> struct Vec2T<float> { typedef float2 T; };
+
 // Equatorial ring pairs (npr = 4 nside, a power of two): real FFT of length npr
-// through a complex radix-2 FFT of length npr/2 in LDS, 2*TQ sequences per block.
-__global__ __launch_bounds__(256) void k_phi_fft(const double* __restrict__ delta, int64_t npix, int32_t nq,
-                                                 int64_t M2, int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                 const double2* __restrict__ tw, double* __restrict__ X, int32_t TQ,
+// through a complex FFT of length M = npr/2 in LDS, 2*TQ sequences per block:
+// bit-reversed load, then the radix-2 stages fused in pairs (one radix-4 pass of
+// 4 loads / 3 twiddles / 4 butterflies per thread item between barriers).  R is
+// the value type (double: plain path; float: mixed path).
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
+                                                 int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                 const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
                                                  int32_t k0, int32_t logM)
 {
-    extern __shared__ double2 zs[];
+    typedef typename Vec2T<R>::T R2;
+    extern __shared__ double2 zs_raw[];
+    R2* zs = (R2*)zs_raw;
     const int k = k0 + blockIdx.x;
     const int q0 = blockIdx.y * TQ;
-    const RingDev R = rings[k];
-    const int N = R.npr, M = N >> 1;
+    const RingDev Rg = rings[k];
+    const int N = Rg.npr, M = N >> 1;
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
-    double2* twr = zs + nf * M;  // (cos, sin)(pi kk / N), kk < 2N, staged in LDS
-    for (int e = threadIdx.x; e < 2 * N; e += blockDim.x) twr[e] = tw[R.tw_off + e];
+    const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
+    R2* twr = zs + nf * M;                // kk <= N staged in LDS
+    for (int e = threadIdx.x; e <= N; e += blockDim.x) {
+        const double2 w = twg[e];
+        twr[e].x = (R)w.x;
+        twr[e].y = (R)w.y;
+    }
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
         const int f = e / M, jj = e - f * M;
         const int ql = f >> 1, side = f & 1;
         const int q = q0 + ql;
-        double re = 0.0, im = 0.0;
+        R re = 0, im = 0;
         if (q < nq && !(side && eq)) {
-            const int64_t base = (int64_t)q * npix + (side ? R.start_s : R.start_n) + 2 * jj;
+            const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
             re = delta[base];
             im = delta[base + 1];
         }
         const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-        zs[f * M + rj] = make_double2(re, im);
+        zs[f * M + rj].x = re;
+        zs[f * M + rj].y = im;
     }
     __syncthreads();
-    for (int st = 1; st <= logM; ++st) {
+    int st = 1;
+    for (; st + 1 <= logM; st += 2) {
+        const int half = 1 << (st - 1);
+        const int tstep = N / half;  // stage st twiddle step; stage st+1 uses tstep/2
+        const int q4 = M >> 2;
+        for (int e = threadIdx.x; e < nf * q4; e += blockDim.x) {
+            const int f = e / q4, b = e - f * q4;
+            const int grp = b / half, pos = b - grp * half;
+            const int a0 = f * M + grp * 4 * half + pos;
+            const int a1 = a0 + half, a2 = a0 + 2 * half, a3 = a0 + 3 * half;
+            const R2 w1 = twr[pos * tstep];
+            const R2 w2 = twr[pos * (tstep >> 1)];
+            const R2 w3 = twr[(pos + half) * (tstep >> 1)];
+            R2 x0 = zs[a0], x1 = zs[a1], x2 = zs[a2], x3 = zs[a3];
+            // stage st: (x0, x1) and (x2, x3) with w1 (e^{-i theta} = (c, -s))
+            R cr = x1.x * w1.x + x1.y * w1.y, ci = x1.y * w1.x - x1.x * w1.y;
+            R2 y0, y1, y2, y3;
+            y0.x = x0.x + cr; y0.y = x0.y + ci; y1.x = x0.x - cr; y1.y = x0.y - ci;
+            cr = x3.x * w1.x + x3.y * w1.y; ci = x3.y * w1.x - x3.x * w1.y;
+            y2.x = x2.x + cr; y2.y = x2.y + ci; y3.x = x2.x - cr; y3.y = x2.y - ci;
+            // stage st+1: (y0, y2) with w2, (y1, y3) with w3
+            cr = y2.x * w2.x + y2.y * w2.y; ci = y2.y * w2.x - y2.x * w2.y;
+            x0.x = y0.x + cr; x0.y = y0.y + ci; x2.x = y0.x - cr; x2.y = y0.y - ci;
+            cr = y3.x * w3.x + y3.y * w3.y; ci = y3.y * w3.x - y3.x * w3.y;
+            x1.x = y1.x + cr; x1.y = y1.y + ci; x3.x = y1.x - cr; x3.y = y1.y - ci;
+            zs[a0] = x0; zs[a1] = x1; zs[a2] = x2; zs[a3] = x3;
+        }
+        __syncthreads();
+    }
+    if (st == logM) {  // one radix-2 stage left
         const int half = 1 << (st - 1);
         const int tstep = N / half;
         for (int e = threadIdx.x; e < nf * (M >> 1); e += blockDim.x) {
             const int f = e / (M >> 1), b = e - f * (M >> 1);
             const int grp = b / half, pos = b - grp * half;
             const int i0 = f * M + grp * 2 * half + pos, i1 = i0 + half;
-            const double2 w = twr[pos * tstep];
-            const double2 a = zs[i0], c = zs[i1];
-            const double cr = c.x * w.x + c.y * w.y;
-            const double ci = c.y * w.x - c.x * w.y;
-            zs[i0] = make_double2(a.x + cr, a.y + ci);
-            zs[i1] = make_double2(a.x - cr, a.y - ci);
+            const R2 w = twr[pos * tstep];
+            const R2 a = zs[i0], c = zs[i1];
+            const R cr = c.x * w.x + c.y * w.y;
+            const R ci = c.y * w.x - c.x * w.y;
+            zs[i0].x = a.x + cr; zs[i0].y = a.y + ci;
+            zs[i1].x = a.x - cr; zs[i1].y = a.y - ci;
         }
         __syncthreads();
     }
@@ -539,18 +596,18 @@ __global__ __launch_bounds__(256) void k_phi_fft(const double* __restrict__ delt
         const int m = e / TQ, ql = e - m * TQ;  // maps fastest: contiguous X stores
         const int q = q0 + ql;
         if (q >= nq) continue;
-        double xr[2], xi[2];
+        R xr[2], xi[2];
         for (int side = 0; side < 2; ++side) {
-            const double2* z = zs + (2 * ql + side) * M;
-            const double2 zm = z[m & (M - 1)];
-            const double2 zc = z[(M - m) & (M - 1)];
-            const double er = 0.5 * (zm.x + zc.x), ei = 0.5 * (zm.y - zc.y);
-            const double orr = 0.5 * (zm.y + zc.y), oi = -0.5 * (zm.x - zc.x);
-            const double2 w = twr[2 * m];
+            const R2* z = zs + (2 * ql + side) * M;
+            const R2 zm = z[m & (M - 1)];
+            const R2 zc = z[(M - m) & (M - 1)];
+            const R er = (R)0.5 * (zm.x + zc.x), ei = (R)0.5 * (zm.y - zc.y);
+            const R orr = (R)0.5 * (zm.y + zc.y), oi = (R)-0.5 * (zm.x - zc.x);
+            const R2 w = twr[2 * m];
             xr[side] = er + (w.x * orr + w.y * oi);
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
-        store_folded(X, M2, K, k, eq, N, R.shift, twr, L, m, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twg, L, m, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -610,9 +667,9 @@ __global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ de
             if (idx >= n2) idx -= n2;
         }
         if (q0 + qa < nq)
-            store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa, a0r, a0i, b0r, b0i);
+            store_folded<double>(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa, a0r, a0i, b0r, b0i);
         if (q0 + qa + 1 < nq)
-            store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa + 1, a1r, a1i, b1r, b1i);
+            store_folded<double>(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa + 1, a1r, a1i, b1r, b1i);
     }
 }
 
@@ -624,26 +681,33 @@ __global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ de
 // 2 frequencies x 4 phases (16 fma per 64 bytes of LDS).  Y overwrites the staged
 // samples (row stride npr + 8 doubles holds 4 (i/2 + 1) complex values).  The
 // LDS footprint stays ~35 KB so that four blocks share a CU.
-__global__ __launch_bounds__(256) void k_phi_polar4(const double* __restrict__ delta, int64_t npix, int32_t nq,
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta, int64_t npix, int32_t nq,
                                                     int64_t M2, int32_t K, int32_t L,
                                                     const RingDev* __restrict__ rings, const double2* __restrict__ tw,
-                                                    double* __restrict__ X, int32_t TQ)
+                                                    R* __restrict__ X, int32_t TQ)
 {
-    extern __shared__ double2 ps[];
+    typedef typename Vec2T<R>::T R2;
+    typedef typename Vec4T<R>::T R4;
+    extern __shared__ double2 ps_raw[];
+    R2* ps = (R2*)ps_raw;
     const int k = blockIdx.x;
     const int q0 = blockIdx.y * TQ;
-    const RingDev R = rings[k];
-    const int npr = R.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
+    const RingDev Rg = rings[k];
+    const int npr = Rg.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
     const int ld = npr + 8;                       // doubles per sequence row
-    const double2* twl = tw + R.tw_off;           // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
-    double2* twi = ps;                            // (cos, sin)(2 pi v/i), v < i
-    double* D = (double*)(ps + ((ni + 1) & ~1));  // [2 TQ][ld]: samples, then Y (32-byte aligned rows)
-    for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];  // 2 pi v/i = pi (8v)/npr
+    const double2* twl = tw + Rg.tw_off;          // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
+    R2* twi = ps;                                 // (cos, sin)(2 pi v/i), v < i
+    R* D = (R*)(ps + ((ni + 1) & ~1));            // [2 TQ][ld]: samples, then Y (aligned rows)
+    for (int e = threadIdx.x; e < ni; e += blockDim.x) {  // 2 pi v/i = pi (8v)/npr
+        twi[e].x = (R)twl[8 * e].x;
+        twi[e].y = (R)twl[8 * e].y;
+    }
     for (int e = threadIdx.x; e < 2 * TQ * npr; e += blockDim.x) {
         const int f = e / npr, j = e - f * npr;
         const int q = q0 + (f >> 1);
-        double v = 0.0;
-        if (q < nq) v = delta[(int64_t)q * npix + ((f & 1) ? R.start_s : R.start_n) + j];
+        R v = 0;
+        if (q < nq) v = delta[(int64_t)q * npix + ((f & 1) ? Rg.start_s : Rg.start_n) + j];
         D[f * ld + j] = v;
     }
     __syncthreads();
@@ -653,19 +717,19 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const double* __restrict__ d
     const int e = threadIdx.x;
     const bool active = e < 2 * TQ * nup;
     const int up = active ? e % nup : 0, f = active ? e / nup : 0;
-    double a[2][4][2];
+    R a[2][4][2];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int ss = 0; ss < 4; ++ss) a[u][ss][0] = a[u][ss][1] = 0.0;
     if (active) {
         const int u0 = 2 * up, u1 = u0 + 1;
-        const double* d0 = D + f * ld;
+        const R* d0 = D + f * ld;
         int i0 = 0, i1 = 0;
         for (int j = 0; j < ni; ++j) {
-            const double2 w0 = twi[i0], w1 = twi[i1];
-            const double4 x = *(const double4*)(d0 + 4 * j);
-            const double xs[4] = {x.x, x.y, x.z, x.w};
+            const R2 w0 = twi[i0], w1 = twi[i1];
+            const R4 x = *(const R4*)(d0 + 4 * j);
+            const R xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
             for (int ss = 0; ss < 4; ++ss) {
                 a[0][ss][0] = fma(xs[ss], w0.x, a[0][ss][0]);
@@ -681,13 +745,16 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const double* __restrict__ d
     }
     __syncthreads();  // every sample read; Y overwrites them
     if (active) {
-        double2* Y = (double2*)(D + f * ld);  // Y[s * nu + u]
+        R2* Y = (R2*)(D + f * ld);  // Y[s * nu + u]
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int uu = 2 * up + u;
             if (uu < nu)
 #pragma unroll
-                for (int ss = 0; ss < 4; ++ss) Y[ss * nu + uu] = make_double2(a[u][ss][0], a[u][ss][1]);
+                for (int ss = 0; ss < 4; ++ss) {
+                    Y[ss * nu + uu].x = a[u][ss][0];
+                    Y[ss * nu + uu].y = a[u][ss][1];
+                }
         }
     }
     __syncthreads();
@@ -700,23 +767,24 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const double* __restrict__ d
         int u = kf % ni;
         const bool cj = u > ni / 2;
         if (cj) u = ni - u;
-        double xr[2], xi[2];
+        R xr[2], xi[2];
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-            const double2* Y = (const double2*)(D + (2 * ql + side) * ld);
-            double sr = 0.0, si = 0.0;
+            const R2* Y = (const R2*)(D + (2 * ql + side) * ld);
+            R sr = 0, si = 0;
 #pragma unroll
             for (int ss = 0; ss < 4; ++ss) {
-                double2 y = Y[ss * nu + u];
+                R2 y = Y[ss * nu + u];
                 if (cj) y.y = -y.y;
-                const double2 w = twl[(int)(((int64_t)2 * kf * ss) % n2)];  // W^{kf s} = (c, -s)
-                sr += y.x * w.x + y.y * w.y;
-                si += y.y * w.x - y.x * w.y;
+                const double2 w64 = twl[(int)(((int64_t)2 * kf * ss) % n2)];  // W^{kf s} = (c, -s)
+                const R wx = (R)w64.x, wy = (R)w64.y;
+                sr += y.x * wx + y.y * wy;
+                si += y.y * wx - y.x * wy;
             }
             xr[side] = sr;
             xi[side] = si;
         }
-        store_folded(X, M2, K, k, false, npr, R.shift, twl, L, kf, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<R>(X, M2, K, k, false, npr, Rg.shift, twl, L, kf, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -751,7 +819,7 @@ template <> struct MfmaT<true> {
 
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 8, GT_BNW = 16 * GT_NTM;
 
-template <bool F32, int NT>
+template <bool F32, int NT, bool AF = false>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
@@ -771,7 +839,10 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             const int e = tid + 256 * u;
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
-            ra[u] = (gk < d.K && gi < rows) ? (T)d.A[(int64_t)gk * d.lda + gi] : (T)0;
+            if (AF)
+                ra[u] = (gk < d.K && gi < rows) ? (T)((const float*)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
+            else
+                ra[u] = (gk < d.K && gi < rows) ? (T)d.A[(int64_t)gk * d.lda + gi] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < NT; ++u) {
@@ -819,7 +890,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32>
+template <bool F32, bool AF = false>
 __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
@@ -832,22 +903,25 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ 
     if (n0 >= d.N) return;
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 5: gemm_tall_body<F32, 5>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 6: gemm_tall_body<F32, 6>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 7: gemm_tall_body<F32, 7>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, 8>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 5: gemm_tall_body<F32, 5, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 6: gemm_tall_body<F32, 6, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 7: gemm_tall_body<F32, 7, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, 8, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
+// f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32)
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
-                               double beta, hipStream_t st, bool f32 = false)
+                               double beta, hipStream_t st, bool f32 = false, bool af32 = false)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (f32)
+    if (af32)
+        hipLaunchKernelGGL((k_gemm_tall<true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (f32)
         hipLaunchKernelGGL(k_gemm_tall<true>, g, dim3(256), 0, st, descs, rows, alpha, beta);
     else
         hipLaunchKernelGGL(k_gemm_tall<false>, g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -951,6 +1025,12 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 __global__ void k_fill_f64(double* p, int64_t n, double v)
@@ -1197,6 +1277,7 @@ struct sfb_plan {
     // work
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
+    DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
@@ -1214,7 +1295,7 @@ struct sfb_plan {
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
     // knot groups (maps with bit-identical knots) and the shared segment search
     int32_t ngroup = 0, nk_max_group = 0;
-    DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk;
+    DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf;
     int32_t sc_nchunk_it = 0, sc_per_xcd = 0;
     bool sc_two = false;
     // per-phase event timing (sfb_plan_profile)
@@ -1402,6 +1483,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         HIP_TRY(hipMemcpy(P->d_gx.p, gx.data(), sizeof(double) * gx.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->d_grh.p, grh.data(), sizeof(double) * grh.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->d_rec4.p, rec4.data(), sizeof(double) * rec4.size(), hipMemcpyHostToDevice));
+        std::vector<float> rec4f(rec4.begin(), rec4.end());  // the mixed path's FP32 records
+        if ((rc = P->d_rec4f.ensure(sizeof(float) * rec4f.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_rec4f.p, rec4f.data(), sizeof(float) * rec4f.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->d_qorder.p, qorder.data(), sizeof(int32_t) * qorder.size(), hipMemcpyHostToDevice));
         // scatter block list: per XCD, its pixel chunks, group by group, maps in
         // pairs of the same group when two record tables fit in LDS
@@ -1556,7 +1640,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     //   syn: G_al = a_in LamT (first K_al ring pairs)       C0: a_out = a_in (I - C0) + b
     //   al : a_out += dOmega R_al Lam[first K_al ring pairs]
     const int64_t nbat = 2 * (L + 1);
-    std::vector<GemmDesc> ana, syn, c0, al;
+    std::vector<GemmDesc> ana, ana32, syn, c0, al;
     double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
     double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
     for (int64_t b = 0; b < nbat; ++b) {
@@ -1572,6 +1656,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         d.ldb = n;
         d.ldc = P->M2;
         ana.push_back(d);
+        // mixed path: X0 holds floats, block b starts b K M2 floats in
+        d.A = (const double*)(P->d_X0.as<float>() + b * (int64_t)P->K * P->M2);
+        ana32.push_back(d);
     }
     for (int slot = 0; slot < 3; ++slot)
         for (int64_t b = 0; b < nbat; ++b) {
@@ -1631,6 +1718,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     if ((rc = P->d_desc_synf.ensure(sizeof(GemmDesc) * synf.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_synf.p, synf.data(), sizeof(GemmDesc) * synf.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_desc_ana.ensure(sizeof(GemmDesc) * ana.size()))) return rc;
+    if ((rc = P->d_desc_ana32.ensure(sizeof(GemmDesc) * ana32.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_ana32.p, ana32.data(), sizeof(GemmDesc) * ana32.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_desc_syn.ensure(sizeof(GemmDesc) * syn.size()))) return rc;
     if ((rc = P->d_desc_c0.ensure(sizeof(GemmDesc) * c0.size()))) return rc;
     if ((rc = P->d_desc_al.ensure(sizeof(GemmDesc) * al.size()))) return rc;
@@ -1725,7 +1814,8 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw,
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
-                      &P->d_it_i, &P->d_it_t, &P->d_sblk};
+                      &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_rec4f, &P->d_swf,
+                      &P->d_desc_ana32};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -1856,7 +1946,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
 }
 
 static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
-                         hipStream_t st)
+                         hipStream_t st, bool f32 = false)
 {
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
@@ -1868,85 +1958,131 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         if ((rc = P->d_it_i.ensure(sizeof(int32_t) * (size_t)(T * P->ngroup))) ||
             (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
             return rc;
-        hipLaunchKernelGGL(k_seg_t, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T, P->ngroup,
-                           P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
-                           P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<double>());
+        if (f32) {
+            if ((rc = P->d_swf.ensure(sizeof(float) * (size_t)T))) return rc;
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(T, 256)), dim3(256), 0, st, T, P->d_sw.as<double>(),
+                               P->d_swf.as<float>());
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_seg_t<float>, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T,
+                               P->ngroup, P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
+                               P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<float>());
+        } else {
+            hipLaunchKernelGGL(k_seg_t<double>, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T,
+                               P->ngroup, P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
+                               P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<double>());
+        }
         LAUNCH_CHECK();
         const int32_t tstride = P->tbl_cap + 1;
         const size_t shm = (size_t)tstride * 32 * (P->sc_two ? 2 : 1);
         const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
         const dim3 g1((unsigned)(8 * P->sc_per_xcd));
-#define SFB_LAUNCH_IT(D, TW)                                                                                      \
-    do {                                                                                                          \
-        HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_it<D, TW>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                    (int)shm));                                                                   \
-        hipLaunchKernelGGL((k_scatter_it<D, TW>), g1, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,        \
-                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_pstart.as<int32_t>(),       \
-                           P->d_perm.as<uint32_t>(), P->d_soff.as<int64_t>(), P->d_sw.as<double>(),               \
-                           P->d_it_i.as<int32_t>(), P->d_it_t.as<double>(), P->d_maps.as<MapInfo>(),              \
-                           P->d_rec4.as<double4>(), inv, d_wa, d_wb, P->lmax, P->d_delta.as<double>());           \
+#define SFB_LAUNCH_IT(D, TW, R, R4, REC, SW)                                                                        \
+    do {                                                                                                           \
+        HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_it<D, TW, R>,                                           \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                        \
+        hipLaunchKernelGGL((k_scatter_it<D, TW, R>), g1, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,      \
+                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_pstart.as<int32_t>(),        \
+                           P->d_perm.as<uint32_t>(), P->d_soff.as<int64_t>(), SW, P->d_it_i.as<int32_t>(),         \
+                           P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), REC, inv, d_wa, d_wb, P->lmax,              \
+                           P->d_delta.as<R>());                                                                    \
     } while (0)
-        if (win_kind == SFB_WIN_DENSE) {
-            if (P->sc_two)
-                SFB_LAUNCH_IT(true, true);
-            else
-                SFB_LAUNCH_IT(true, false);
+        const double* sw64 = P->d_sw.as<double>();
+        const float* sw32 = P->d_swf.as<float>();
+        const double4* r64 = P->d_rec4.as<double4>();
+        const float4* r32 = P->d_rec4f.as<float4>();
+        const bool dense = win_kind == SFB_WIN_DENSE;
+        if (f32) {
+            if (dense && P->sc_two) SFB_LAUNCH_IT(true, true, float, float4, r32, sw32);
+            else if (dense) SFB_LAUNCH_IT(true, false, float, float4, r32, sw32);
+            else if (P->sc_two) SFB_LAUNCH_IT(false, true, float, float4, r32, sw32);
+            else SFB_LAUNCH_IT(false, false, float, float4, r32, sw32);
         } else {
-            if (P->sc_two)
-                SFB_LAUNCH_IT(false, true);
-            else
-                SFB_LAUNCH_IT(false, false);
+            if (dense && P->sc_two) SFB_LAUNCH_IT(true, true, double, double4, r64, sw64);
+            else if (dense) SFB_LAUNCH_IT(true, false, double, double4, r64, sw64);
+            else if (P->sc_two) SFB_LAUNCH_IT(false, true, double, double4, r64, sw64);
+            else SFB_LAUNCH_IT(false, false, double, double4, r64, sw64);
         }
 #undef SFB_LAUNCH_IT
         LAUNCH_CHECK();
         return SFB_OK;
     }
     dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
-    if (win_kind == SFB_WIN_DENSE)
-        hipLaunchKernelGGL(k_scatter<true>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
-                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
-                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta.as<double>());
-    else
-        hipLaunchKernelGGL(k_scatter<false>, grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
-                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
-                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta.as<double>());
+#define SFB_LAUNCH_GS(D, R)                                                                                   \
+    hipLaunchKernelGGL((k_scatter<D, R>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),         \
+                       P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(), \
+                       P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta.as<R>())
+    if (win_kind == SFB_WIN_DENSE) {
+        if (f32) SFB_LAUNCH_GS(true, float);
+        else SFB_LAUNCH_GS(true, double);
+    } else {
+        if (f32) SFB_LAUNCH_GS(false, float);
+        else SFB_LAUNCH_GS(false, double);
+    }
+#undef SFB_LAUNCH_GS
     LAUNCH_CHECK();
     return SFB_OK;
 }
 
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, direct DFT for the polar ones.
-static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st)
+// The polar kernel's work items (2 TQ x ceil((i/2 + 1)/2) <= 256) and LDS limit it
+// to nside <= 1024; beyond, the direct-DFT kernel (FP64 only) runs.
+static bool polar4_fits(const sfb_plan* P, size_t vbytes, int32_t* TQ4, size_t* shp)
+{
+    const int64_t npr_max = 4 * (P->nside - 1);
+    const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
+    *TQ4 = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, 256 / (2 * nup_max)));
+    *shp = 2 * vbytes * (size_t)(npr_max / 4 + 1) + vbytes * 2 * (size_t)*TQ4 * (size_t)(npr_max + 8);
+    return 2 * nup_max <= 256 && (int64_t)*shp <= P->lds_max;
+}
+
+// delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
+// for the polar ones.  f32: delta and X0 hold floats (mixed path).
+static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int64_t nside = P->nside;
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
     const int64_t M = 2 * nside;
-    size_t shf = sizeof(double2) * (2 * (size_t)P->TQF * (size_t)M + 2 * (size_t)(2 * M));
-    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-    hipLaunchKernelGGL(k_phi_fft, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
-                       P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                       P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQF, k_eq0, P->logM);
+    const size_t vb = f32 ? sizeof(float) : sizeof(double);
+    const size_t shf = 2 * vb * (2 * (size_t)P->TQF * (size_t)M + 2 * (size_t)M + 1);
+    if (f32) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
+        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
+                           P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->TQF, k_eq0, P->logM);
+    } else {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
+        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
+                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQF, k_eq0, P->logM);
+    }
     LAUNCH_CHECK();
-    if (k_eq0 > 0) {
+    if (k_eq0 == 0) return SFB_OK;
+    int32_t TQ4;
+    size_t shp;
+    if (!polar4_fits(P, vb, &TQ4, &shp)) {
+        if (f32) return fail(SFB_ERR_ARG, "internal: FP32 phi-transform beyond nside 1024");
         const int64_t npr_max = 4 * (nside - 1);
-        const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
-        const int32_t TQ4 = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, 256 / (2 * nup_max)));
-        size_t shp = sizeof(double2) * (size_t)(npr_max / 4 + 1) + sizeof(double) * 2 * (size_t)TQ4 * (size_t)(npr_max + 8);
-        if (2 * nup_max > 256 || (int64_t)shp > P->lds_max) {
-            size_t sho = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
-            HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sho));
-            hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), sho, st,
-                               P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                               P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP);
-            LAUNCH_CHECK();
-            return SFB_OK;
-        }
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
+        size_t sho = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sho));
+        hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), sho, st,
+                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP);
+        LAUNCH_CHECK();
+        return SFB_OK;
+    }
+    if (f32) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
+        hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
+                           P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), TQ4);
+    } else {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
+        hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), TQ4);
-        LAUNCH_CHECK();
     }
+    LAUNCH_CHECK();
     return SFB_OK;
 }
 
@@ -1997,18 +2133,19 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
 }
 
 // delta (first nq maps) -> a, with niter refinements.
-static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st)
+static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
 {
     const int32_t L = (int32_t)P->lmax;
     const double domega = 4.0 * kPi / (double)P->npix;
     {
         PhaseTimer timer(P, SFB_PHASE_PHI, st);
-        int rc = phi_transform(P, nq, st);
+        int rc = phi_transform(P, nq, st, f32);
         if (rc) return rc;
     }
     {
         PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-        launch_gemm(P->d_desc_ana.as<GemmDesc>(), (unsigned)(2 * (L + 1)), 2 * nq, P->nb[0], domega, 0.0, st);
+        launch_gemm((f32 ? P->d_desc_ana32 : P->d_desc_ana).as<GemmDesc>(), (unsigned)(2 * (L + 1)), 2 * nq,
+                    P->nb[0], domega, 0.0, st, f32, f32);
         LAUNCH_CHECK();
     }
     return jacobi_stage(P, nq, st);
@@ -2035,8 +2172,13 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     HIP_TRY(hipSetDevice(P->device));
     hipStream_t st = stream ? (hipStream_t)stream : P->stream;
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
-    if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st))) return rc;
-    if ((rc = sht_stage(P, (int32_t)P->nmaps, st))) return rc;
+    // mixed path: FP32 spline values, delta maps and phi-transform when the FP32
+    // phi kernels fit (nside <= 1024), FP32 MFMA for every Legendre pass
+    int32_t tq4;
+    size_t shp4;
+    const bool f32 = P->precision == SFB_PREC_MIXED && polar4_fits(P, sizeof(float), &tq4, &shp4);
+    if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st, f32))) return rc;
+    if ((rc = sht_stage(P, (int32_t)P->nmaps, st, f32))) return rc;
     {
         PhaseTimer timer(P, SFB_PHASE_PACK, st);
         hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
@@ -2395,7 +2537,7 @@ extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, 
     HIP_TRY(hipSetDevice(P->device));
     HIP_TRY(hipMemcpyAsync(P->d_delta.p, maps, sizeof(double) * (size_t)(nmap * P->npix), hipMemcpyHostToDevice,
                            P->stream));
-    int rc = phi_transform(P, (int32_t)nmap, P->stream);
+    int rc = phi_transform(P, (int32_t)nmap, P->stream, false);
     if (rc) return rc;
     // out[(b * K + k) * 2 nmap + i]
     HIP_TRY(hipMemcpy2DAsync(x_out, sizeof(double) * 2 * (size_t)nmap, P->d_X0.p, sizeof(double) * (size_t)P->M2,
