@@ -49,19 +49,26 @@ def catalogue(N, rmin, rmax, seed):
     return np.stack([r, th, ph])
 
 
-def algorithmic_work(am, N, niter):
-    """Per-launch algorithmic bytes / flops of every phase."""
+def algorithmic_work(am, N, niter, f32=False, ngroup=None):
+    """Per-launch algorithmic bytes / flops of every phase (f32: the mixed path's
+    FP32 delta maps and ring coefficients)."""
     L, nside = am.lmax, am.nside
     npix, K, nmaps = 12 * nside * nside, 2 * nside, am.nmaps
     M2 = 2 * nmaps
+    vb = 4 if f32 else 8
     nlm_pairs = (L + 1) * (L + 2) // 2  # sum over (m, parity) of the l count
-    delta_bytes = nmaps * npix * 8
-    x_bytes = (L + 1) * 2 * K * M2 * 8
+    delta_bytes = nmaps * npix * vb
+    x_bytes = (L + 1) * 2 * K * M2 * vb
     a_bytes = nlm_pairs * M2 * 8
+    if ngroup is None:
+        ngroup = len({am.knot_x[am.knot_off[q]:am.knot_off[q + 1]].tobytes() for q in range(am.nmaps)})
     return {
-        "scatter": {"bytes": delta_bytes + N * 16 + (npix + 1) * 4 + npix * 8,
-                    "flops": 20.0 * N * nmaps},
-        "phi": {"bytes": delta_bytes + x_bytes, "flops": 4.0 * npix * (L + 1) * nmaps},
+        # delta maps written once; per galaxy its weight, segment and t read once
+        "scatter": {"bytes": delta_bytes + N * (vb + 4 + vb) + (npix + 1) * 4 + npix * 8,
+                    "flops": 10.0 * N * nmaps},
+        # per knot group and galaxy: r read, (segment, t) written
+        "segment": {"bytes": ngroup * N * (8 + 4 + vb), "flops": 0.0},
+        "phi": {"bytes": delta_bytes + x_bytes, "flops": 0.0},
         # a = dOmega * X Lambda with N/S ring-pair symmetry: 2 flop x rows x K x sum n_b
         "analysis": {"bytes": x_bytes + a_bytes, "flops": 2.0 * M2 * K * nlm_pairs},
         "synthesis": {"bytes": x_bytes + a_bytes, "flops": 2.0 * M2 * K * nlm_pairs},
@@ -200,7 +207,7 @@ def main():
 
     head = "mixed" if "mixed" in results else "fp64"
     dt, phases, calls, anlm = results[head]
-    work = algorithmic_work(am, N, niter)
+    work = algorithmic_work(am, N, niter, f32=(head == "mixed"))
 
     # dominant kernel of the headline path and its roofline
     per_launch = {k: (v[0] / max(v[1], 1)) for k, v in phases.items() if v[1] > 0}
@@ -220,6 +227,8 @@ def main():
 
     # HBM bytes per launch of the dominant kernel from the FETCH_SIZE / WRITE_SIZE
     # counter passes (scaled by the same counters on the calibration copy)
+    roof["algorithmic_bytes_per_launch"] = w["bytes"]
+    roof["launch_ms"] = t_launch * 1e3
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
@@ -227,7 +236,6 @@ def main():
         if tr:
             roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
             roof["traffic_unit"] = "B/launch"
-            roof["algorithmic_bytes"] = w["bytes"]
 
     # CPU baseline + accuracy on a bounded sample of maps
     cpu = None

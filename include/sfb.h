@@ -39,8 +39,9 @@ extern "C" {
 #define SFB_ERR_NOMEM 6      /* host allocation failed */
 
 #define SFB_PREC_FP64 0      /* plain path: reference method, FP64 throughout */
-#define SFB_PREC_MIXED 1     /* FP64 delta maps and phi-transform, Legendre steps on FP32 MFMA
-                                with split operands for the first analysis pass */
+#define SFB_PREC_MIXED 1     /* FP32 spline values, delta maps and phi-transform, every Legendre
+                                pass on FP32 MFMA; FP64 setup, segment search, lambda tables and
+                                GEMM epilogues */
 
 #define SFB_WIN_DENSE 0      /* win_a = W[npix, lmax+1, nmax] */
 #define SFB_WIN_SEPARABLE 1  /* win_a = mask[npix], win_b = w_ln[lmax+1, nmax] */
@@ -155,13 +156,14 @@ int sfb_cat2amln_vjp_device(sfb_plan* p, int64_t N, const double* d_rtp, const d
 /* ---- per-phase timing ------------------------------------------------------ */
 
 #define SFB_PHASE_CATALOG 0    /* ang2pix, the two stable sorts, gathers, pixel segments */
-#define SFB_PHASE_SCATTER 1    /* spline evaluation + per-pixel sums -> delta maps */
+#define SFB_PHASE_SCATTER 1    /* spline evaluation + ordered per-pixel sums -> delta maps */
 #define SFB_PHASE_PHI 2        /* per-ring phi-transform */
 #define SFB_PHASE_ANALYSIS 3   /* Legendre analysis GEMMs (1 + niter launches) */
 #define SFB_PHASE_SYNTHESIS 4  /* Legendre synthesis GEMMs (niter launches) */
 #define SFB_PHASE_RESIDUAL 5   /* ring-Fourier residual with aliasing (niter launches) */
 #define SFB_PHASE_PACK 6       /* a_nlm packing and the non-finite check */
-#define SFB_NPHASE 7
+#define SFB_PHASE_SEGMENT 7    /* per knot group and galaxy: spline segment and t (shared by the group's maps) */
+#define SFB_NPHASE 8
 
 /* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and
  * clears the totals.  No reference counterpart (the reference prints @time). */

@@ -1317,13 +1317,15 @@ struct PhaseTimer {
         if (P->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, st);
     }
-    ~PhaseTimer()
+    void end()
     {
         if (a && b) {
             (void)hipEventRecord(b, st);
             P->pend[ph].emplace_back(a, b);
         }
+        a = b = nullptr;
     }
+    ~PhaseTimer() { end(); }
 };
 
 // After the stream has drained: fold the pending event pairs into the totals.
@@ -1950,11 +1952,11 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 {
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
-    PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
     if (P->sell_ok && !P->sell_empty) {
         // shared segment search per knot group, then three fma per map and galaxy
         const int64_t T = std::max<int64_t>(P->sell_total, 1);
         int rc;
+        PhaseTimer tseg(P, SFB_PHASE_SEGMENT, st);
         if ((rc = P->d_it_i.ensure(sizeof(int32_t) * (size_t)(T * P->ngroup))) ||
             (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
             return rc;
@@ -1972,6 +1974,8 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<double>());
         }
         LAUNCH_CHECK();
+        tseg.end();
+        PhaseTimer tsc(P, SFB_PHASE_SCATTER, st);
         const int32_t tstride = P->tbl_cap + 1;
         const size_t shm = (size_t)tstride * 32 * (P->sc_two ? 2 : 1);
         const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
@@ -2006,6 +2010,7 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         LAUNCH_CHECK();
         return SFB_OK;
     }
+    PhaseTimer timer(P, SFB_PHASE_SCATTER, st);
     dim3 grid(nblk(P->npix, 256), (unsigned)P->nmaps);
 #define SFB_LAUNCH_GS(D, R)                                                                                   \
     hipLaunchKernelGGL((k_scatter<D, R>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),         \

@@ -58,7 +58,7 @@ SIGNATURES = {
     "sfb_plan_phase_times": (ctypes.c_int, [_P, _P, _P, _P]),
 }
 
-PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack"]
+PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment"]
 
 _lib = None
 
