@@ -153,6 +153,32 @@ int sfb_cat2amln_vjp(sfb_plan* p, int64_t N, const double* rtp, const double* w,
 int sfb_cat2amln_vjp_device(sfb_plan* p, int64_t N, const double* d_rtp, const double* d_w, double nbar,
                             const double* d_dL_danlm, double* d_dr, double* d_dw, void* stream);
 
+/* ---- window and field transforms (the path's input producers) ----------------- */
+
+/* win_rhat_ln(win, wmodes, amodes) (src/windows.jl:244-270) on the device.
+ * r[nr] are the window's radial bin centres and dr their width (wmodes.r, wmodes.Δr).
+ * Dense: win_a is the nr x npix window (column-major, r fastest) and out is
+ * W[npix, lmax+1, nmax] = dr * sum_k win[k, p] r_k^2 g_nl(r_k), NaN where l > lmax_n[n]
+ * (one MFMA GEMM per n).  Separable (SeparableArray(phi, mask), :260-270): win_a = phi[nr]
+ * and out = w_ln[lmax+1, nmax]; the mask passes through unchanged. */
+int sfb_win_rhat_ln(sfb_plan* p, int64_t nr, const double* r, double dr, int32_t win_kind, const double* win_a,
+                    double* out);
+int sfb_win_rhat_ln_device(sfb_plan* p, int64_t nr, const double* d_r, double dr, int32_t win_kind,
+                           const double* d_win_a, double* d_out, void* stream);
+
+/* field2anlm(f_xyz, wmodes, amodes) (src/cat2anlm.jl:323-358, the default field2anlm_v2):
+ * f is nr x npix (column-major); every shell goes through the plan's map2alm, then
+ * f_nlm += dr r^2 g_nl(r) a_lm.  out holds 2*nlmsize doubles. */
+int sfb_field2anlm(sfb_plan* p, int64_t nr, const double* r, double dr, const double* f, double* out);
+int sfb_field2anlm_device(sfb_plan* p, int64_t nr, const double* d_r, double dr, const double* d_f, double* d_out,
+                          void* stream);
+
+/* anlm2field(f_nlm, wmodes, amodes) (src/cat2anlm.jl:383-422): per shell
+ * a_lm = sum_n g_nl(r) f_nlm, then alm2map; f_out is nr x npix (column-major). */
+int sfb_anlm2field(sfb_plan* p, int64_t nr, const double* r, const double* f_nlm, double* f_out);
+int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const double* d_f_nlm, double* d_f_out,
+                          void* stream);
+
 /* ---- per-phase timing ------------------------------------------------------ */
 
 #define SFB_PHASE_CATALOG 0    /* ang2pix, the two stable sorts, gathers, pixel segments */

@@ -890,17 +890,11 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32, bool AF = false>
-__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
-                                                   double beta)
+template <bool F32, bool AF>
+__device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
+                                              int n0, typename MfmaT<F32>::T (*As)[GT_BM],
+                                              typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
-    typedef typename MfmaT<F32>::T T;
-    __shared__ T As[GT_BK][GT_BM];
-    __shared__ T Bs[GT_BK][GT_BNW];
-    const GemmDesc d = descs[blockIdx.z];
-    const int i0 = blockIdx.x * GT_BM;
-    const int n0 = blockIdx.y * GT_BNW;
-    if (n0 >= d.N) return;
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
         case 1: gemm_tall_body<F32, 1, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
@@ -913,6 +907,43 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ 
         default: gemm_tall_body<F32, 8, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
+
+template <bool F32, bool AF = false>
+__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+                                                   double beta)
+{
+    typedef typename MfmaT<F32>::T T;
+    __shared__ T As[GT_BK][GT_BM];
+    __shared__ T Bs[GT_BK][GT_BNW];
+    const GemmDesc d = descs[blockIdx.z];
+    const int i0 = blockIdx.x * GT_BM;
+    const int n0 = blockIdx.y * GT_BNW;
+    if (n0 >= d.N) return;
+    gemm_dispatch<F32, AF>(d, rows, alpha, beta, i0, n0, As, Bs);
+}
+
+// Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
+// XCD b mod 8, and XCD x takes row tiles x, x + 8, ... each followed by all its
+// (column tile, batch) blocks, so a row tile of A is read from HBM once and then
+// served by that XCD's L2.
+template <bool F32>
+__global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+                                                       double beta, int32_t gx, int32_t gy, int32_t gz)
+{
+    typedef typename MfmaT<F32>::T T;
+    __shared__ T As[GT_BK][GT_BM];
+    __shared__ T Bs[GT_BK][GT_BNW];
+    const int b = blockIdx.x, xcd = b & 7, loc = b >> 3;
+    const int per = gy * gz;
+    const int rt = (loc / per) * 8 + xcd, rem = loc - (loc / per) * per;
+    if (rt >= gx) return;
+    const GemmDesc d = descs[rem / gy];
+    const int i0 = rt * GT_BM;
+    const int n0 = (rem % gy) * GT_BNW;
+    if (n0 >= d.N) return;
+    gemm_dispatch<F32, false>(d, rows, alpha, beta, i0, n0, As, Bs);
+}
+
 
 // f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32)
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
@@ -1226,6 +1257,152 @@ __global__ void k_vjp_reduce(int64_t N, int32_t ngroups, const double2* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Window and field transforms on the device: win_rhat_ln (src/windows.jl:244-270),
+// field2anlm (src/cat2anlm.jl:323-358) and anlm2field (:383-422).
+// ---------------------------------------------------------------------------
+
+// gtab[q][k] = g_q(r_k), the reference's spline value (boundary extrapolation)
+__global__ void k_gnl_table(int32_t nq, int64_t nr, const double* __restrict__ r, const MapInfo* __restrict__ maps,
+                            const double4* __restrict__ rec, double* __restrict__ gtab)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (k >= nr || q >= nq) return;
+    const MapInfo mi = maps[q];
+    double v, dv;
+    spline_vd(mi, rec + mi.koff, r[k], v, dv);
+    gtab[(int64_t)q * nr + k] = v;
+}
+
+// B[n-1][k][l] = r_k^2 g_{n,l}(r_k), zero for l > lmax_n[n] (the GEMM operand of win_rhat_ln)
+__global__ void k_wrl_B(int32_t nmax, int32_t L, int64_t nr, const double* __restrict__ r,
+                        const int32_t* __restrict__ qbase, const int32_t* __restrict__ lmaxn,
+                        const double* __restrict__ gtab, double* __restrict__ B)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nmax * nr * (L + 1)) return;
+    const int l = (int)(e % (L + 1));
+    const int64_t k = (e / (L + 1)) % nr;
+    const int n = (int)(e / ((int64_t)(L + 1) * nr));
+    double v = 0.0;
+    if (l <= lmaxn[n]) v = r[k] * r[k] * gtab[(int64_t)(qbase[n] + l) * nr + k];
+    B[e] = v;
+}
+
+// out[c * ldo + (r - r0)] = in[r + R c] for r in [r0, r0 + nrw), c < C: rows of a
+// column-major R x C matrix into row-major [r][c] blocks (32 x 32 LDS tiles)
+__global__ void k_transpose_block(int64_t R, int64_t C, int64_t r0, int64_t nrw, const double* __restrict__ in,
+                                  double* __restrict__ out, int64_t ldo, int32_t to_rows)
+{
+    __shared__ double tile[32][33];
+    const int64_t c0 = (int64_t)blockIdx.x * 32, rb = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    if (to_rows) {
+        // in: column-major R x C, read rows r0 + rb + .. ; out[(rr) * ldo + c]
+        for (int j = ty; j < 32; j += 8) {
+            const int64_t c = c0 + j, rr = rb + tx;
+            tile[j][tx] = (c < C && rr < nrw) ? in[(r0 + rr) + R * c] : 0.0;
+        }
+        __syncthreads();
+        for (int j = ty; j < 32; j += 8) {
+            const int64_t rr = rb + j, c = c0 + tx;
+            if (rr < nrw && c < C) out[rr * ldo + c] = tile[tx][j];
+        }
+    } else {
+        // inverse: in[(rr) * ldo + c] -> out column-major R x C at rows r0 + rr
+        for (int j = ty; j < 32; j += 8) {
+            const int64_t rr = rb + j, c = c0 + tx;
+            tile[j][tx] = (rr < nrw && c < C) ? in[rr * ldo + c] : 0.0;
+        }
+        __syncthreads();
+        for (int j = ty; j < 32; j += 8) {
+            const int64_t c = c0 + j, rr = rb + tx;
+            if (c < C && rr < nrw) out[(r0 + rr) + R * c] = tile[tx][j];
+        }
+    }
+}
+
+// NaN where the reference leaves W_rhat_ln unset (l > lmax_n[n])
+__global__ void k_nan_cols(double* __restrict__ W, int64_t npix, int32_t L, int32_t nmax,
+                           const int32_t* __restrict__ lmaxn)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y, n = blockIdx.z;
+    if (p >= npix || l <= lmaxn[n]) return;
+    W[p + npix * ((int64_t)l + (int64_t)(L + 1) * n)] = __builtin_nan("");
+}
+
+// separable window: w_ln[l, n] = dr * sum_k (r_k^2 g_nl(r_k)) phi_k, NaN for l > lmax_n[n]
+__global__ void k_wrl_sep(int32_t nmax, int32_t L, int64_t nr, const double* __restrict__ r,
+                          const double* __restrict__ phi, const int32_t* __restrict__ qbase,
+                          const int32_t* __restrict__ lmaxn, const double* __restrict__ gtab, double dr,
+                          double* __restrict__ wln)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (L + 1) * nmax) return;
+    const int l = e % (L + 1), n = e / (L + 1);
+    if (l > lmaxn[n]) {
+        wln[e] = __builtin_nan("");
+        return;
+    }
+    const double* g = gtab + (int64_t)(qbase[n] + l) * nr;
+    double acc = 0.0;
+    for (int64_t k = 0; k < nr; ++k) acc += r[k] * r[k] * g[k] * phi[k];
+    wln[e] = dr * acc;
+}
+
+// field2anlm: out[n,l,m] += sum_s dr r_s^2 g_nl(r_s) a^{(s)}_{l m} over the shells s of
+// this chunk (maps of the SHT), in the a layout of the Legendre step
+__global__ void k_field_contract(int32_t nmaps, int32_t Q, int64_t s0, int64_t nr, const double* __restrict__ r,
+                                 double dr, const double* __restrict__ gtab, const MapInfo* __restrict__ maps,
+                                 const int64_t* __restrict__ aoff, const int64_t* __restrict__ out_base, int64_t M2,
+                                 const double* __restrict__ a, double* __restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (q >= nmaps) return;
+    const int l = maps[q].l;
+    if (m > l) return;
+    const int par = (l - m) & 1, j = (l - m) >> 1;
+    const double* src = a + aoff[m * 2 + par] + (int64_t)j * M2;
+    const double* g = gtab + (int64_t)q * nr;
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < Q; ++s) {
+        const double rs = r[s0 + s];
+        const double w = g[s0 + s] * rs * rs * dr;
+        sr += w * src[2 * s];
+        si += w * src[2 * s + 1];
+    }
+    out[2 * (out_base[q] + m)] += sr;
+    out[2 * (out_base[q] + m) + 1] += si;
+}
+
+// anlm2field: a^{(s)}_{l m} = sum_n g_nl(r_s) f_nlm for the shells s of this chunk
+__global__ void k_field_alm(int32_t nmax, int32_t L, int32_t Q, int64_t s0, int64_t nr,
+                            const double* __restrict__ gtab, const int32_t* __restrict__ qbase,
+                            const int32_t* __restrict__ lmaxn, const int64_t* __restrict__ out_base,
+                            const int64_t* __restrict__ aoff, int64_t M2, const double* __restrict__ fnlm,
+                            double* __restrict__ a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y, l = blockIdx.z;
+    if (s >= Q || m > l) return;
+    double sr = 0.0, si = 0.0;
+    for (int n = 0; n < nmax; ++n) {
+        if (l > lmaxn[n]) continue;
+        const int q = qbase[n] + l;
+        const double g = gtab[(int64_t)q * nr + s0 + s];
+        const int64_t o = 2 * (out_base[q] + m);
+        sr += g * fnlm[o];
+        si += g * fnlm[o + 1];
+    }
+    const int par = (l - m) & 1, j = (l - m) >> 1;
+    double* dst = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * s;
+    dst[0] = sr;
+    dst[1] = si;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 struct DevBuf {
@@ -1278,6 +1455,8 @@ struct sfb_plan {
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
+    DevBuf d_qbase, d_lmaxn;       // per n: map index of (n, 0) and lmax_n (int32)
+    DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
@@ -1530,6 +1709,18 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     HIP_TRY(hipMemcpy(P->d_seg.p, seg.data(), sizeof(double) * seg.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_outbase.ensure(sizeof(int64_t) * (size_t)P->nmaps))) return rc;
     HIP_TRY(hipMemcpy(P->d_outbase.p, P->out_base.data(), sizeof(int64_t) * (size_t)P->nmaps, hipMemcpyHostToDevice));
+    {
+        std::vector<int32_t> qb((size_t)P->nmax), ln((size_t)P->nmax);
+        int32_t acc = 0;
+        for (int64_t n = 0; n < P->nmax; ++n) {
+            qb[(size_t)n] = acc;
+            ln[(size_t)n] = (int32_t)lmax_n[n];
+            acc += (int32_t)lmax_n[n] + 1;
+        }
+        if ((rc = P->d_qbase.ensure(4 * qb.size())) || (rc = P->d_lmaxn.ensure(4 * ln.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_qbase.p, qb.data(), 4 * qb.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_lmaxn.p, ln.data(), 4 * ln.size(), hipMemcpyHostToDevice));
+    }
 
     // rings and twiddles
     std::vector<sfb::RingPair> rp = sfb::ring_pairs(P->nside);
@@ -1817,7 +2008,8 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_rec4f, &P->d_swf,
-                      &P->d_desc_ana32};
+                      &P->d_desc_ana32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
+                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -2409,6 +2601,209 @@ extern "C" int sfb_cat2amln_batch(sfb_plan* P, int32_t B, const int64_t* N_b, co
     if (B > 0)
         HIP_TRY(hipMemcpyAsync(anlm_out, P->h_out.p, sizeof(double) * 2 * (size_t)P->nlmsize * (size_t)B,
                                hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// win_rhat_ln, field2anlm, anlm2field
+// ---------------------------------------------------------------------------
+static int radial_table(sfb_plan* P, int64_t nr, const double* d_r, hipStream_t st)
+{
+    int rc;
+    if ((rc = P->d_gtab.ensure(sizeof(double) * (size_t)(P->nmaps * nr)))) return rc;
+    hipLaunchKernelGGL(k_gnl_table, dim3(nblk(nr, 256), (unsigned)P->nmaps), dim3(256), 0, st, (int32_t)P->nmaps, nr,
+                       d_r, P->d_maps.as<MapInfo>(), P->d_rec.as<double4>(), P->d_gtab.as<double>());
+    LAUNCH_CHECK();
+    return SFB_OK;
+}
+
+extern "C" int sfb_win_rhat_ln_device(sfb_plan* P, int64_t nr, const double* d_r, double dr, int32_t win_kind,
+                                      const double* d_win_a, double* d_out, void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !d_r || !d_win_a || !d_out) return fail(SFB_ERR_ARG, "bad win_rhat_ln arguments");
+    if (win_kind != SFB_WIN_DENSE && win_kind != SFB_WIN_SEPARABLE) return fail(SFB_ERR_ARG, "unknown window kind");
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    const int32_t L = (int32_t)P->lmax, nmax = (int32_t)P->nmax;
+    int rc;
+    if ((rc = radial_table(P, nr, d_r, st))) return rc;
+    if (win_kind == SFB_WIN_SEPARABLE) {
+        hipLaunchKernelGGL(k_wrl_sep, dim3(nblk((L + 1) * nmax, 256)), dim3(256), 0, st, nmax, L, nr, d_r, d_win_a,
+                           P->d_qbase.as<int32_t>(), P->d_lmaxn.as<int32_t>(), P->d_gtab.as<double>(), dr, d_out);
+        LAUNCH_CHECK();
+        HIP_TRY(hipStreamSynchronize(st));
+        return SFB_OK;
+    }
+    // dense: W[p, l, n] = dr * sum_k win[k, p] r_k^2 g_nl(r_k), one GEMM per n over the
+    // pixel-fastest copy of the window
+    const int64_t nB = (int64_t)nmax * nr * (L + 1);
+    if ((rc = P->d_wB.ensure(sizeof(double) * (size_t)nB)) ||
+        (rc = P->d_winT.ensure(sizeof(double) * (size_t)(nr * P->npix))) ||
+        (rc = P->d_desc_tmp.ensure(sizeof(GemmDesc) * (size_t)nmax)))
+        return rc;
+    hipLaunchKernelGGL(k_wrl_B, dim3(nblk(nB, 256)), dim3(256), 0, st, nmax, L, nr, d_r, P->d_qbase.as<int32_t>(),
+                       P->d_lmaxn.as<int32_t>(), P->d_gtab.as<double>(), P->d_wB.as<double>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_transpose_block, dim3(nblk(P->npix, 32), nblk(nr, 32)), dim3(256), 0, st, nr, P->npix,
+                       (int64_t)0, nr, d_win_a, P->d_winT.as<double>(), P->npix, 1);
+    LAUNCH_CHECK();
+    std::vector<GemmDesc> desc((size_t)nmax);
+    for (int32_t n = 0; n < nmax; ++n) {
+        GemmDesc& d = desc[(size_t)n];
+        d.A = P->d_winT.as<double>();
+        d.B = P->d_wB.as<double>() + (int64_t)n * nr * (L + 1);
+        d.C = d_out + P->npix * (int64_t)(L + 1) * n;
+        d.Cin = nullptr;
+        d.K = (int32_t)nr;
+        d.N = L + 1;
+        d.lda = P->npix;
+        d.ldb = L + 1;
+        d.ldc = P->npix;
+    }
+    HIP_TRY(hipMemcpyAsync(P->d_desc_tmp.p, desc.data(), sizeof(GemmDesc) * desc.size(), hipMemcpyHostToDevice, st));
+    const int32_t gx = (int32_t)((P->npix + GT_BM - 1) / GT_BM), gy = (L + 1 + GT_BNW - 1) / GT_BNW, gz = nmax;
+    const int64_t nblocks = (int64_t)8 * ((gx + 7) / 8) * gy * gz;
+    hipLaunchKernelGGL(k_gemm_rowshare<false>, dim3((unsigned)nblocks), dim3(256), 0, st,
+                       P->d_desc_tmp.as<GemmDesc>(), (int32_t)P->npix, dr, 0.0, gx, gy, gz);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nan_cols, dim3(nblk(P->npix, 256), (unsigned)(L + 1), (unsigned)nmax), dim3(256), 0, st,
+                       d_out, P->npix, L, nmax, P->d_lmaxn.as<int32_t>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(st));
+    return SFB_OK;
+}
+
+extern "C" int sfb_win_rhat_ln(sfb_plan* P, int64_t nr, const double* r, double dr, int32_t win_kind,
+                               const double* win_a, double* out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !r || !win_a || !out) return fail(SFB_ERR_ARG, "bad win_rhat_ln arguments");
+    if (win_kind != SFB_WIN_DENSE && win_kind != SFB_WIN_SEPARABLE) return fail(SFB_ERR_ARG, "unknown window kind");
+    HIP_TRY(hipSetDevice(P->device));
+    const size_t nin = (win_kind == SFB_WIN_DENSE) ? (size_t)(nr * P->npix) : (size_t)nr;
+    const size_t nout = (size_t)((P->lmax + 1) * P->nmax) * (win_kind == SFB_WIN_DENSE ? (size_t)P->npix : 1);
+    int rc;
+    if ((rc = P->d_rr.ensure(sizeof(double) * (size_t)nr)) || (rc = P->h_fa.ensure(sizeof(double) * nin)) ||
+        (rc = P->h_fo.ensure(sizeof(double) * nout)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(P->d_rr.p, r, sizeof(double) * (size_t)nr, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->h_fa.p, win_a, sizeof(double) * nin, hipMemcpyHostToDevice, P->stream));
+    if ((rc = sfb_win_rhat_ln_device(P, nr, P->d_rr.as<double>(), dr, win_kind, P->h_fa.as<double>(),
+                                     P->h_fo.as<double>(), P->stream)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, P->h_fo.p, sizeof(double) * nout, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_field2anlm_device(sfb_plan* P, int64_t nr, const double* d_r, double dr, const double* d_f,
+                                     double* d_out, void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !d_r || !d_f || !d_out) return fail(SFB_ERR_ARG, "bad field2anlm arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    int rc;
+    if ((rc = radial_table(P, nr, d_r, st))) return rc;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(2 * P->nlmsize, 256)), dim3(256), 0, st, d_out, 2 * P->nlmsize, 0.0);
+    LAUNCH_CHECK();
+    // shells in chunks of at most nmaps, each through the plan's map2alm (niter as planned)
+    for (int64_t s0 = 0; s0 < nr; s0 += P->nmaps) {
+        const int32_t Q = (int32_t)std::min<int64_t>(P->nmaps, nr - s0);
+        hipLaunchKernelGGL(k_transpose_block, dim3(nblk(P->npix, 32), nblk(Q, 32)), dim3(256), 0, st, nr, P->npix,
+                           s0, (int64_t)Q, d_f, P->d_delta.as<double>(), P->npix, 1);
+        LAUNCH_CHECK();
+        if ((rc = sht_stage(P, Q, st, false))) return rc;
+        hipLaunchKernelGGL(k_field_contract, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
+                           (int32_t)P->nmaps, Q, s0, nr, d_r, dr, P->d_gtab.as<double>(), P->d_maps.as<MapInfo>(),
+                           P->d_aoff.as<int64_t>(), P->d_outbase.as<int64_t>(), P->M2, P->a_final, d_out);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return SFB_OK;
+}
+
+extern "C" int sfb_field2anlm(sfb_plan* P, int64_t nr, const double* r, double dr, const double* f, double* out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !r || !f || !out) return fail(SFB_ERR_ARG, "bad field2anlm arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    int rc;
+    const size_t nin = (size_t)(nr * P->npix);
+    if ((rc = P->d_rr.ensure(sizeof(double) * (size_t)nr)) || (rc = P->h_fa.ensure(sizeof(double) * nin)) ||
+        (rc = P->h_fo.ensure(sizeof(double) * 2 * (size_t)P->nlmsize)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(P->d_rr.p, r, sizeof(double) * (size_t)nr, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->h_fa.p, f, sizeof(double) * nin, hipMemcpyHostToDevice, P->stream));
+    if ((rc = sfb_field2anlm_device(P, nr, P->d_rr.as<double>(), dr, P->h_fa.as<double>(), P->h_fo.as<double>(),
+                                    P->stream)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, P->h_fo.p, sizeof(double) * 2 * (size_t)P->nlmsize, hipMemcpyDeviceToHost,
+                           P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_anlm2field_device(sfb_plan* P, int64_t nr, const double* d_r, const double* d_fnlm,
+                                     double* d_f_out, void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !d_r || !d_fnlm || !d_f_out) return fail(SFB_ERR_ARG, "bad anlm2field arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    int rc;
+    if ((rc = radial_table(P, nr, d_r, st))) return rc;
+    const int32_t L = (int32_t)P->lmax;
+    const unsigned nbat = (unsigned)(2 * (L + 1));
+    const int32_t TQS = 8;
+    const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
+    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+    for (int64_t s0 = 0; s0 < nr; s0 += P->nmaps) {
+        const int32_t Q = (int32_t)std::min<int64_t>(P->nmaps, nr - s0);
+        // a^{(s)}_{lm} for every (l, m) of the chunk's shells, then alm2map per shell
+        hipLaunchKernelGGL(k_field_alm, dim3(nblk(Q, 64), (unsigned)(L + 1), (unsigned)(L + 1)), dim3(64), 0, st,
+                           (int32_t)P->nmax, L, Q, s0, nr, P->d_gtab.as<double>(), P->d_qbase.as<int32_t>(),
+                           P->d_lmaxn.as<int32_t>(), P->d_outbase.as<int64_t>(), P->d_aoff.as<int64_t>(), P->M2,
+                           d_fnlm, P->d_a.as<double>());
+        LAUNCH_CHECK();
+        launch_gemm(P->d_desc_synf.as<GemmDesc>(), nbat, 2 * Q, P->K, 1.0, 0.0, st);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)P->K, nblk(Q, TQS)), dim3(256), shs, st, P->d_X0.as<double>(),
+                           P->npix, Q, P->M2, P->K, L, P->d_rings.as<RingDev>(), 1.0, P->d_delta.as<double>(), TQS);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_transpose_block, dim3(nblk(P->npix, 32), nblk(Q, 32)), dim3(256), 0, st, nr, P->npix,
+                           s0, (int64_t)Q, P->d_delta.as<double>(), d_f_out, P->npix, 0);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return SFB_OK;
+}
+
+extern "C" int sfb_anlm2field(sfb_plan* P, int64_t nr, const double* r, const double* fnlm, double* f_out)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nr < 1 || !r || !fnlm || !f_out) return fail(SFB_ERR_ARG, "bad anlm2field arguments");
+    HIP_TRY(hipSetDevice(P->device));
+    int rc;
+    const size_t nout = (size_t)(nr * P->npix);
+    if ((rc = P->d_rr.ensure(sizeof(double) * (size_t)nr)) ||
+        (rc = P->h_fa.ensure(sizeof(double) * 2 * (size_t)P->nlmsize)) || (rc = P->h_fo.ensure(sizeof(double) * nout)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(P->d_rr.p, r, sizeof(double) * (size_t)nr, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->h_fa.p, fnlm, sizeof(double) * 2 * (size_t)P->nlmsize, hipMemcpyHostToDevice,
+                           P->stream));
+    if ((rc = sfb_anlm2field_device(P, nr, P->d_rr.as<double>(), P->h_fa.as<double>(), P->h_fo.as<double>(),
+                                    P->stream)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(f_out, P->h_fo.p, sizeof(double) * nout, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
     return SFB_OK;
 }

@@ -12,7 +12,8 @@ All numerical work runs in libsfb_mi355x.so (HIP for gfx950 plus native host
 setup); this package only marshals arguments.
 """
 from ._lib import SFBError, lib  # noqa: F401
-from .cat2anlm import Plan, ang2pix_ring, cat2amln, cat2amln_vjp, cat2nlm, get_plan  # noqa: F401
+from .cat2anlm import (Plan, anlm2field, ang2pix_ring, cat2amln, cat2amln_vjp, cat2nlm, field2anlm,  # noqa: F401
+                       get_plan)  # noqa: F401
 from .modes import (AnlmModes, GNL, Spline1D, estimate_nside, getidx, getlmsize, getnlm,  # noqa: F401
                     getnlmsize, isvalidnlm)
 from .windows import (ConfigurationSpaceModes, SeparableArray, get_rbounds, integrate_window,  # noqa: F401
@@ -21,5 +22,5 @@ from .windows import (ConfigurationSpaceModes, SeparableArray, get_rbounds, inte
 __all__ = [
     "AnlmModes", "GNL", "Spline1D", "estimate_nside", "getidx", "getlmsize", "getnlm", "getnlmsize",
     "isvalidnlm", "ConfigurationSpaceModes", "SeparableArray", "get_rbounds", "integrate_window",
-    "win_rhat_ln", "window_r", "cat2amln", "cat2amln_vjp", "cat2nlm", "Plan", "get_plan", "ang2pix_ring", "SFBError",
+    "win_rhat_ln", "window_r", "cat2amln", "cat2amln_vjp", "cat2nlm", "field2anlm", "anlm2field", "Plan", "get_plan", "ang2pix_ring", "SFBError",
 ]

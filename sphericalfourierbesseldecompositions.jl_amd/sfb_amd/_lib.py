@@ -46,6 +46,12 @@ SIGNATURES = {
     "sfb_cat2amln_device": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _i32, _P, _P, _P, _P]),
     "sfb_cat2amln_batch": (ctypes.c_int, [_P, _i32, _P, _P, _P, _P, _i32, _P, _P, _P]),
     "sfb_cat2amln_batch_device": (ctypes.c_int, [_P, _i32, _P, _P, _P, _P, _i32, _P, _P, _P, _P]),
+    "sfb_win_rhat_ln": (ctypes.c_int, [_P, _i64, _P, _dbl, _i32, _P, _P]),
+    "sfb_win_rhat_ln_device": (ctypes.c_int, [_P, _i64, _P, _dbl, _i32, _P, _P, _P]),
+    "sfb_field2anlm": (ctypes.c_int, [_P, _i64, _P, _dbl, _P, _P]),
+    "sfb_field2anlm_device": (ctypes.c_int, [_P, _i64, _P, _dbl, _P, _P, _P]),
+    "sfb_anlm2field": (ctypes.c_int, [_P, _i64, _P, _P, _P]),
+    "sfb_anlm2field_device": (ctypes.c_int, [_P, _i64, _P, _P, _P, _P]),
     "sfb_cat2amln_vjp": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _P, _P, _P, _P]),
     "sfb_cat2amln_vjp_device": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _P, _P, _P, _P]),
     "sfb_ang2pix_ring": (ctypes.c_int, [_i32, _i64, _i64, _P, _P, _P]),

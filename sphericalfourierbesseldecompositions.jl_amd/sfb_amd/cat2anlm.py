@@ -110,6 +110,52 @@ class Plan:
                                             dp(d_win_a), dp(d_win_b), dp(d_out),
                                             None if stream is None else ctypes.c_void_p(stream)))
 
+    # -- window and field transforms (src/windows.jl:244-270, src/cat2anlm.jl:323-422)
+    def win_rhat_ln(self, win, wmodes):
+        """win_rhat_ln on the device: dense (nr, npix) window -> (npix, lmax+1, nmax)
+        array, or SeparableArray(phi, mask) -> SeparableArray(mask, w_ln)."""
+        am = self.amodes
+        r = L.f64(wmodes.r)
+        if isinstance(win, SeparableArray):
+            phi = L.f64(np.asarray(win.arr1).reshape(-1))
+            if phi.size != r.size:
+                raise ValueError("phi must have one value per radial bin")
+            out = np.empty((am.lmax + 1) * am.nmax)
+            L.check(L.lib().sfb_win_rhat_ln(self._h, r.size, L.ptr(r), float(wmodes.dr), L.SFB_WIN_SEPARABLE,
+                                            L.ptr(phi), L.ptr(out)))
+            return SeparableArray(np.asarray(win.arr2, dtype=float), out.reshape(am.lmax + 1, am.nmax, order="F"),
+                                  name1="mask", name2="w_ln")
+        w = np.asarray(win, dtype=np.float64)
+        if w.shape != (r.size, self.npix):
+            raise ValueError(f"dense window must be (nr, npix) = {(r.size, self.npix)}")
+        wf = np.asfortranarray(w)
+        out = np.empty((self.npix, am.lmax + 1, am.nmax), order="F")
+        L.check(L.lib().sfb_win_rhat_ln(self._h, r.size, L.ptr(r), float(wmodes.dr), L.SFB_WIN_DENSE,
+                                        ctypes.c_void_p(wf.ctypes.data), ctypes.c_void_p(out.ctypes.data)))
+        return out
+
+    def field2anlm(self, f_xyz, wmodes):
+        """field2anlm (src/cat2anlm.jl:323-358): nr x npix field -> a_nlm."""
+        f = np.asfortranarray(np.asarray(f_xyz, dtype=np.float64))
+        if f.shape != (wmodes.nr, self.npix):
+            raise ValueError(f"field must be (nr, npix) = {(wmodes.nr, self.npix)}")
+        r = L.f64(wmodes.r)
+        out = np.empty(2 * self.nlmsize)
+        L.check(L.lib().sfb_field2anlm(self._h, r.size, L.ptr(r), float(wmodes.dr), ctypes.c_void_p(f.ctypes.data),
+                                       L.ptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def anlm2field(self, f_nlm, wmodes):
+        """anlm2field (src/cat2anlm.jl:383-422): a_nlm -> nr x npix field."""
+        c = np.asarray(f_nlm, dtype=np.complex128).reshape(-1)
+        if c.size != self.nlmsize:
+            raise ValueError(f"f_nlm must have {self.nlmsize} entries")
+        cot = L.f64(np.stack([c.real, c.imag], axis=1).reshape(-1))
+        r = L.f64(wmodes.r)
+        out = np.empty((r.size, self.npix), order="F")
+        L.check(L.lib().sfb_anlm2field(self._h, r.size, L.ptr(r), L.ptr(cot), ctypes.c_void_p(out.ctypes.data)))
+        return out
+
     def cat2amln_batch(self, catalogues, nbars, win_rhat_ln, weights=None):
         """cat2amln of every catalogue in `catalogues` (3 x N_b arrays) with its own nbar
         and the shared window; returns a (B, nlmsize) complex array."""
@@ -198,6 +244,16 @@ def get_plan(amodes, niter=3, precision="fp64", device=0) -> Plan:
         p = Plan(amodes, niter, precision, device)
         _plans[key] = p
     return p
+
+
+def field2anlm(f_xyz, wmodes, amodes, *, niter=3, device=0):
+    """field2anlm(f_xyz, wmodes, amodes) (src/cat2anlm.jl:361-370) on the device."""
+    return get_plan(amodes, niter, "fp64", device).field2anlm(f_xyz, wmodes)
+
+
+def anlm2field(f_nlm, wmodes, amodes, *, device=0):
+    """anlm2field(f_nlm, wmodes, amodes) (src/cat2anlm.jl:373-422) on the device."""
+    return get_plan(amodes, 3, "fp64", device).anlm2field(f_nlm, wmodes)
 
 
 def cat2amln_vjp(rθϕ, amodes, nbar, dL_danlm, weight=None, *, niter=3, precision="fp64", device=0):

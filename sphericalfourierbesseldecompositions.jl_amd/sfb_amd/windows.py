@@ -86,10 +86,16 @@ def integrate_window(win, wmodes):
     return dr * domega * np.sum(radial * r ** 2)
 
 
-def win_rhat_ln(win, wmodes, amodes: AnlmModes):
+def win_rhat_ln(win, wmodes, amodes: AnlmModes, *, device=None):
     """W[p, l, n-1] = dr sum_r r^2 g_nl(r) win[r, p] (src/windows.jl:244-257), NaN where
     l > lmax_n[n].  A separable window ``SeparableArray(phi[r], mask[p])`` gives
-    ``SeparableArray(mask, w_ln[l, n-1], name1="mask", name2="w_ln")`` (:260-270)."""
+    ``SeparableArray(mask, w_ln[l, n-1], name1="mask", name2="w_ln")`` (:260-270).
+
+    device=None evaluates on the host (NumPy); device=k runs sfb_win_rhat_ln on GPU k
+    (one MFMA GEMM per n for a dense window)."""
+    if device is not None:
+        from .cat2anlm import get_plan
+        return get_plan(amodes, device=device).win_rhat_ln(win, wmodes)
     r, dr = window_r(wmodes)
     gnl = amodes.basisfunctions
     if isinstance(win, SeparableArray):
