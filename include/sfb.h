@@ -135,6 +135,34 @@ int sfb_cat2amln_batch_device(sfb_plan* p, int32_t B, const int64_t* N_b, const 
                               const double* nbar_b, int32_t win_kind, const double* d_win_a,
                               const double* d_win_b, double* d_anlm_out, void* stream);
 
+/* ---- catalogue preparation and the C_l(n, n') consumer ------------------------ */
+
+/* xyz2rtp(xyz) (src/SphericalFourierBesselDecompositions.jl:517-532): 3 x N Cartesian
+ * positions -> 3 x N (r, theta, phi), phi = atan(y/r, x/r) in (-pi, pi]. */
+int sfb_xyz2rtp(int32_t device, int64_t N, const double* xyz, double* rtp);
+
+/* winweights2galweights(weights, wmodes, rθϕ) (src/cat2anlm.jl:218-238): w_out[i] =
+ * weights[bin, pix] for the nr x npix (column-major) weights, bin from the bin edges
+ * rbounds[nr+1] (get_rbounds(wmodes), r == rmax -> the last bin).  A radius outside the
+ * bins is an error (the reference's BoundsError). */
+int sfb_winweights2galweights(int32_t device, int64_t nside, int64_t nr, const double* rbounds,
+                              const double* weights, int64_t N, const double* rtp, double* w_out);
+
+/* apply_window(rθϕ, win, rmin, rmax, r, Δr) (src/windows.jl:169-210): keeps galaxy i when
+ * rmin <= r <= rmax and u[i] <= win[idx_r, pix] / maximum(win), idx_r = ceil((r - rmin)/Δr)
+ * (0 -> 1, > nr -> rejected).  u[N] are the caller's uniform draws (the reference draws
+ * them from its own RNG).  rtp_out receives the kept galaxies in input order, *n_out
+ * their count. */
+int sfb_apply_window(int32_t device, int64_t nside, int64_t nr, double rmin, double rmax, double dr,
+                     const double* win, int64_t N, const double* rtp, const double* u, double* rtp_out,
+                     int64_t* n_out);
+
+/* amln2clnn(anlm1, anlm2, cmodes) (src/SphericalFourierBesselDecompositions.jl:130-150)
+ * for the (l, n1, n2) triples lnn[3 x nlnn] of a ClnnModes over the plan's modes:
+ * C = (Re(a1_l0 a2_l0) + 2 sum_{m>=1} Re(a1_lm conj(a2_lm))) / (2l + 1). */
+int sfb_amln2clnn(sfb_plan* p, int64_t nlnn, const int64_t* lnn, const double* anlm1, const double* anlm2,
+                  double* clnn);
+
 /* ---- adjoint ----------------------------------------------------------------- */
 
 /* Vector-Jacobian product of cat2amln at (rθϕ, w, nbar): given dL_danlm (2*nlmsize

@@ -1403,6 +1403,131 @@ __global__ void k_field_alm(int32_t nmax, int32_t L, int32_t Q, int64_t s0, int6
 }
 
 // ---------------------------------------------------------------------------
+// Catalogue preparation and the power-spectrum consumer: xyz2rtp
+// (src/SphericalFourierBesselDecompositions.jl:517-532), apply_window
+// (src/windows.jl:169-210), winweights2galweights (src/cat2anlm.jl:218-238) and
+// amln2clnn (src/SphericalFourierBesselDecompositions.jl:114-150).
+// ---------------------------------------------------------------------------
+__global__ void k_xyz2rtp(int64_t N, const double* __restrict__ xyz, double* __restrict__ rtp)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    const double r = sqrt(x * x + y * y + z * z);
+    rtp[3 * i] = r;
+    rtp[3 * i + 1] = acos(z / r);
+    rtp[3 * i + 2] = atan2(y / r, x / r);
+}
+
+// searchsortedlast: number of entries of the sorted a[0..n) that are <= v
+__device__ __forceinline__ int64_t search_last(const double* __restrict__ a, int64_t n, double v)
+{
+    int64_t lo = 0, hi = n;  // answer in [lo, hi]
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (a[mid - 1] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// w_out[i] = weights[bin, pix] (nr x npix, column-major), bin by the caller's bin edges
+// rbounds[nr+1] (r == rmax -> the last bin); flag |= 1 for a radius outside the bins
+__global__ void k_galweights(int64_t N, const double* __restrict__ rtp, int64_t nside, int64_t nr,
+                             const double* __restrict__ rbounds, const double* __restrict__ weights,
+                             double* __restrict__ w_out, int32_t* __restrict__ flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double r = rtp[3 * i], th = rtp[3 * i + 1], ph = rtp[3 * i + 2];
+    if (!isfinite(th) || !isfinite(ph)) {
+        atomicOr(flag, 2);
+        w_out[i] = __builtin_nan("");
+        return;
+    }
+    const int64_t pix = ang2pix_ring_dev(nside, th, ph);
+    const int64_t bin = (r == rbounds[nr]) ? nr : search_last(rbounds, nr + 1, r);
+    if (bin < 1 || bin > nr) {
+        atomicOr(flag, 1);
+        w_out[i] = __builtin_nan("");
+        return;
+    }
+    w_out[i] = weights[(bin - 1) + nr * pix];
+}
+
+// apply_window's acceptance: keep[i] = (rmin <= r <= rmax) && u[i] <= win[idx_r, pix] / max(win)
+__global__ void k_window_keep(int64_t N, const double* __restrict__ rtp, const double* __restrict__ u, int64_t nside,
+                              int64_t nr, double rmin, double rmax, double dr, const double* __restrict__ win,
+                              double ooWmax, int32_t* __restrict__ keep)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double r = rtp[3 * i], th = rtp[3 * i + 1], ph = rtp[3 * i + 2];
+    int32_t k = 0;
+    if (rmin <= r && r <= rmax && isfinite(th) && isfinite(ph)) {
+        int64_t idx_r = (int64_t)ceil((r - rmin) / dr);
+        if (idx_r == 0) idx_r = 1;
+        if (idx_r <= nr) {
+            const int64_t pix = ang2pix_ring_dev(nside, th, ph);
+            k = (u[i] <= win[(idx_r - 1) + nr * pix] * ooWmax) ? 1 : 0;
+        }
+    }
+    keep[i] = k;
+}
+
+// stable compaction of the kept galaxies: count per block of 1024, scan, scatter
+__global__ __launch_bounds__(1024) void k_keep_count(int64_t N, const int32_t* __restrict__ keep,
+                                                     int64_t* __restrict__ cnt)
+{
+    __shared__ int32_t part[1024];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    part[threadIdx.x] = (i < N) ? keep[i] : 0;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[blockIdx.x] = part[0];
+}
+
+__global__ __launch_bounds__(1024) void k_keep_scatter(int64_t N, const int32_t* __restrict__ keep,
+                                                       const int64_t* __restrict__ boff,
+                                                       const double* __restrict__ rtp, double* __restrict__ out)
+{
+    __shared__ int32_t sc[1024];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const int32_t k = (i < N) ? keep[i] : 0;
+    sc[threadIdx.x] = k;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+        const int32_t v = ((int)threadIdx.x >= off) ? sc[threadIdx.x - off] : 0;
+        __syncthreads();
+        sc[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (k) {
+        const int64_t o = boff[blockIdx.x] + sc[threadIdx.x] - 1;
+        out[3 * o] = rtp[3 * i];
+        out[3 * o + 1] = rtp[3 * i + 1];
+        out[3 * o + 2] = rtp[3 * i + 2];
+    }
+}
+
+// clnn[i] = (Re(a1_l0 a2_l0) + 2 sum_{m>=1} Re(a1_lm conj(a2_lm))) / (2l + 1) for
+// (l, n1, n2) = lnn[:, i] (alm2cl, src/SphericalFourierBesselDecompositions.jl:114-128)
+__global__ void k_amln2clnn(int64_t nlnn, const int64_t* __restrict__ lnn, const int32_t* __restrict__ qbase,
+                            const int64_t* __restrict__ out_base, const double* __restrict__ a1,
+                            const double* __restrict__ a2, double* __restrict__ clnn)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlnn) return;
+    const int64_t l = lnn[3 * i], n1 = lnn[3 * i + 1], n2 = lnn[3 * i + 2];
+    const int64_t b1 = 2 * out_base[qbase[n1 - 1] + l], b2 = 2 * out_base[qbase[n2 - 1] + l];
+    double c = a1[b1] * a2[b2] - a1[b1 + 1] * a2[b2 + 1];
+    for (int64_t m = 1; m <= l; ++m) c += 2.0 * (a1[b1 + 2 * m] * a2[b2 + 2 * m] + a1[b1 + 2 * m + 1] * a2[b2 + 2 * m + 1]);
+    clnn[i] = c / (double)(2 * l + 1);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 struct DevBuf {
@@ -1456,6 +1581,7 @@ struct sfb_plan {
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
     DevBuf d_qbase, d_lmaxn;       // per n: map index of (n, 0) and lmax_n (int32)
+    std::vector<int32_t> lmaxn_h;  // lmax_n on the host
     DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
@@ -1717,6 +1843,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             ln[(size_t)n] = (int32_t)lmax_n[n];
             acc += (int32_t)lmax_n[n] + 1;
         }
+        P->lmaxn_h = ln;
         if ((rc = P->d_qbase.ensure(4 * qb.size())) || (rc = P->d_lmaxn.ensure(4 * ln.size()))) return rc;
         HIP_TRY(hipMemcpy(P->d_qbase.p, qb.data(), 4 * qb.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->d_lmaxn.p, ln.data(), 4 * ln.size(), hipMemcpyHostToDevice));
@@ -2805,6 +2932,130 @@ extern "C" int sfb_anlm2field(sfb_plan* P, int64_t nr, const double* r, const do
         return rc;
     HIP_TRY(hipMemcpyAsync(f_out, P->h_fo.p, sizeof(double) * nout, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Catalogue preparation and amln2clnn entry points
+// ---------------------------------------------------------------------------
+extern "C" int sfb_xyz2rtp(int32_t device, int64_t N, const double* xyz, double* rtp)
+{
+    sfb::clear_error();
+    if (N < 0 || (N > 0 && (!xyz || !rtp))) return fail(SFB_ERR_ARG, "bad xyz2rtp arguments");
+    if (N == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(device));
+    double *d_in = nullptr, *d_out = nullptr;
+    HIP_TRY(hipMalloc(&d_in, sizeof(double) * 3 * (size_t)N));
+    hipError_t e = hipMalloc(&d_out, sizeof(double) * 3 * (size_t)N);
+    if (e == hipSuccess) e = hipMemcpy(d_in, xyz, sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_xyz2rtp, dim3(nblk(N, 256)), dim3(256), 0, 0, N, d_in, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(rtp, d_out, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToHost);
+    (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(SFB_ERR_DEVICE, "xyz2rtp: %s", hipGetErrorString(e));
+    return SFB_OK;
+}
+
+extern "C" int sfb_winweights2galweights(int32_t device, int64_t nside, int64_t nr, const double* rbounds,
+                                         const double* weights, int64_t N, const double* rtp, double* w_out)
+{
+    sfb::clear_error();
+    if (nside < 1 || nr < 1 || !rbounds || !weights || N < 0 || (N > 0 && (!rtp || !w_out)))
+        return fail(SFB_ERR_ARG, "bad winweights2galweights arguments");
+    if (N == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(device));
+    const int64_t npix = 12 * nside * nside;
+    DevBuf b, wt, in, out, fl;
+    int rc;
+    if ((rc = b.ensure(8 * (size_t)(nr + 1))) || (rc = wt.ensure(8 * (size_t)(nr * npix))) ||
+        (rc = in.ensure(24 * (size_t)N)) || (rc = out.ensure(8 * (size_t)N)) || (rc = fl.ensure(4)))
+        return rc;
+    int32_t flag = 0;
+    HIP_TRY(hipMemcpy(b.p, rbounds, 8 * (size_t)(nr + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(wt.p, weights, 8 * (size_t)(nr * npix), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(in.p, rtp, 24 * (size_t)N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fl.p, &flag, 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_galweights, dim3(nblk(N, 256)), dim3(256), 0, 0, N, in.as<double>(), nside, nr,
+                       b.as<double>(), wt.as<double>(), out.as<double>(), fl.as<int32_t>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpy(w_out, out.p, 8 * (size_t)N, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&flag, fl.p, 4, hipMemcpyDeviceToHost));
+    b.release(); wt.release(); in.release(); out.release(); fl.release();
+    if (flag & 2) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
+    if (flag & 1) return fail(SFB_ERR_ARG, "BoundsError: r out of bounds of the radial bins");
+    return SFB_OK;
+}
+
+extern "C" int sfb_apply_window(int32_t device, int64_t nside, int64_t nr, double rmin, double rmax, double dr,
+                                const double* win, int64_t N, const double* rtp, const double* u, double* rtp_out,
+                                int64_t* n_out)
+{
+    sfb::clear_error();
+    if (nside < 1 || nr < 1 || !win || N < 0 || !n_out || (N > 0 && (!rtp || !u || !rtp_out)))
+        return fail(SFB_ERR_ARG, "bad apply_window arguments");
+    *n_out = 0;
+    if (N == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(device));
+    const int64_t npix = 12 * nside * nside;
+    double wmax = -INFINITY;
+    for (int64_t i = 0; i < nr * npix; ++i) wmax = std::max(wmax, win[i]);
+    const int64_t nb = (N + 1023) / 1024;
+    DevBuf dw, din, du, dk, dc, dof, dout;
+    int rc;
+    if ((rc = dw.ensure(8 * (size_t)(nr * npix))) || (rc = din.ensure(24 * (size_t)N)) ||
+        (rc = du.ensure(8 * (size_t)N)) || (rc = dk.ensure(4 * (size_t)N)) || (rc = dc.ensure(8 * (size_t)nb)) ||
+        (rc = dof.ensure(8 * (size_t)(nb + 1))) || (rc = dout.ensure(24 * (size_t)N)))
+        return rc;
+    HIP_TRY(hipMemcpy(dw.p, win, 8 * (size_t)(nr * npix), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(din.p, rtp, 24 * (size_t)N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(du.p, u, 8 * (size_t)N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_window_keep, dim3(nblk(N, 256)), dim3(256), 0, 0, N, din.as<double>(), du.as<double>(), nside,
+                       nr, rmin, rmax, dr, dw.as<double>(), 1.0 / wmax, dk.as<int32_t>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nb), dim3(1024), 0, 0, N, dk.as<int32_t>(), dc.as<int64_t>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, 0, (int32_t)nb, dc.as<int64_t>(), dof.as<int64_t>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nb), dim3(1024), 0, 0, N, dk.as<int32_t>(), dof.as<int64_t>(),
+                       din.as<double>(), dout.as<double>());
+    LAUNCH_CHECK();
+    int64_t kept = 0;
+    HIP_TRY(hipMemcpy(&kept, dof.as<int64_t>() + nb, 8, hipMemcpyDeviceToHost));
+    if (kept > 0) HIP_TRY(hipMemcpy(rtp_out, dout.p, 24 * (size_t)kept, hipMemcpyDeviceToHost));
+    *n_out = kept;
+    return SFB_OK;
+}
+
+extern "C" int sfb_amln2clnn(sfb_plan* P, int64_t nlnn, const int64_t* lnn, const double* anlm1, const double* anlm2,
+                             double* clnn)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (nlnn < 0 || (nlnn > 0 && (!lnn || !anlm1 || !anlm2 || !clnn))) return fail(SFB_ERR_ARG, "bad amln2clnn arguments");
+    for (int64_t i = 0; i < nlnn; ++i) {
+        const int64_t l = lnn[3 * i], n1 = lnn[3 * i + 1], n2 = lnn[3 * i + 2];
+        if (n1 < 1 || n2 < 1 || n1 > P->nmax || n2 > P->nmax || l < 0 || l > P->lmaxn_h[(size_t)(n1 - 1)] ||
+            l > P->lmaxn_h[(size_t)(n2 - 1)])
+            return fail(SFB_ERR_ARG, "lnn entry %lld is not a mode of the plan", (long long)i);
+    }
+    if (nlnn == 0) return SFB_OK;
+    HIP_TRY(hipSetDevice(P->device));
+    DevBuf dl, d1, d2, dc;
+    int rc;
+    if ((rc = dl.ensure(24 * (size_t)nlnn)) || (rc = d1.ensure(16 * (size_t)P->nlmsize)) ||
+        (rc = d2.ensure(16 * (size_t)P->nlmsize)) || (rc = dc.ensure(8 * (size_t)nlnn)))
+        return rc;
+    HIP_TRY(hipMemcpy(dl.p, lnn, 24 * (size_t)nlnn, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d1.p, anlm1, 16 * (size_t)P->nlmsize, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2.p, anlm2, 16 * (size_t)P->nlmsize, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_amln2clnn, dim3(nblk(nlnn, 256)), dim3(256), 0, 0, nlnn, dl.as<int64_t>(),
+                       P->d_qbase.as<int32_t>(), P->d_outbase.as<int64_t>(), d1.as<double>(), d2.as<double>(),
+                       dc.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpy(clnn, dc.p, 8 * (size_t)nlnn, hipMemcpyDeviceToHost));
     return SFB_OK;
 }
 

@@ -1,0 +1,81 @@
+"""ClnnModes and amln2clnn: the pseudo-power-spectrum consumer of a_nlm.
+
+ClnnModes (src/modes.jl:280-447): the (l, n1, n2) triples with |n2 - n1| <= Δnmax and
+|k_n2l - k_n1l| <= Δkmax (n1 <= n2 for a single AnlmModes), sorted by l, then Δn, then
+n1 (sort_lnn, :342-362).  amln2clnn (src/SphericalFourierBesselDecompositions.jl:114-150)
+runs on the device.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .modes import AnlmModes
+
+
+class ClnnModes:
+    def __init__(self, amodes: AnlmModes, *, Δkmax=np.inf, Δnmax=np.iinfo(np.int64).max):
+        self.amodes = amodes
+        lnn = []
+        dk_out, dn_out = -np.inf, 0
+        for l in range(amodes.lmax + 1):
+            nmax_l = int(amodes.nmax_l[l])
+            for n1 in range(1, nmax_l + 1):
+                for n2 in range(n1, nmax_l + 1):
+                    dk = amodes.knl[n2 - 1, l] - amodes.knl[n1 - 1, l]
+                    if abs(dk) <= Δkmax and abs(n2 - n1) <= Δnmax:
+                        lnn.append((l, n1, n2))
+                        dk_out = max(abs(dk), dk_out)
+                        dn_out = max(abs(n2 - n1), dn_out)
+        lnn.sort(key=lambda t: (t[0], t[2] - t[1], t[1]))
+        self.lnn = np.array(lnn, dtype=np.int64).reshape(-1, 3)
+        self.Δkmax, self.Δnmax = dk_out, dn_out
+        self._index = {tuple(t): i + 1 for i, t in enumerate(lnn)}
+
+    def getlnnsize(self):
+        return self.lnn.shape[0]
+
+
+def getlnnsize(cmodes: ClnnModes):
+    return cmodes.getlnnsize()
+
+
+def getlnn(cmodes: ClnnModes, idx):
+    """(l, n1, n2) of the 1-based index idx."""
+    l, n1, n2 = cmodes.lnn[idx - 1]
+    return int(l), int(n1), int(n2)
+
+
+def getidx(cmodes: ClnnModes, l, n1, n2):
+    """1-based index of (l, n1, n2); n1 and n2 are swapped when n1 > n2."""
+    if n1 > n2:
+        n1, n2 = n2, n1
+    try:
+        return cmodes._index[(int(l), int(n1), int(n2))]
+    except KeyError:
+        raise KeyError(f"Cannot find index of (l, n1, n2) = {(l, n1, n2)}") from None
+
+
+def amln2clnn(anlm1, *args, device=0):
+    """amln2clnn(anlm, cmodes) or amln2clnn(anlm1, anlm2, cmodes) on the device."""
+    from .cat2anlm import get_plan
+    if len(args) == 1:
+        anlm2, cmodes = anlm1, args[0]
+    else:
+        anlm2, cmodes = args
+    plan = get_plan(cmodes.amodes, device=device)
+
+    def inter(a):
+        a = np.asarray(a, dtype=np.complex128).reshape(-1)
+        if a.size != plan.nlmsize:
+            raise ValueError(f"a_nlm must have {plan.nlmsize} entries")
+        return L.f64(np.stack([a.real, a.imag], axis=1).reshape(-1))
+
+    a1, a2 = inter(anlm1), inter(anlm2)
+    lnn = np.ascontiguousarray(cmodes.lnn, dtype=np.int64)
+    out = np.empty(lnn.shape[0])
+    L.check(L.lib().sfb_amln2clnn(plan.handle, lnn.shape[0], L.ptr(lnn), L.ptr(a1), L.ptr(a2), L.ptr(out)))
+    return out
+
+
+anlm2clnn = amln2clnn
