@@ -17,8 +17,9 @@
 module SFBMI355X
 
 import ChainRulesCore
+import SphericalFourierBesselDecompositions as SFB
 
-export cat2amln, cat2nlm, cat2amln_vjp
+export cat2amln, cat2nlm, cat2amln_vjp, cat2amln_batch, win_rhat_ln, field2anlm, anlm2field, amln2clnn
 
 const libsfb = get(ENV, "SFB_MI355X_LIB",
                    joinpath(@__DIR__, "..", "sfb_amd", "libsfb_mi355x.so"))
@@ -140,6 +141,98 @@ function cat2amln_vjp(rθϕ, amodes, nbar, dL_danlm, weight=ones(eltype(rθϕ), 
     check(rc)
     return dr, dw, dnbar[]
 end
+
+"""
+    cat2amln_batch(catalogs, amodes, nbars, win_rhat_ln, weights=nothing; niter=3, precision=:fp64, device=0)
+
+cat2amln of every 3 x N_b catalogue in `catalogs` (with its nbar) through one plan;
+returns a vector of a_nlm vectors.  Equivalent to calling cat2amln per mock.
+"""
+function cat2amln_batch(catalogs, amodes, nbars, win_rhat_ln, weights=nothing;
+                        niter::Integer=3, precision::Symbol=:fp64, device::Integer=0)
+    p = plan(amodes; niter, precision, device)
+    B = length(catalogs)
+    Nb = Int64[size(c, 2) for c in catalogs]
+    rtp = Matrix{Float64}(reduce(hcat, catalogs; init=zeros(3, 0)))
+    w = weights === nothing ? Float64[] : Vector{Float64}(reduce(vcat, weights))
+    nb = Vector{Float64}(collect(nbars))
+    kind, a, b = window_arrays(win_rhat_ln)
+    out = Vector{Float64}(undef, 2 * p.nlmsize * max(B, 1))
+    GC.@preserve Nb rtp w nb a b out begin
+        rc = ccall((:sfb_cat2amln_batch, libsfb), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int32, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}),
+                   p.ptr, Int32(B), Nb, isempty(rtp) ? C_NULL : pointer(rtp), isempty(w) ? C_NULL : pointer(w),
+                   nb, kind, pointer(a), kind == 1 ? pointer(b) : C_NULL, out)
+    end
+    check(rc)
+    z = reinterpret(ComplexF64, out)
+    return [z[(i - 1) * p.nlmsize + 1:i * p.nlmsize] for i in 1:B]
+end
+
+"""
+    win_rhat_ln(win, wmodes, amodes; device=0)
+
+The reference's win_rhat_ln (src/windows.jl:244-270) on the GPU: a dense nr x npix
+window gives the (npix, lmax+1, nmax) array (NaN where l > lmax_n[n]); a SeparableArray
+(phi, mask) gives SeparableArray(mask, w_ln).
+"""
+function win_rhat_ln(win, wmodes, amodes; device::Integer=0)
+    p = plan(amodes; device)
+    r = Vector{Float64}(collect(wmodes.r))
+    if hasfield(typeof(win), :arr1) && hasfield(typeof(win), :arr2)
+        phi = Vector{Float64}(vec(getfield(win, :arr1)))
+        out = Matrix{Float64}(undef, amodes.lmax + 1, amodes.nmax)
+        rc = ccall((:sfb_win_rhat_ln, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Float64, Int32, Ptr{Float64},
+                   Ptr{Float64}), p.ptr, length(r), r, Float64(wmodes.Δr), Int32(1), phi, out)
+        check(rc)
+        return SFB.SeparableArray(getfield(win, :arr2), out, name1=:mask, name2=:w_ln)
+    end
+    w = Matrix{Float64}(win)
+    out = Array{Float64}(undef, p.npix, amodes.lmax + 1, amodes.nmax)
+    rc = ccall((:sfb_win_rhat_ln, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Float64, Int32, Ptr{Float64},
+               Ptr{Float64}), p.ptr, length(r), r, Float64(wmodes.Δr), Int32(0), w, out)
+    check(rc)
+    return out
+end
+
+"""field2anlm(f_xyz, wmodes, amodes) (src/cat2anlm.jl:361-370) on the GPU."""
+function field2anlm(f_xyz, wmodes, amodes; device::Integer=0)
+    p = plan(amodes; device)
+    r = Vector{Float64}(collect(wmodes.r))
+    f = Matrix{Float64}(f_xyz)
+    out = Vector{Float64}(undef, 2 * p.nlmsize)
+    rc = ccall((:sfb_field2anlm, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Float64, Ptr{Float64}, Ptr{Float64}),
+               p.ptr, length(r), r, Float64(wmodes.Δr), f, out)
+    check(rc)
+    return copy(reinterpret(ComplexF64, out))
+end
+
+"""anlm2field(f_nlm, wmodes, amodes) (src/cat2anlm.jl:373-422) on the GPU."""
+function anlm2field(f_nlm, wmodes, amodes; device::Integer=0)
+    p = plan(amodes; device)
+    r = Vector{Float64}(collect(wmodes.r))
+    c = Vector{ComplexF64}(f_nlm)
+    out = Matrix{Float64}(undef, length(r), p.npix)
+    rc = ccall((:sfb_anlm2field, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{ComplexF64}, Ptr{Float64}),
+               p.ptr, length(r), r, c, out)
+    check(rc)
+    return out
+end
+
+"""amln2clnn(anlm1, anlm2, cmodes) (src/SphericalFourierBesselDecompositions.jl:130-150) on
+the GPU, for a ClnnModes over a single AnlmModes."""
+function amln2clnn(anlm1, anlm2, cmodes; device::Integer=0)
+    p = plan(cmodes.amodesA; device)
+    lnn = Matrix{Int64}(cmodes.lnn)
+    a1, a2 = Vector{ComplexF64}(anlm1), Vector{ComplexF64}(anlm2)
+    out = Vector{Float64}(undef, size(lnn, 2))
+    rc = ccall((:sfb_amln2clnn, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64},
+               Ptr{Float64}), p.ptr, size(lnn, 2), lnn, a1, a2, out)
+    check(rc)
+    return out
+end
+amln2clnn(anlm, cmodes; kw...) = amln2clnn(anlm, anlm, cmodes; kw...)
 
 function ChainRulesCore.rrule(::typeof(cat2amln), rθϕ, amodes, nbar, win_rhat_ln,
                               weight=ones(eltype(rθϕ), size(rθϕ, 2)); kw...)
