@@ -817,7 +817,8 @@ template <> struct MfmaT<true> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 8, GT_BNW = 16 * GT_NTM;
+constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
+constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
 template <bool F32, int NT, bool AF = false>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
@@ -832,10 +833,11 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
     typename M::V acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = M::zero();
-    T ra[4], rb[NT];
+    constexpr int BLD = BNW * GT_BK / 256;  // B elements per thread per k-step
+    T ra[GT_ALD], rb[BLD];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
@@ -845,7 +847,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
                 ra[u] = (gk < d.K && gi < rows) ? (T)d.A[(int64_t)gk * d.lda + gi] : (T)0;
         }
 #pragma unroll
-        for (int u = 0; u < NT; ++u) {
+        for (int u = 0; u < BLD; ++u) {
             const int e = tid + 256 * u;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
@@ -855,12 +857,12 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
     fetch(0);
     for (int k0 = 0; k0 < d.K; k0 += GT_BK) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
             As[e >> 6][e & 63] = ra[u];
         }
 #pragma unroll
-        for (int u = 0; u < NT; ++u) {
+        for (int u = 0; u < BLD; ++u) {
             const int e = tid + 256 * u;
             Bs[e / BNW][e % BNW] = rb[u];
         }
@@ -901,10 +903,7 @@ __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, d
         case 2: gemm_tall_body<F32, 2, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
         case 3: gemm_tall_body<F32, 3, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
         case 4: gemm_tall_body<F32, 4, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 5: gemm_tall_body<F32, 5, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 6: gemm_tall_body<F32, 6, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 7: gemm_tall_body<F32, 7, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, 8, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
@@ -2099,9 +2098,9 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         P->logM = 0;
         while ((1LL << P->logM) < M) ++P->logM;
         const int64_t budget = std::min<int64_t>(P->lds_max, 98304);
-        // 2 TQF sequences of M complex values plus the 2 N4 twiddles; TQF <= 4 keeps
-        // several blocks resident per CU
-        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, (budget - 2 * N4 * 16) / (2 * M * 16)));
+        // 2 TQF sequences of M complex values plus the N4 + 1 twiddles; TQF <= 2 keeps
+        // about six blocks resident per CU
+        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(2, (budget - 2 * N4 * 16) / (2 * M * 16)));
         // polar: 2 npr twiddles + 2 rings x TQP maps x (npr+1) doubles, npr <= 4 nside - 4
         const int64_t per = 2 * (N4 + 1) * 8;
         P->TQP = (int32_t)std::max<int64_t>(2, std::min<int64_t>(16, (budget - 2 * N4 * 16) / per) & ~1LL);
@@ -2368,12 +2367,14 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
     const int64_t M = 2 * nside;
     const size_t vb = f32 ? sizeof(float) : sizeof(double);
-    const size_t shf = 2 * vb * (2 * (size_t)P->TQF * (size_t)M + 2 * (size_t)M + 1);
+    // FP32 sequences take half the LDS: twice as many per block at the same occupancy
+    const int32_t tqf = f32 ? 2 * P->TQF : P->TQF;
+    const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)M + 2 * (size_t)M + 1);
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(256), shf, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->TQF, k_eq0, P->logM);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
@@ -2421,7 +2422,9 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
     P->a_final = P->d_a.as<double>();
-    const bool mixed = P->precision == SFB_PREC_MIXED;
+    // the Jacobi GEMMs stream FP64 a, G, R and are bandwidth-bound: FP32 operands gain
+    // nothing here, so both paths run them in FP64
+    const bool mixed = false;
     dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
