@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--cpu-maps", type=int, default=48, help="maps the CPU baseline sample runs")
+    ap.add_argument("--cpu-maps", type=int, default=480, help="maps the CPU baseline sample runs (about 13 s on 16 threads)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mocks", type=int, default=64,
