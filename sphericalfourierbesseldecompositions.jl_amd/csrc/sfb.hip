@@ -381,22 +381,72 @@ struct ScatterBlock {
     int32_t qa, qb, chunk, pad;
 };
 
+template <typename R> struct Vec4T;
+template <> struct Vec4T<double> { typedef double4 T; };
+template <> struct Vec4T<float> { typedef float4 T; };
+
+// Records in LDS.  An FP64 record is two 16-byte halves; stored in order, every
+// record's first half would sit in an even 16-byte bank group, so each of the two
+// reads of a random gather would spread over half the bank groups.  Odd records
+// are stored with their halves swapped: the read of (d, c) at 32 i + 16 (i & 1)
+// then lands on every bank group alike, and so does the read of (b, a).
+// FP32 records are one 16-byte read and stay in order.
+__device__ __forceinline__ void lds_rec_put(double4* sm, int k, double4 v)
+{
+    double2* s2 = (double2*)sm;
+    const int a = 2 * k + (k & 1);
+    s2[a] = make_double2(v.x, v.y);
+    s2[a ^ 1] = make_double2(v.z, v.w);
+}
+__device__ __forceinline__ double4 lds_rec_get(const double4* sm, int i)
+{
+    const double2* s2 = (const double2*)sm;
+    const int a = 2 * i + (i & 1);
+    const double2 lo = s2[a], hi = s2[a ^ 1];
+    return make_double4(lo.x, lo.y, hi.x, hi.y);
+}
+__device__ __forceinline__ void lds_rec_put(float4* sm, int k, float4 v) { sm[k] = v; }
+__device__ __forceinline__ float4 lds_rec_get(const float4* sm, int i) { return sm[i]; }
+
+// Galaxy rows of the sliced layout: row k holds slots 64 k .. 64 k + 63, one per
+// pixel (lane) of its slice, so slice s covers rows soff[s]/64 .. soff[s+1]/64 - 1.
+// UNIT (no weights given, w = 1): no weight loads; padding slots are masked by the
+// lane's galaxy count instead of carrying w = 0 (acc + 0 leaves acc unchanged).
+template <bool UNIT, typename R> struct RowLoad {
+    int i[4];
+    R t[4], w[4];
+};
+
+template <bool UNIT, typename R>
+__device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const int32_t* __restrict__ gi,
+                                          const R* __restrict__ gt, const R* __restrict__ sw, int64_t r,
+                                          int64_t rlast, int lane)
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t o = min(r + u, rlast) * 64 + lane;  // rows past the run re-read its last row
+        L.i[u] = gi[o];
+        L.t[u] = gt[o];
+        if (!UNIT) L.w[u] = sw[o];
+    }
+}
+
 // delta maps from the sliced layout and the group's (i, t).  Block list order is
 // XCD-aware: workgroup b runs on XCD b mod 8, and blk[x * per_xcd + k] is the k-th
 // block of XCD x; each XCD walks its own pixel chunks group by group, so the
 // group's (i, t), the weights of those chunks and the group's records stay in
 // that XCD's L2.  Two maps per block share every (i, t, w) load.
+// Each wave takes a contiguous run of the chunk's slices and streams their rows four
+// at a time, the next four rows in flight while these are summed; slice ends fall on
+// row boundaries common to the whole wave, where every lane writes its pixel's delta
+// and restarts its running sum.  Per pixel the sum runs over its galaxies in order.
 // R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
 // w and the records rounded to FP32, FP32 Horner and pixel sums, FP32 delta.
-template <typename R> struct Vec4T;
-template <> struct Vec4T<double> { typedef double4 T; };
-template <> struct Vec4T<float> { typedef float4 T; };
-
-template <bool DENSE, bool TWO, typename R>
+template <bool DENSE, bool TWO, bool UNIT, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
-                                                     int32_t tstride, const int32_t* __restrict__ pstart,
-                                                     const uint32_t* __restrict__ perm,
+                                                     int32_t tstride, const uint32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ pstart,
                                                      const int64_t* __restrict__ soff, const R* __restrict__ sw,
                                                      const int32_t* __restrict__ it_i, const R* __restrict__ it_t,
                                                      const MapInfo* __restrict__ maps,
@@ -412,9 +462,11 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     const bool two = TWO && B.qb >= 0;
     const MapInfo ma = maps[B.qa];
     const MapInfo mb = maps[two ? B.qb : B.qa];
-    for (int k = threadIdx.x; k <= ma.nk; k += blockDim.x) sm4[k] = rec4[ma.koff + B.qa + k];
+    for (int k = threadIdx.x; k <= ma.nk; k += blockDim.x) lds_rec_put(sm4, k, rec4[ma.koff + B.qa + k]);
     if (two)
-        for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) sm4[tstride + k] = rec4[mb.koff + B.qb + k];
+        for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) lds_rec_put(sm4 + tstride, k, rec4[mb.koff + B.qb + k]);
+    const R4* sma = sm4;
+    const R4* smb = sm4 + (two ? tstride : 0);
     const int32_t* gi = it_i + (int64_t)ma.pad * T;
     const R* gt = it_t + (int64_t)ma.pad * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -422,45 +474,80 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     const int64_t colb = mb.l + (lmax + 1) * (int64_t)(mb.n - 1);
     const double wcola = DENSE ? 0.0 : win_b[cola];
     const double wcolb = DENSE ? 0.0 : win_b[colb];
-    const int s0 = B.chunk * spc, s1 = min(nslice, s0 + spc);
-    __syncthreads();  // records staged
-    for (int s = s0 + wave; s < s1; s += nwave) {
-        const int64_t e = (int64_t)s * 64 + lane;
-        const bool valid = e < npix;
-        const uint32_t p = valid ? perm[e] : 0u;
-        const int32_t cnt = valid ? pstart[p + 1] - pstart[p] : 0;
-        const int64_t base = soff[s] + lane;
-        R acca = 0, accb = 0;
-        for (int j0 = 0; j0 < cnt; j0 += 4) {
-            R tt[4], ww[4];
-            int ii[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int64_t o = base + 64 * (int64_t)min(j0 + u, cnt - 1);
-                ii[u] = gi[o];
-                tt[u] = gt[o];
-                ww[u] = sw[o];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const R4 c = sm4[ii[u]];  // (d, c, b, a)
-                const R v = fma(tt[u], fma(tt[u], fma(tt[u], c.w, c.z), c.y), c.x);
-                if (j0 + u < cnt) acca = acca + ww[u] * v;
-                if (two) {
-                    const R4 c2 = sm4[tstride + ii[u]];
-                    const R v2 = fma(tt[u], fma(tt[u], fma(tt[u], c2.w, c2.z), c2.y), c2.x);
-                    if (j0 + u < cnt) accb = accb + ww[u] * v2;
-                }
-            }
+    const int c0 = B.chunk * spc, c1 = min(nslice, c0 + spc);
+    const int per_w = (c1 - c0 + nwave - 1) / nwave;
+    const int s_end = min(c1, c0 + (wave + 1) * per_w);
+    int s = c0 + wave * per_w;
+    __syncthreads();  // records staged; no barrier below
+    if (s >= s_end) return;
+
+    // pixel (and, for UNIT, galaxy count) of this lane in slice sl
+    auto pixel_of = [&](int sl, uint32_t& px, int32_t& cn) {
+        const int64_t e = (int64_t)sl * 64 + lane;
+        px = e < npix ? perm[e] : 0u;
+        cn = 0;
+        if (UNIT && e < npix) cn = pstart[px + 1] - pstart[px];
+    };
+    auto write = [&](int sl, uint32_t px, R acca, R accb) {
+        if ((int64_t)sl * 64 + lane >= npix) return;
+        const double Wa = DENSE ? win_a[px + npix * cola] : __dmul_rn(win_a[px], wcola);
+        delta[(int64_t)B.qa * npix + px] = (R)__dsub_rn(__dmul_rn((double)acca, inv), Wa);
+        if (two) {
+            const double Wb = DENSE ? win_a[px + npix * colb] : __dmul_rn(win_a[px], wcolb);
+            delta[(int64_t)B.qb * npix + px] = (R)__dsub_rn(__dmul_rn((double)accb, inv), Wb);
         }
-        if (valid) {
-            const double Wa = DENSE ? win_a[p + npix * cola] : __dmul_rn(win_a[p], wcola);
-            delta[(int64_t)B.qa * npix + p] = (R)__dsub_rn(__dmul_rn((double)acca, inv), Wa);
-            if (two) {
-                const double Wb = DENSE ? win_a[p + npix * colb] : __dmul_rn(win_a[p], wcolb);
-                delta[(int64_t)B.qb * npix + p] = (R)__dsub_rn(__dmul_rn((double)accb, inv), Wb);
-            }
+    };
+
+    const int64_t rlast = (soff[s_end] >> 6) - 1;
+    int64_t r = soff[s] >> 6;            // next row to sum
+    int64_t first = r;                   // first row of slice s
+    int64_t bound = soff[s + 1] >> 6;    // first row after slice s
+    uint32_t p;
+    int32_t cnt;
+    pixel_of(s, p, cnt);
+    R acca = 0, accb = 0;
+    // slices that end at row rr (empty ones included) are written; s moves on
+    auto close_slices = [&](int64_t rr) {
+        while (bound == rr && s < s_end) {
+            write(s, p, acca, accb);
+            acca = 0;
+            accb = 0;
+            if (++s >= s_end) break;
+            pixel_of(s, p, cnt);
+            first = bound;
+            bound = soff[s + 1] >> 6;
         }
+    };
+    close_slices(r);
+    if (r > rlast) return;
+
+    RowLoad<UNIT, R> cur, nxt;
+    load_rows<UNIT, R>(cur, gi, gt, sw, r, rlast, lane);
+    for (; r <= rlast; r += 4) {
+        load_rows<UNIT, R>(nxt, gi, gt, sw, r + 4, rlast, lane);
+        R4 ca[4], cb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ca[u] = lds_rec_get(sma, cur.i[u]);  // (d, c, b, a)
+            if (TWO) cb[u] = lds_rec_get(smb, cur.i[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (r + u > rlast) break;
+            R va = fma(cur.t[u], fma(cur.t[u], fma(cur.t[u], ca[u].w, ca[u].z), ca[u].y), ca[u].x);
+            R vb = 0;
+            if (TWO) vb = fma(cur.t[u], fma(cur.t[u], fma(cur.t[u], cb[u].w, cb[u].z), cb[u].y), cb[u].x);
+            if (UNIT) {
+                const bool live = r + u - first < cnt;
+                acca = acca + (live ? va : (R)0);
+                if (TWO) accb = accb + (live ? vb : (R)0);
+            } else {
+                acca = acca + cur.w[u] * va;
+                if (TWO) accb = accb + cur.w[u] * vb;
+            }
+            close_slices(r + u + 1);
+        }
+        cur = nxt;
     }
 }
 
@@ -1602,6 +1689,7 @@ struct sfb_plan {
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf;
     int32_t sc_nchunk_it = 0, sc_per_xcd = 0;
     bool sc_two = false;
+    bool unit_w = false;  // the last catalogue came without weights (w = 1)
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[SFB_NPHASE];
@@ -2167,6 +2255,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
     P->ncat = N;
+    P->unit_w = d_w == nullptr;
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
     LAUNCH_CHECK();
@@ -2257,12 +2346,43 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         if ((rc = P->d_sr.ensure(8 * (size_t)std::max<int64_t>(total, 1))) ||
             (rc = P->d_sw.ensure(8 * (size_t)std::max<int64_t>(total, 1))))
             return rc;
+        // padding slots: r = -inf (the y_first pseudo-record with t = -0, a finite
+        // value) and w = 0, so they add an exact +-0 to every running pixel sum
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(total, 256)), dim3(256), 0, st, P->d_sr.as<double>(), total,
+                           -INFINITY);
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(total, 256)), dim3(256), 0, st, P->d_sw.as<double>(), total, 0.0);
+        LAUNCH_CHECK();
         hipLaunchKernelGGL(k_sell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_pixkey2.as<uint32_t>(),
                            P->d_pstart.as<int32_t>(), P->d_invperm.as<uint32_t>(), P->d_soff.as<int64_t>(),
                            P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_sr.as<double>(), P->d_sw.as<double>());
         LAUNCH_CHECK();
     }
     return SFB_OK;
+}
+
+// k_scatter_it for the window kind, one or two maps per block and unit or given weights
+template <typename R>
+static int launch_scatter_it(sfb_plan* P, bool dense, bool unit, dim3 grid, size_t shm, hipStream_t st, int32_t spc,
+                             int64_t T, int32_t tstride, const R* sw, const typename Vec4T<R>::T* rec, double inv,
+                             const double* d_wa, const double* d_wb)
+{
+    auto go = [&](auto kern) -> int {
+        HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,
+                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_perm.as<uint32_t>(),
+                           P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw, P->d_it_i.as<int32_t>(),
+                           P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa, d_wb, P->lmax,
+                           P->d_delta.as<R>());
+        LAUNCH_CHECK();
+        return SFB_OK;
+    };
+    const bool two = P->sc_two;
+    if (dense) {
+        if (two) return unit ? go(k_scatter_it<true, true, true, R>) : go(k_scatter_it<true, true, false, R>);
+        return unit ? go(k_scatter_it<true, false, true, R>) : go(k_scatter_it<true, false, false, R>);
+    }
+    if (two) return unit ? go(k_scatter_it<false, true, true, R>) : go(k_scatter_it<false, true, false, R>);
+    return unit ? go(k_scatter_it<false, false, true, R>) : go(k_scatter_it<false, false, false, R>);
 }
 
 static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
@@ -2279,10 +2399,12 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
             (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
             return rc;
         if (f32) {
-            if ((rc = P->d_swf.ensure(sizeof(float) * (size_t)T))) return rc;
-            hipLaunchKernelGGL(k_to_f32, dim3(nblk(T, 256)), dim3(256), 0, st, T, P->d_sw.as<double>(),
-                               P->d_swf.as<float>());
-            LAUNCH_CHECK();
+            if (!P->unit_w) {
+                if ((rc = P->d_swf.ensure(sizeof(float) * (size_t)T))) return rc;
+                hipLaunchKernelGGL(k_to_f32, dim3(nblk(T, 256)), dim3(256), 0, st, T, P->d_sw.as<double>(),
+                                   P->d_swf.as<float>());
+                LAUNCH_CHECK();
+            }
             hipLaunchKernelGGL(k_seg_t<float>, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T,
                                P->ngroup, P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
                                P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<float>());
@@ -2298,33 +2420,14 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         const size_t shm = (size_t)tstride * 32 * (P->sc_two ? 2 : 1);
         const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
         const dim3 g1((unsigned)(8 * P->sc_per_xcd));
-#define SFB_LAUNCH_IT(D, TW, R, R4, REC, SW)                                                                        \
-    do {                                                                                                           \
-        HIP_TRY(hipFuncSetAttribute((const void*)k_scatter_it<D, TW, R>,                                           \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                        \
-        hipLaunchKernelGGL((k_scatter_it<D, TW, R>), g1, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,      \
-                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_pstart.as<int32_t>(),        \
-                           P->d_perm.as<uint32_t>(), P->d_soff.as<int64_t>(), SW, P->d_it_i.as<int32_t>(),         \
-                           P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), REC, inv, d_wa, d_wb, P->lmax,              \
-                           P->d_delta.as<R>());                                                                    \
-    } while (0)
-        const double* sw64 = P->d_sw.as<double>();
-        const float* sw32 = P->d_swf.as<float>();
-        const double4* r64 = P->d_rec4.as<double4>();
-        const float4* r32 = P->d_rec4f.as<float4>();
         const bool dense = win_kind == SFB_WIN_DENSE;
-        if (f32) {
-            if (dense && P->sc_two) SFB_LAUNCH_IT(true, true, float, float4, r32, sw32);
-            else if (dense) SFB_LAUNCH_IT(true, false, float, float4, r32, sw32);
-            else if (P->sc_two) SFB_LAUNCH_IT(false, true, float, float4, r32, sw32);
-            else SFB_LAUNCH_IT(false, false, float, float4, r32, sw32);
-        } else {
-            if (dense && P->sc_two) SFB_LAUNCH_IT(true, true, double, double4, r64, sw64);
-            else if (dense) SFB_LAUNCH_IT(true, false, double, double4, r64, sw64);
-            else if (P->sc_two) SFB_LAUNCH_IT(false, true, double, double4, r64, sw64);
-            else SFB_LAUNCH_IT(false, false, double, double4, r64, sw64);
-        }
-#undef SFB_LAUNCH_IT
+        if (f32)
+            rc = launch_scatter_it<float>(P, dense, P->unit_w, g1, shm, st, spc, T, tstride, P->d_swf.as<float>(),
+                                          P->d_rec4f.as<float4>(), inv, d_wa, d_wb);
+        else
+            rc = launch_scatter_it<double>(P, dense, P->unit_w, g1, shm, st, spc, T, tstride, P->d_sw.as<double>(),
+                                           P->d_rec4.as<double4>(), inv, d_wa, d_wb);
+        if (rc) return rc;
         LAUNCH_CHECK();
         return SFB_OK;
     }

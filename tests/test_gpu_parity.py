@@ -87,10 +87,11 @@ def test_delta_map_crowded_pixels_bitexact():
     plan = S.get_plan(am)
     om = _oracle_modes_from(am)
     for q in (0, am.nmaps // 2, am.nmaps - 1):
-        say("crowded delta map", q)
-        g = plan.delta_map(cat, nbar, win, w, q)
-        c = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), w, q)
-        assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
+        for ww in (w, None):  # given weights, and none (w = 1: no weight loads, padding masked by counts)
+            say("crowded delta map", q, "weights" if ww is not None else "unit")
+            g = plan.delta_map(cat, nbar, win, ww, q)
+            c = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), ww, q)
+            assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
 
 
 def test_delta_map_weights_dense_bitexact(c1):
