@@ -332,47 +332,73 @@ struct GroupInfo {
 // Markstein's final correction step, which returns the correctly rounded quotient
 // (checked against IEEE division on every distinct segment of the tables).
 // TT = float rounds t once for the FP32 evaluation of the mixed path.
+// One block: group blockIdx.y on slots [blockIdx.x * per_blk, ...): the group's knots
+// and 1/h go to LDS once, then each thread searches four slots at a time.
+// i is 16-bit: the sliced layout is only used when a map's records fit in LDS
+// (nk + 1 <= 5120 FP64 records).
 template <typename TT>
-__global__ void k_seg_t(int64_t T, int32_t ngroup, const double* __restrict__ sr, const GroupInfo* __restrict__ groups,
-                        const double* __restrict__ gx, const double* __restrict__ grh, int32_t* __restrict__ it_i,
-                        TT* __restrict__ it_t)
+__global__ __launch_bounds__(256) void k_seg_t(int64_t T, int64_t per_blk, const double* __restrict__ sr,
+                                               const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
+                                               const double* __restrict__ grh, uint16_t* __restrict__ it_i,
+                                               TT* __restrict__ it_t)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ double sm_seg[];
     const int g = blockIdx.y;
-    if (e >= T) return;
     const GroupInfo G = groups[g];
-    const double r = sr[e];
-    int i;
-    double t;
-    if (r >= G.xfirst && r <= G.xlast) {
-        const double* x = gx + G.xoff;
-        i = min(max((int)((r - G.xfirst) * G.invdx), 0), G.nk - 2);
-        double x0 = x[i], x1 = x[i + 1];
-        while (i > 0 && r < x0) {
-            --i;
-            x1 = x0;
-            x0 = x[i];
-        }
-        while (i < G.nk - 2 && r >= x1) {
-            ++i;
-            x0 = x1;
-            x1 = x[i + 1];
-        }
-        const double h = x1 - x0, n = r - x0, y = grh[G.xoff + i];
-        const double q0 = n * y;
-        t = fma(fma(-h, q0, n), y, q0);
-    } else if (r < G.xfirst) {
-        i = G.nk - 1;
-        t = -0.0;
-    } else if (r > G.xlast) {
-        i = G.nk;
-        t = -0.0;
-    } else {  // NaN
-        i = 0;
-        t = r;
+    double* xs = sm_seg;
+    double* rh = sm_seg + G.nk;
+    for (int k = threadIdx.x; k < G.nk; k += blockDim.x) {
+        xs[k] = gx[G.xoff + k];
+        rh[k] = grh[G.xoff + k];
     }
-    it_i[(int64_t)g * T + e] = i;
-    it_t[(int64_t)g * T + e] = (TT)t;
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * per_blk, e1 = min(T, e0 + per_blk);
+    uint16_t* oi = it_i + (int64_t)g * T;
+    TT* ot = it_t + (int64_t)g * T;
+    for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += 4 * (int64_t)blockDim.x) {
+        double rr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t e = eb + u * (int64_t)blockDim.x;
+            rr[u] = e < e1 ? sr[e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t e = eb + u * (int64_t)blockDim.x;
+            if (e >= e1) break;
+            const double r = rr[u];
+            int i;
+            double t;
+            if (r >= G.xfirst && r <= G.xlast) {
+                i = min(max((int)((r - G.xfirst) * G.invdx), 0), G.nk - 2);
+                double x0 = xs[i], x1 = xs[i + 1];
+                while (i > 0 && r < x0) {
+                    --i;
+                    x1 = x0;
+                    x0 = xs[i];
+                }
+                while (i < G.nk - 2 && r >= x1) {
+                    ++i;
+                    x0 = x1;
+                    x1 = xs[i + 1];
+                }
+                const double h = x1 - x0, n = r - x0, y = rh[i];
+                const double q0 = n * y;
+                t = fma(fma(-h, q0, n), y, q0);
+            } else if (r < G.xfirst) {
+                i = G.nk - 1;
+                t = -0.0;
+            } else if (r > G.xlast) {
+                i = G.nk;
+                t = -0.0;
+            } else {  // NaN
+                i = 0;
+                t = r;
+            }
+            oi[e] = (uint16_t)i;
+            ot[e] = (TT)t;
+        }
+    }
 }
 
 // One block of the scatter: maps qa and qb (same knot group; qb < 0 if none) on
@@ -418,7 +444,7 @@ template <bool UNIT, typename R> struct RowLoad {
 };
 
 template <bool UNIT, typename R>
-__device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const int32_t* __restrict__ gi,
+__device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const uint16_t* __restrict__ gi,
                                           const R* __restrict__ gt, const R* __restrict__ sw, int64_t r,
                                           int64_t rlast, int lane)
 {
@@ -448,7 +474,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                                                      int32_t tstride, const uint32_t* __restrict__ perm,
                                                      const int32_t* __restrict__ pstart,
                                                      const int64_t* __restrict__ soff, const R* __restrict__ sw,
-                                                     const int32_t* __restrict__ it_i, const R* __restrict__ it_t,
+                                                     const uint16_t* __restrict__ it_i, const R* __restrict__ it_t,
                                                      const MapInfo* __restrict__ maps,
                                                      const typename Vec4T<R>::T* __restrict__ rec4, double inv,
                                                      const double* __restrict__ win_a,
@@ -467,7 +493,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) lds_rec_put(sm4 + tstride, k, rec4[mb.koff + B.qb + k]);
     const R4* sma = sm4;
     const R4* smb = sm4 + (two ? tstride : 0);
-    const int32_t* gi = it_i + (int64_t)ma.pad * T;
+    const uint16_t* gi = it_i + (int64_t)ma.pad * T;
     const R* gt = it_t + (int64_t)ma.pad * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int64_t cola = ma.l + (lmax + 1) * (int64_t)(ma.n - 1);
@@ -2370,7 +2396,7 @@ static int launch_scatter_it(sfb_plan* P, bool dense, bool unit, dim3 grid, size
         HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,
                            P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_perm.as<uint32_t>(),
-                           P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw, P->d_it_i.as<int32_t>(),
+                           P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw, P->d_it_i.as<uint16_t>(),
                            P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa, d_wb, P->lmax,
                            P->d_delta.as<R>());
         LAUNCH_CHECK();
@@ -2395,9 +2421,17 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         const int64_t T = std::max<int64_t>(P->sell_total, 1);
         int rc;
         PhaseTimer tseg(P, SFB_PHASE_SEGMENT, st);
-        if ((rc = P->d_it_i.ensure(sizeof(int32_t) * (size_t)(T * P->ngroup))) ||
+        if ((rc = P->d_it_i.ensure(sizeof(uint16_t) * (size_t)(T * P->ngroup))) ||
             (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
             return rc;
+        // 64 slot ranges per group; LDS holds the group's knots and 1/h
+        const int64_t per_blk = ((T + 63) / 64 + 1023) / 1024 * 1024;
+        const dim3 gseg((unsigned)((T + per_blk - 1) / per_blk), (unsigned)P->ngroup);
+        const size_t shm_seg = 2 * sizeof(double) * (size_t)P->tbl_cap;
+        HIP_TRY(hipFuncSetAttribute((const void*)k_seg_t<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shm_seg));
+        HIP_TRY(hipFuncSetAttribute((const void*)k_seg_t<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shm_seg));
         if (f32) {
             if (!P->unit_w) {
                 if ((rc = P->d_swf.ensure(sizeof(float) * (size_t)T))) return rc;
@@ -2405,13 +2439,13 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                    P->d_swf.as<float>());
                 LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_seg_t<float>, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T,
-                               P->ngroup, P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
-                               P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<float>());
+            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(256), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+                               P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
+                               P->d_it_i.as<uint16_t>(), P->d_it_t.as<float>());
         } else {
-            hipLaunchKernelGGL(k_seg_t<double>, dim3(nblk(T, 256), (unsigned)P->ngroup), dim3(256), 0, st, T,
-                               P->ngroup, P->d_sr.as<double>(), P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(),
-                               P->d_grh.as<double>(), P->d_it_i.as<int32_t>(), P->d_it_t.as<double>());
+            hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(256), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+                               P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
+                               P->d_it_i.as<uint16_t>(), P->d_it_t.as<double>());
         }
         LAUNCH_CHECK();
         tseg.end();
