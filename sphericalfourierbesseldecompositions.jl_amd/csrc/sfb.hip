@@ -401,10 +401,11 @@ __global__ __launch_bounds__(256) void k_seg_t(int64_t T, int64_t per_blk, const
     }
 }
 
-// One block of the scatter: maps qa and qb (same knot group; qb < 0 if none) on
-// the pixel slices of chunk c.
+// One block of the scatter: up to four maps q[0..] of one knot group (-1 = none)
+// on the pixel slices of chunk c.
 struct ScatterBlock {
-    int32_t qa, qb, chunk, pad;
+    int32_t q[4];
+    int32_t chunk, pad[3];
 };
 
 template <typename R> struct Vec4T;
@@ -461,14 +462,15 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const uint16_t* _
 // XCD-aware: workgroup b runs on XCD b mod 8, and blk[x * per_xcd + k] is the k-th
 // block of XCD x; each XCD walks its own pixel chunks group by group, so the
 // group's (i, t), the weights of those chunks and the group's records stay in
-// that XCD's L2.  Two maps per block share every (i, t, w) load.
+// that XCD's L2.  NM maps per block (as many record tables as LDS holds: two FP64,
+// four FP32) share every (i, t, w) load.
 // Each wave takes a contiguous run of the chunk's slices and streams their rows four
 // at a time, the next four rows in flight while these are summed; slice ends fall on
 // row boundaries common to the whole wave, where every lane writes its pixel's delta
-// and restarts its running sum.  Per pixel the sum runs over its galaxies in order.
+// and restarts its running sums.  Per pixel the sum runs over its galaxies in order.
 // R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
 // w and the records rounded to FP32, FP32 Horner and pixel sums, FP32 delta.
-template <bool DENSE, bool TWO, bool UNIT, typename R>
+template <bool DENSE, int NM, bool UNIT, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
                                                      int32_t tstride, const uint32_t* __restrict__ perm,
@@ -485,21 +487,27 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     extern __shared__ double4 sm_raw[];
     R4* sm4 = (R4*)sm_raw;
     const ScatterBlock B = blk[(int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3)];
-    const bool two = TWO && B.qb >= 0;
-    const MapInfo ma = maps[B.qa];
-    const MapInfo mb = maps[two ? B.qb : B.qa];
-    for (int k = threadIdx.x; k <= ma.nk; k += blockDim.x) lds_rec_put(sm4, k, rec4[ma.koff + B.qa + k]);
-    if (two)
-        for (int k = threadIdx.x; k <= mb.nk; k += blockDim.x) lds_rec_put(sm4 + tstride, k, rec4[mb.koff + B.qb + k]);
-    const R4* sma = sm4;
-    const R4* smb = sm4 + (two ? tstride : 0);
-    const uint16_t* gi = it_i + (int64_t)ma.pad * T;
-    const R* gt = it_t + (int64_t)ma.pad * T;
+    bool has[NM];
+    int64_t col[NM];
+    double wcol[NM];
+    const R4* tab[NM];
+    int pad = 0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        has[k] = B.q[k] >= 0;
+        // a missing map reads the first map's table and writes nothing
+        const int q = has[k] ? B.q[k] : B.q[0];
+        const MapInfo m = maps[q];
+        if (k == 0) pad = m.pad;
+        if (has[k])
+            for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) lds_rec_put(sm4 + k * tstride, j, rec4[m.koff + q + j]);
+        tab[k] = sm4 + (has[k] ? k : 0) * tstride;
+        col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
+        wcol[k] = DENSE ? 0.0 : win_b[col[k]];
+    }
+    const uint16_t* gi = it_i + (int64_t)pad * T;
+    const R* gt = it_t + (int64_t)pad * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int64_t cola = ma.l + (lmax + 1) * (int64_t)(ma.n - 1);
-    const int64_t colb = mb.l + (lmax + 1) * (int64_t)(mb.n - 1);
-    const double wcola = DENSE ? 0.0 : win_b[cola];
-    const double wcolb = DENSE ? 0.0 : win_b[colb];
     const int c0 = B.chunk * spc, c1 = min(nslice, c0 + spc);
     const int per_w = (c1 - c0 + nwave - 1) / nwave;
     const int s_end = min(c1, c0 + (wave + 1) * per_w);
@@ -514,13 +522,16 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         cn = 0;
         if (UNIT && e < npix) cn = pstart[px + 1] - pstart[px];
     };
-    auto write = [&](int sl, uint32_t px, R acca, R accb) {
+    R acc[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) acc[k] = 0;
+    auto write = [&](int sl, uint32_t px) {
         if ((int64_t)sl * 64 + lane >= npix) return;
-        const double Wa = DENSE ? win_a[px + npix * cola] : __dmul_rn(win_a[px], wcola);
-        delta[(int64_t)B.qa * npix + px] = (R)__dsub_rn(__dmul_rn((double)acca, inv), Wa);
-        if (two) {
-            const double Wb = DENSE ? win_a[px + npix * colb] : __dmul_rn(win_a[px], wcolb);
-            delta[(int64_t)B.qb * npix + px] = (R)__dsub_rn(__dmul_rn((double)accb, inv), Wb);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            if (!has[k]) continue;
+            const double W = DENSE ? win_a[px + npix * col[k]] : __dmul_rn(win_a[px], wcol[k]);
+            delta[(int64_t)B.q[k] * npix + px] = (R)__dsub_rn(__dmul_rn((double)acc[k], inv), W);
         }
     };
 
@@ -531,13 +542,12 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     uint32_t p;
     int32_t cnt;
     pixel_of(s, p, cnt);
-    R acca = 0, accb = 0;
     // slices that end at row rr (empty ones included) are written; s moves on
     auto close_slices = [&](int64_t rr) {
         while (bound == rr && s < s_end) {
-            write(s, p, acca, accb);
-            acca = 0;
-            accb = 0;
+            write(s, p);
+#pragma unroll
+            for (int k = 0; k < NM; ++k) acc[k] = 0;
             if (++s >= s_end) break;
             pixel_of(s, p, cnt);
             first = bound;
@@ -551,25 +561,21 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     load_rows<UNIT, R>(cur, gi, gt, sw, r, rlast, lane);
     for (; r <= rlast; r += 4) {
         load_rows<UNIT, R>(nxt, gi, gt, sw, r + 4, rlast, lane);
-        R4 ca[4], cb[4];
+        R4 c[4][NM];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            ca[u] = lds_rec_get(sma, cur.i[u]);  // (d, c, b, a)
-            if (TWO) cb[u] = lds_rec_get(smb, cur.i[u]);
-        }
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < NM; ++k) c[u][k] = lds_rec_get(tab[k], cur.i[u]);  // (d, c, b, a)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
-            R va = fma(cur.t[u], fma(cur.t[u], fma(cur.t[u], ca[u].w, ca[u].z), ca[u].y), ca[u].x);
-            R vb = 0;
-            if (TWO) vb = fma(cur.t[u], fma(cur.t[u], fma(cur.t[u], cb[u].w, cb[u].z), cb[u].y), cb[u].x);
-            if (UNIT) {
-                const bool live = r + u - first < cnt;
-                acca = acca + (live ? va : (R)0);
-                if (TWO) accb = accb + (live ? vb : (R)0);
-            } else {
-                acca = acca + cur.w[u] * va;
-                if (TWO) accb = accb + cur.w[u] * vb;
+            const R t = cur.t[u];
+            const bool live = !UNIT || r + u - first < cnt;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                const R v = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
+                if (UNIT) acc[k] = acc[k] + (live ? v : (R)0);
+                else acc[k] = acc[k] + cur.w[u] * v;
             }
             close_slices(r + u + 1);
         }
@@ -1712,9 +1718,9 @@ struct sfb_plan {
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
     // knot groups (maps with bit-identical knots) and the shared segment search
     int32_t ngroup = 0, nk_max_group = 0;
-    DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf;
-    int32_t sc_nchunk_it = 0, sc_per_xcd = 0;
-    bool sc_two = false;
+    DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
+    int32_t sc_nchunk_it = 0;
+    int32_t sc_nm[2] = {1, 1}, sc_per_xcd[2] = {0, 0};  // [0] FP64, [1] FP32 scatter
     bool unit_w = false;  // the last catalogue came without weights (w = 1)
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
@@ -1910,35 +1916,45 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         // scatter block list: per XCD, its pixel chunks, group by group, maps in
         // pairs of the same group when two record tables fit in LDS
         P->sell_ok = (int64_t)(P->tbl_cap + 1) * 32 <= P->lds_max;
-        P->sc_two = (int64_t)2 * (P->tbl_cap + 1) * 32 <= P->lds_max;
+        // record tables per block: as many as LDS holds, up to two FP64 (32 B records)
+        // or four FP32 (16 B); more FP64 tables would not fit the kernel's registers
+        for (int f = 0; f < 2; ++f) {
+            const int64_t rb = f ? 16 : 32;
+            int nm = 1;
+            while (nm < (f ? 4 : 2) && (int64_t)(2 * nm) * (P->tbl_cap + 1) * rb <= P->lds_max) nm *= 2;
+            P->sc_nm[f] = nm;
+        }
         const int64_t nslice = (P->npix + 63) / 64;
         int32_t nchunk = 8;
         while (nchunk < 128 && nslice / (nchunk * 2) >= 32) nchunk *= 2;
         P->sc_nchunk_it = nchunk;
         std::vector<std::vector<int32_t>> members(groups.size());
         for (int32_t qq : qorder) members[(size_t)gid[(size_t)qq]].push_back(qq);
-        std::vector<ScatterBlock> per[8];
-        for (int x = 0; x < 8; ++x)
-            for (size_t g = 0; g < members.size(); ++g)
-                for (int32_t c = x; c < nchunk; c += 8) {
-                    const auto& mm = members[g];
-                    const size_t step = P->sc_two ? 2 : 1;
-                    for (size_t k = 0; k < mm.size(); k += step) {
-                        ScatterBlock b{};
-                        b.qa = mm[k];
-                        b.qb = (P->sc_two && k + 1 < mm.size()) ? mm[k + 1] : -1;
-                        b.chunk = c;
-                        per[x].push_back(b);
+        for (int f = 0; f < 2; ++f) {
+            const size_t step = (size_t)P->sc_nm[f];
+            std::vector<ScatterBlock> per[8];
+            for (int x = 0; x < 8; ++x)
+                for (size_t g = 0; g < members.size(); ++g)
+                    for (int32_t c = x; c < nchunk; c += 8) {
+                        const auto& mm = members[g];
+                        for (size_t k = 0; k < mm.size(); k += step) {
+                            ScatterBlock b{};
+                            for (size_t j = 0; j < 4; ++j) b.q[j] = (j < step && k + j < mm.size()) ? mm[k + j] : -1;
+                            b.chunk = c;
+                            per[x].push_back(b);
+                        }
                     }
-                }
-        P->sc_per_xcd = (int32_t)per[0].size();
-        std::vector<ScatterBlock> all;
-        for (int x = 0; x < 8; ++x) {
-            if ((int32_t)per[x].size() != P->sc_per_xcd) return fail(SFB_ERR_ARG, "internal: uneven scatter blocks");
-            all.insert(all.end(), per[x].begin(), per[x].end());
+            P->sc_per_xcd[f] = (int32_t)per[0].size();
+            std::vector<ScatterBlock> all;
+            for (int x = 0; x < 8; ++x) {
+                if ((int32_t)per[x].size() != P->sc_per_xcd[f])
+                    return fail(SFB_ERR_ARG, "internal: uneven scatter blocks");
+                all.insert(all.end(), per[x].begin(), per[x].end());
+            }
+            DevBuf& d = f ? P->d_sblk32 : P->d_sblk;
+            if ((rc = d.ensure(sizeof(ScatterBlock) * all.size()))) return rc;
+            HIP_TRY(hipMemcpy(d.p, all.data(), sizeof(ScatterBlock) * all.size(), hipMemcpyHostToDevice));
         }
-        if ((rc = P->d_sblk.ensure(sizeof(ScatterBlock) * all.size()))) return rc;
-        HIP_TRY(hipMemcpy(P->d_sblk.p, all.data(), sizeof(ScatterBlock) * all.size(), hipMemcpyHostToDevice));
     }
     if ((rc = P->d_maps.ensure(sizeof(MapInfo) * P->maps.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_maps.p, P->maps.data(), sizeof(MapInfo) * P->maps.size(), hipMemcpyHostToDevice));
@@ -2247,7 +2263,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw,
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
-                      &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_rec4f, &P->d_swf,
+                      &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf,
                       &P->d_desc_ana32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
@@ -2386,30 +2402,54 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     return SFB_OK;
 }
 
-// k_scatter_it for the window kind, one or two maps per block and unit or given weights
-template <typename R>
-static int launch_scatter_it(sfb_plan* P, bool dense, bool unit, dim3 grid, size_t shm, hipStream_t st, int32_t spc,
-                             int64_t T, int32_t tstride, const R* sw, const typename Vec4T<R>::T* rec, double inv,
-                             const double* d_wa, const double* d_wb)
+// k_scatter_it for the window kind, the maps per block and unit or given weights
+template <typename R> struct ScatterNM;
+template <> struct ScatterNM<double> { static constexpr int MAX = 2; };  // two 32-byte tables fill LDS
+template <> struct ScatterNM<float> { static constexpr int MAX = 4; };
+
+template <typename R, int NM>
+static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size_t shm, hipStream_t st,
+                             int32_t spc, int64_t T, int32_t tstride, const ScatterBlock* blk, int32_t per_xcd,
+                             const R* sw, const typename Vec4T<R>::T* rec, double inv, const double* d_wa,
+                             const double* d_wb)
 {
     auto go = [&](auto kern) -> int {
         HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,
-                           P->d_sblk.as<ScatterBlock>(), P->sc_per_xcd, tstride, P->d_perm.as<uint32_t>(),
-                           P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw, P->d_it_i.as<uint16_t>(),
-                           P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa, d_wb, P->lmax,
-                           P->d_delta.as<R>());
+        hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T, blk, per_xcd, tstride,
+                           P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw,
+                           P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
+                           d_wb, P->lmax, P->d_delta.as<R>());
         LAUNCH_CHECK();
         return SFB_OK;
     };
-    const bool two = P->sc_two;
-    if (dense) {
-        if (two) return unit ? go(k_scatter_it<true, true, true, R>) : go(k_scatter_it<true, true, false, R>);
-        return unit ? go(k_scatter_it<true, false, true, R>) : go(k_scatter_it<true, false, false, R>);
-    }
-    if (two) return unit ? go(k_scatter_it<false, true, true, R>) : go(k_scatter_it<false, true, false, R>);
-    return unit ? go(k_scatter_it<false, false, true, R>) : go(k_scatter_it<false, false, false, R>);
+    if (dense) return unit ? go(k_scatter_it<true, NM, true, R>) : go(k_scatter_it<true, NM, false, R>);
+    return unit ? go(k_scatter_it<false, NM, true, R>) : go(k_scatter_it<false, NM, false, R>);
 }
+
+template <typename R>
+static int launch_scatter_it(sfb_plan* P, bool dense, hipStream_t st, int64_t T, const R* sw,
+                             const typename Vec4T<R>::T* rec, double inv, const double* d_wa, const double* d_wb)
+{
+    const int f = sizeof(R) == 4;
+    const int nm = std::min(P->sc_nm[f], ScatterNM<R>::MAX);
+    if (nm != P->sc_nm[f]) return fail(SFB_ERR_ARG, "internal: scatter block list and kernel disagree");
+    const int32_t tstride = P->tbl_cap + 1;
+    const size_t shm = (size_t)tstride * sizeof(typename Vec4T<R>::T) * (size_t)nm;
+    const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
+    const dim3 grid((unsigned)(8 * P->sc_per_xcd[f]));
+    const ScatterBlock* blk = (f ? P->d_sblk32 : P->d_sblk).template as<ScatterBlock>();
+    const int32_t per = P->sc_per_xcd[f];
+    const bool unit = P->unit_w;
+    if (nm == 4)
+        return launch_scatter_nm<R, ScatterNM<R>::MAX>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw,
+                                                        rec, inv, d_wa, d_wb);
+    if (nm == 2)
+        return launch_scatter_nm<R, 2>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv, d_wa,
+                                       d_wb);
+    return launch_scatter_nm<R, 1>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv, d_wa,
+                                   d_wb);
+}
+
 
 static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
                          hipStream_t st, bool f32 = false)
@@ -2450,17 +2490,13 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         LAUNCH_CHECK();
         tseg.end();
         PhaseTimer tsc(P, SFB_PHASE_SCATTER, st);
-        const int32_t tstride = P->tbl_cap + 1;
-        const size_t shm = (size_t)tstride * 32 * (P->sc_two ? 2 : 1);
-        const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
-        const dim3 g1((unsigned)(8 * P->sc_per_xcd));
         const bool dense = win_kind == SFB_WIN_DENSE;
         if (f32)
-            rc = launch_scatter_it<float>(P, dense, P->unit_w, g1, shm, st, spc, T, tstride, P->d_swf.as<float>(),
-                                          P->d_rec4f.as<float4>(), inv, d_wa, d_wb);
+            rc = launch_scatter_it<float>(P, dense, st, T, P->d_swf.as<float>(), P->d_rec4f.as<float4>(), inv, d_wa,
+                                          d_wb);
         else
-            rc = launch_scatter_it<double>(P, dense, P->unit_w, g1, shm, st, spc, T, tstride, P->d_sw.as<double>(),
-                                           P->d_rec4.as<double4>(), inv, d_wa, d_wb);
+            rc = launch_scatter_it<double>(P, dense, st, T, P->d_sw.as<double>(), P->d_rec4.as<double4>(), inv,
+                                           d_wa, d_wb);
         if (rc) return rc;
         LAUNCH_CHECK();
         return SFB_OK;
