@@ -1084,14 +1084,21 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
 // carried by the dense per-m matrix C0; this kernel writes minus the m' != m
 // terms in E/O form for the first K_al ring pairs (the rings with npr <= 2 lmax).
 // G holds the parity halves Ge, Go per ring pair: north = Ge + Go, south = Ge - Go.
+// Grid: one block per (ring pair k, tile of 256 maps, m), XCD-aware: workgroup b
+// runs on XCD b mod 8, and XCD x takes the (k, tile) units x, x + 8, ..., each with
+// its L + 1 values of m in a row, so the G of a unit (all m of one ring pair and
+// tile, which every m of the unit reads) is fetched from HBM once into that XCD's L2.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_alias(const double* __restrict__ G, double* __restrict__ R,
                                                const RingDev* __restrict__ rings, int32_t K, int32_t K_al, int32_t L,
-                                               int64_t M2, int32_t nq)
+                                               int64_t M2, int32_t nq, int32_t nqt)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    const int m = blockIdx.z;
+    const int b = blockIdx.x, xcd = b & 7, loc = b >> 3;
+    const int unit = (loc / (L + 1)) * 8 + xcd;
+    const int m = loc - (loc / (L + 1)) * (L + 1);
+    if (unit >= K_al * nqt) return;
+    const int k = unit / nqt;
+    const int q = (unit - k * nqt) * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr;
@@ -2598,7 +2605,8 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     // the Jacobi GEMMs stream FP64 a, G, R and are bandwidth-bound: FP32 operands gain
     // nothing here, so both paths run them in FP64
     const bool mixed = false;
-    dim3 gr(nblk(nq, 256), (unsigned)std::max(Kal, 1), (unsigned)(L + 1));
+    const int32_t nqt = (int32_t)nblk(nq, 256);
+    const dim3 gr((unsigned)(8 * ((std::max(Kal, 1) * nqt + 7) / 8) * (L + 1)));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
         const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>() + slot * nbat;
@@ -2613,7 +2621,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
             {
                 PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
                 hipLaunchKernelGGL(k_alias, gr, dim3(256), 0, st, P->d_G.as<double>(), P->d_R.as<double>(),
-                                   P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq);
+                                   P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
                 LAUNCH_CHECK();
             }
         }
