@@ -257,18 +257,33 @@ __global__ void k_sell_keys(int64_t npix, const int32_t* __restrict__ pstart, ui
 // pixel permutation; one thread per slice
 __global__ void k_sell_slices(int64_t npix, int32_t nslice, const uint32_t* __restrict__ perm,
                               const int32_t* __restrict__ pstart, int64_t* __restrict__ width,
-                              uint32_t* __restrict__ invperm)
+                              uint32_t* __restrict__ invperm, int2* __restrict__ pinfo)
 {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslice) return;
     const int64_t e0 = 64 * (int64_t)s, e1 = min(npix, e0 + 64);
     int32_t m = 0;
-    for (int64_t e = e0; e < e1; ++e) {
+    for (int64_t e = e0; e < e0 + 64; ++e) {
+        if (e >= e1) {  // lanes past the last pixel: no pixel, no galaxies
+            pinfo[e] = make_int2(0, 0);
+            continue;
+        }
         const uint32_t p = perm[e];
         invperm[p] = (uint32_t)e;
-        m = max(m, pstart[p + 1] - pstart[p]);
+        const int32_t c = pstart[p + 1] - pstart[p];
+        pinfo[e] = make_int2((int)p, c);
+        m = max(m, c);
     }
     width[s] = 64 * (int64_t)m;
+}
+
+// separable window: the pixel factor of every slice lane, win_a[pixel] (0 past npix)
+__global__ void k_slice_win(int64_t nlane, int64_t npix, const int2* __restrict__ pinfo,
+                            const double* __restrict__ win_a, double* __restrict__ pwin)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nlane) return;
+    pwin[e] = e < npix ? win_a[pinfo[e].x] : 0.0;
 }
 
 // exclusive prefix sum of n int64 values in one block of 1024 threads; out[n] = total
@@ -473,8 +488,8 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const uint16_t* _
 template <bool DENSE, int NM, bool UNIT, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
-                                                     int32_t tstride, const uint32_t* __restrict__ perm,
-                                                     const int32_t* __restrict__ pstart,
+                                                     int32_t tstride, const int2* __restrict__ pinfo,
+                                                     const double* __restrict__ pwin,
                                                      const int64_t* __restrict__ soff, const R* __restrict__ sw,
                                                      const uint16_t* __restrict__ it_i, const R* __restrict__ it_t,
                                                      const MapInfo* __restrict__ maps,
@@ -515,43 +530,56 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     __syncthreads();  // records staged; no barrier below
     if (s >= s_end) return;
 
-    // pixel (and, for UNIT, galaxy count) of this lane in slice sl
-    auto pixel_of = [&](int sl, uint32_t& px, int32_t& cn) {
-        const int64_t e = (int64_t)sl * 64 + lane;
-        px = e < npix ? perm[e] : 0u;
-        cn = 0;
-        if (UNIT && e < npix) cn = pstart[px + 1] - pstart[px];
+    // Per-lane state of a slice (pixel, galaxy count, window values), loaded one
+    // slice ahead so that closing a slice only stores.
+    struct SliceSt {
+        int2 pc;
+        double w[NM];
+    };
+    auto load_st = [&](int sl, SliceSt& st) {
+        const int64_t e = (int64_t)min(sl, nslice - 1) * 64 + lane;
+        st.pc = pinfo[e];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) st.w[k] = DENSE ? 0.0 : pwin[e];
+    };
+    auto load_dense_w = [&](SliceSt& st) {  // needs st.pc: issued when the slice starts
+        if (DENSE)
+#pragma unroll
+            for (int k = 0; k < NM; ++k) st.w[k] = win_a[st.pc.x + npix * col[k]];
     };
     R acc[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) acc[k] = 0;
-    auto write = [&](int sl, uint32_t px) {
-        if ((int64_t)sl * 64 + lane >= npix) return;
-#pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            if (!has[k]) continue;
-            const double W = DENSE ? win_a[px + npix * col[k]] : __dmul_rn(win_a[px], wcol[k]);
-            delta[(int64_t)B.q[k] * npix + px] = (R)__dsub_rn(__dmul_rn((double)acc[k], inv), W);
-        }
-    };
 
     const int64_t rlast = (soff[s_end] >> 6) - 1;
-    int64_t r = soff[s] >> 6;            // next row to sum
-    int64_t first = r;                   // first row of slice s
-    int64_t bound = soff[s + 1] >> 6;    // first row after slice s
-    uint32_t p;
-    int32_t cnt;
-    pixel_of(s, p, cnt);
+    int64_t r = soff[s] >> 6;             // next row to sum
+    int64_t first = r;                    // first row of slice s
+    int64_t bound = soff[s + 1] >> 6;     // first row after slice s
+    int64_t nbound = soff[min(s + 2, nslice)] >> 6;  // first row after slice s + 1
+    SliceSt cs, ns;
+    load_st(s, cs);
+    load_dense_w(cs);
+    load_st(s + 1, ns);
     // slices that end at row rr (empty ones included) are written; s moves on
     auto close_slices = [&](int64_t rr) {
         while (bound == rr && s < s_end) {
-            write(s, p);
+            if ((int64_t)s * 64 + lane < npix) {
+#pragma unroll
+                for (int k = 0; k < NM; ++k) {
+                    if (!has[k]) continue;
+                    const double W = DENSE ? cs.w[k] : __dmul_rn(cs.w[k], wcol[k]);
+                    delta[(int64_t)B.q[k] * npix + cs.pc.x] = (R)__dsub_rn(__dmul_rn((double)acc[k], inv), W);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < NM; ++k) acc[k] = 0;
             if (++s >= s_end) break;
-            pixel_of(s, p, cnt);
+            cs = ns;
+            load_dense_w(cs);
+            load_st(s + 1, ns);
             first = bound;
-            bound = soff[s + 1] >> 6;
+            bound = nbound;
+            nbound = soff[min(s + 2, nslice)] >> 6;
         }
     };
     close_slices(r);
@@ -570,7 +598,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
             const R t = cur.t[u];
-            const bool live = !UNIT || r + u - first < cnt;
+            const bool live = !UNIT || r + u - first < cs.pc.y;
 #pragma unroll
             for (int k = 0; k < NM; ++k) {
                 const R v = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
@@ -1723,6 +1751,7 @@ struct sfb_plan {
     int32_t sell_nslice = 0;
     int64_t sell_total = 0;
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
+    DevBuf d_pinfo, d_pwin;  // per slice lane: (pixel, count), separable window factor
     // knot groups (maps with bit-identical knots) and the shared segment search
     int32_t ngroup = 0, nk_max_group = 0;
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
@@ -2270,7 +2299,8 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw,
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
-                      &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf,
+                      &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
+                      &P->d_pwin,
                       &P->d_desc_ana32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
@@ -2345,6 +2375,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         if ((rc = P->d_skey.ensure(4 * (size_t)P->npix)) || (rc = P->d_skey2.ensure(4 * (size_t)P->npix)) ||
             (rc = P->d_sval.ensure(4 * (size_t)P->npix)) || (rc = P->d_perm.ensure(4 * (size_t)P->npix)) ||
             (rc = P->d_invperm.ensure(4 * (size_t)P->npix)) ||
+            (rc = P->d_pinfo.ensure(sizeof(int2) * 64 * (size_t)P->sell_nslice)) ||
             (rc = P->d_swidth.ensure(8 * (size_t)P->sell_nslice)) ||
             (rc = P->d_soff.ensure(8 * (size_t)(P->sell_nslice + 1))))
             return rc;
@@ -2382,7 +2413,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
                                           16 + wbits, st));
         hipLaunchKernelGGL(k_sell_slices, dim3(nblk(P->sell_nslice, 256)), dim3(256), 0, st, P->npix, P->sell_nslice,
                            P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_swidth.as<int64_t>(),
-                           P->d_invperm.as<uint32_t>());
+                           P->d_invperm.as<uint32_t>(), P->d_pinfo.as<int2>());
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, P->sell_nslice, P->d_swidth.as<int64_t>(),
                            P->d_soff.as<int64_t>());
@@ -2423,7 +2454,7 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
     auto go = [&](auto kern) -> int {
         HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T, blk, per_xcd, tstride,
-                           P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_soff.as<int64_t>(), sw,
+                           P->d_pinfo.as<int2>(), P->d_pwin.as<double>(), P->d_soff.as<int64_t>(), sw,
                            P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
                            d_wb, P->lmax, P->d_delta.as<R>());
         LAUNCH_CHECK();
@@ -2498,6 +2529,13 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         tseg.end();
         PhaseTimer tsc(P, SFB_PHASE_SCATTER, st);
         const bool dense = win_kind == SFB_WIN_DENSE;
+        if (!dense) {
+            const int64_t nlane = 64 * (int64_t)P->sell_nslice;
+            if ((rc = P->d_pwin.ensure(sizeof(double) * (size_t)nlane))) return rc;
+            hipLaunchKernelGGL(k_slice_win, dim3(nblk(nlane, 256)), dim3(256), 0, st, nlane, P->npix,
+                               P->d_pinfo.as<int2>(), d_wa, P->d_pwin.as<double>());
+            LAUNCH_CHECK();
+        }
         if (f32)
             rc = launch_scatter_it<float>(P, dense, st, T, P->d_swf.as<float>(), P->d_rec4f.as<float4>(), inv, d_wa,
                                           d_wb);
