@@ -79,6 +79,12 @@ struct GemmDesc {
     int64_t lda, ldb, ldc;
 };
 
+// Descriptor pointers read from memory are generic (flat) to the compiler; they point
+// to global memory, and global loads (unlike flat ones) do not hold the LDS counter.
+typedef const __attribute__((address_space(1))) double* gcdptr;
+typedef __attribute__((address_space(1))) double* gdptr;
+typedef const __attribute__((address_space(1))) float* gcfptr;
+
 // ---------------------------------------------------------------------------
 // ang2pix, ring scheme (Gorski et al. 2005), same operation order as the
 // host restatement so the indices are bit-identical for identical cos(theta).
@@ -650,48 +656,15 @@ template <typename R> struct Vec2T;
 template <> struct Vec2T<double> { typedef double2 T; };
 template <> struct Vec2T<float> { typedef float2 T; };
 
-// Equatorial ring pairs (npr = 4 nside, a power of two): real FFT of length npr
-// through a complex FFT of length M = npr/2 in LDS, 2*TQ sequences per block:
-// bit-reversed load, then the radix-2 stages fused in pairs (one radix-4 pass of
-// 4 loads / 3 twiddles / 4 butterflies per thread item between barriers).  R is
-// the value type (double: plain path; float: mixed path).
+// In-place forward DFT (e^{-i}) of nf sequences of length M = N/2 held bit-reversed
+// in LDS: the radix-2 stages fused in pairs (one radix-4 pass of 4 loads / 3
+// twiddles / 4 butterflies per thread item between barriers), then a last radix-2
+// stage when log2 M is odd.  twr[kk] = (cos, sin)(pi kk / N), kk <= N.
 template <typename R>
-__global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
-                                                 int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                 const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
-                                                 int32_t k0, int32_t logM)
+__device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int M, int N, int logM,
+                                           const typename Vec2T<R>::T* twr)
 {
     typedef typename Vec2T<R>::T R2;
-    extern __shared__ double2 zs_raw[];
-    R2* zs = (R2*)zs_raw;
-    const int k = k0 + blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
-    const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1;
-    const bool eq = (k == K - 1);
-    const int nf = 2 * TQ;
-    const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
-    R2* twr = zs + nf * M;                // kk <= N staged in LDS
-    for (int e = threadIdx.x; e <= N; e += blockDim.x) {
-        const double2 w = twg[e];
-        twr[e].x = (R)w.x;
-        twr[e].y = (R)w.y;
-    }
-    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e / M, jj = e - f * M;
-        const int ql = f >> 1, side = f & 1;
-        const int q = q0 + ql;
-        R re = 0, im = 0;
-        if (q < nq && !(side && eq)) {
-            const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
-            re = delta[base];
-            im = delta[base + 1];
-        }
-        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-        zs[f * M + rj].x = re;
-        zs[f * M + rj].y = im;
-    }
-    __syncthreads();
     int st = 1;
     for (; st + 1 <= logM; st += 2) {
         const int half = 1 << (st - 1);
@@ -737,6 +710,51 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
         }
         __syncthreads();
     }
+}
+
+// Equatorial ring pairs (npr = 4 nside, a power of two): real FFT of length npr
+// through a complex FFT of length M = npr/2 in LDS, 2*TQ sequences per block:
+// bit-reversed load, then the radix-2 stages fused in pairs (one radix-4 pass of
+// 4 loads / 3 twiddles / 4 butterflies per thread item between barriers).  R is
+// the value type (double: plain path; float: mixed path).
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
+                                                 int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                 const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
+                                                 int32_t k0, int32_t logM)
+{
+    typedef typename Vec2T<R>::T R2;
+    extern __shared__ double2 zs_raw[];
+    R2* zs = (R2*)zs_raw;
+    const int k = k0 + blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev Rg = rings[k];
+    const int N = Rg.npr, M = N >> 1;
+    const bool eq = (k == K - 1);
+    const int nf = 2 * TQ;
+    const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
+    R2* twr = zs + nf * M;                // kk <= N staged in LDS
+    for (int e = threadIdx.x; e <= N; e += blockDim.x) {
+        const double2 w = twg[e];
+        twr[e].x = (R)w.x;
+        twr[e].y = (R)w.y;
+    }
+    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+        const int f = e / M, jj = e - f * M;
+        const int ql = f >> 1, side = f & 1;
+        const int q = q0 + ql;
+        R re = 0, im = 0;
+        if (q < nq && !(side && eq)) {
+            const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
+            re = delta[base];
+            im = delta[base + 1];
+        }
+        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+        zs[f * M + rj].x = re;
+        zs[f * M + rj].y = im;
+    }
+    __syncthreads();
+    fft_passes<R>(zs, nf, M, N, logM, twr);
     // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M
     const int nm = min(L, M);
     for (int e = threadIdx.x; e < TQ * (nm + 1); e += blockDim.x) {
@@ -755,6 +773,89 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
         store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twg, L, m, q, xr[0], xi[0], xr[1], xi[1]);
+    }
+}
+
+// Equatorial ring pairs, the synthesis direction (inverse of k_phi_fft):
+//   f_j = Re sum_{m<=L} H_m e^{2 pi i m j/N},  H_m = c_m (E_m +- O_m) e^{i m phi_0},
+// c_0 = 1, c_m = 2, north +, south -, phi_0 = shift pi/N, out = scale f.  With the
+// Hermitian spectrum X_k = H_k [k <= L] + conj(H_{N-k}) [N-k <= L] (X_0 = 2 Re H_0),
+// x = IDFT_N(X)/2 splits into the length-M = N/2 inverse DFT of
+//   Z_k = E_k + i O_k,  E_k = (X_k + conj(X_{M-k}))/2,  O_k = (X_k - conj(X_{M-k})) e^{2 pi i k/N}/2,
+// whose outputs are z_j = f_{2j} + i f_{2j+1}.  IDFT(Z) = conj(DFT(conj Z)): the block
+// loads conj(Z) bit-reversed and runs the forward passes of k_phi_fft.  FP64, 2 TQ
+// sequences (map, ring) per block.
+__global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
+                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
+                                                  const double2* __restrict__ tw, double scale,
+                                                  double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM)
+{
+    extern __shared__ double2 zi_raw[];
+    const int k = k0 + blockIdx.x;
+    const int q0 = blockIdx.y * TQ;
+    const RingDev Rg = rings[k];
+    const int N = Rg.npr, M = N >> 1;
+    const bool eq = (k == K - 1);
+    const int nf = 2 * TQ;
+    double2* zs = zi_raw;                 // nf x M
+    double2* twr = zs + nf * M;           // N + 1 twiddles
+    double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
+    const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
+    for (int e = threadIdx.x; e <= N; e += blockDim.x) twr[e] = twg[e];
+    for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
+        const int m = e / TQ, ql = e - m * TQ;  // maps fastest: contiguous X loads
+        const int q = q0 + ql;
+        double er = 0, ei = 0, orr = 0, oi = 0;
+        if (q < nq) {
+            const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+            er = X[be];
+            ei = X[be + 1];
+            if (m + 1 <= L) {
+                const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+                orr = X[bo];
+                oi = X[bo + 1];
+            }
+        }
+        const double c = m ? 2.0 : 1.0;
+        const double2 ph = twg[(int)(((int64_t)m * Rg.shift) % (2 * N))];  // e^{i m phi_0}
+        const double nr = c * (er + orr), ni = c * (ei + oi), sr = c * (er - orr), si = c * (ei - oi);
+        hs[(2 * ql + 0) * (L + 1) + m] = make_double2(nr * ph.x - ni * ph.y, nr * ph.y + ni * ph.x);
+        hs[(2 * ql + 1) * (L + 1) + m] = make_double2(sr * ph.x - si * ph.y, sr * ph.y + si * ph.x);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+        const int f = e / M, kk = e - f * M;
+        const double2* h = hs + f * (L + 1);
+        auto spec = [&](int b) -> double2 {  // X_b, 0 <= b <= M
+            if (b == 0) return make_double2(2.0 * h[0].x, 0.0);
+            double2 v = make_double2(0.0, 0.0);
+            if (b <= L) v = h[b];
+            if (N - b <= L) {
+                v.x += h[N - b].x;
+                v.y -= h[N - b].y;
+            }
+            return v;
+        };
+        const double2 xa = spec(kk), xb = spec(M - kk);  // X_k, X_{M-k}
+        const double er = 0.5 * (xa.x + xb.x), ei = 0.5 * (xa.y - xb.y);
+        const double dr = 0.5 * (xa.x - xb.x), di = 0.5 * (xa.y + xb.y);
+        const double2 w = twr[2 * kk];  // e^{2 pi i k/N}
+        const double orr = dr * w.x - di * w.y, oi = dr * w.y + di * w.x;
+        // Z = E + i O; conj(Z) goes in bit-reversed
+        const int rk = (int)(__brev((unsigned)kk) >> (32 - logM));
+        zs[f * M + rk] = make_double2(er - oi, -(ei + orr));
+    }
+    __syncthreads();
+    fft_passes<double>(zs, nf, M, N, logM, twr);
+    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+        const int f = e / M, j = e - f * M;
+        const int ql = f >> 1, side = f & 1;
+        const int q = q0 + ql;
+        if (q >= nq || (side && eq)) continue;
+        const double2 z = zs[e];  // conj(z_j)
+        const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * j;
+        out[base] = scale * z.x;
+        out[base + 1] = -scale * z.y;
     }
 }
 
@@ -989,16 +1090,16 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
             if (AF)
-                ra[u] = (gk < d.K && gi < rows) ? (T)((const float*)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
+                ra[u] = (gk < d.K && gi < rows) ? (T)((gcfptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
             else
-                ra[u] = (gk < d.K && gi < rows) ? (T)d.A[(int64_t)gk * d.lda + gi] : (T)0;
+                ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
             const int e = tid + 256 * u;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
-            rb[u] = (gk < d.K && gn < d.N) ? (T)d.B[(int64_t)gk * d.ldb + gn] : (T)0;
+            rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B)[(int64_t)gk * d.ldb + gn] : (T)0;
         }
     };
     fetch(0);
@@ -1033,8 +1134,8 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             if (n < d.N && i < rows) {
                 const int64_t o = (int64_t)n * d.ldc + i;
                 double val = alpha * (double)acc[t][v];
-                if (beta != 0.0) val = val + beta * (d.Cin ? d.Cin[o] : d.C[o]);
-                d.C[o] = val;
+                if (beta != 0.0) val = val + beta * ((gcdptr)(d.Cin ? d.Cin : d.C))[o];
+                ((gdptr)d.C)[o] = val;
             }
         }
 }
@@ -2697,6 +2798,37 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
     return jacobi_stage(P, nq, st);
 }
 
+// scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the direct sum on
+// the polar ring pairs, the inverse FFT on the equatorial ones (direct everywhere when
+// the FFT's sequences do not fit in LDS)
+static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
+{
+    const int32_t K = P->K, L = (int32_t)P->lmax;
+    const int64_t N = 4 * P->nside, M = N / 2;
+    int32_t TQI = 2;
+    auto ifft_lds = [&](int32_t tq) { return sizeof(double2) * (size_t)(2 * tq * M + N + 1 + 2 * tq * (L + 1)); };
+    while (TQI > 0 && (int64_t)ifft_lds(TQI) > P->lds_max) --TQI;
+    const int32_t k_eq0 = TQI > 0 ? (int32_t)(P->nside - 1) : K;  // first ring pair on the FFT
+    if (k_eq0 > 0) {
+        const int32_t TQS = 8;
+        const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)k_eq0, nblk(nq, TQS)), dim3(256), shs, st,
+                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), scale,
+                           P->d_delta.as<double>(), TQS);
+        LAUNCH_CHECK();
+    }
+    if (k_eq0 < K) {
+        const size_t shi = ifft_lds(TQI);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shi));
+        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(K - k_eq0), nblk(nq, TQI)), dim3(256), shi, st,
+                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM);
+        LAUNCH_CHECK();
+    }
+    return SFB_OK;
+}
+
 static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, const double* wb)
 {
     if (win_kind != SFB_WIN_DENSE && win_kind != SFB_WIN_SEPARABLE) return fail(SFB_ERR_ARG, "unknown window kind %d", win_kind);
@@ -2821,12 +2953,7 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     launch_gemm(P->d_desc_synf.as<GemmDesc>() + slot * nbat, nbat, 2 * nq, P->K, 1.0, 0.0, st);
     LAUNCH_CHECK();
     const double domega = 4.0 * kPi / (double)P->npix;
-    const int32_t TQS = 8;
-    const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
-    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
-    hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)P->K, nblk(nq, TQS)), dim3(256), shs, st, P->d_X0.as<double>(),
-                       P->npix, nq, P->M2, P->K, L, P->d_rings.as<RingDev>(), domega, P->d_delta.as<double>(), TQS);
-    LAUNCH_CHECK();
+    if ((rc = phi_synthesis(P, nq, domega, st))) return rc;
     // per-galaxy sums over the maps, in 32 map groups, then the ordered reduction
     const int32_t MG = (nq + 31) / 32;
     const int32_t ngroups = (nq + MG - 1) / MG;
@@ -3111,9 +3238,6 @@ extern "C" int sfb_anlm2field_device(sfb_plan* P, int64_t nr, const double* d_r,
     if ((rc = radial_table(P, nr, d_r, st))) return rc;
     const int32_t L = (int32_t)P->lmax;
     const unsigned nbat = (unsigned)(2 * (L + 1));
-    const int32_t TQS = 8;
-    const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
-    HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
     for (int64_t s0 = 0; s0 < nr; s0 += P->nmaps) {
         const int32_t Q = (int32_t)std::min<int64_t>(P->nmaps, nr - s0);
         // a^{(s)}_{lm} for every (l, m) of the chunk's shells, then alm2map per shell
@@ -3124,9 +3248,7 @@ extern "C" int sfb_anlm2field_device(sfb_plan* P, int64_t nr, const double* d_r,
         LAUNCH_CHECK();
         launch_gemm(P->d_desc_synf.as<GemmDesc>(), nbat, 2 * Q, P->K, 1.0, 0.0, st);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)P->K, nblk(Q, TQS)), dim3(256), shs, st, P->d_X0.as<double>(),
-                           P->npix, Q, P->M2, P->K, L, P->d_rings.as<RingDev>(), 1.0, P->d_delta.as<double>(), TQS);
-        LAUNCH_CHECK();
+        if ((rc = phi_synthesis(P, Q, 1.0, st))) return rc;
         hipLaunchKernelGGL(k_transpose_block, dim3(nblk(P->npix, 32), nblk(Q, 32)), dim3(256), 0, st, nr, P->npix,
                            s0, (int64_t)Q, P->d_delta.as<double>(), d_f_out, P->npix, 0);
         LAUNCH_CHECK();

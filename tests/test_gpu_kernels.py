@@ -11,7 +11,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (64, 16, 64), (70, 37, 23), (130, 256, 71)])
+# K not a multiple of 4 or of the 16-deep k-step, several row and column tiles
+@pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (64, 16, 64), (70, 37, 23), (130, 256, 71), (300, 80, 200),
+                                      (33, 81, 90), (65, 3, 17), (64, 1, 1)])
 def test_gemm_f64_matches_numpy(rows, K, N):
     rng = np.random.default_rng(rows + K + N)
     A = rng.normal(size=(K, rows))
