@@ -84,6 +84,7 @@ struct GemmDesc {
 typedef const __attribute__((address_space(1))) double* gcdptr;
 typedef __attribute__((address_space(1))) double* gdptr;
 typedef const __attribute__((address_space(1))) float* gcfptr;
+typedef __attribute__((address_space(1))) float* gfptr;
 
 // ---------------------------------------------------------------------------
 // ang2pix, ring scheme (Gorski et al. 2005), same operation order as the
@@ -1068,7 +1069,7 @@ template <> struct MfmaT<true> {
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
 constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
-template <bool F32, int NT, bool AF = false>
+template <bool F32, int NT, bool AF = false, bool CF = false>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
@@ -1134,28 +1135,34 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             if (n < d.N && i < rows) {
                 const int64_t o = (int64_t)n * d.ldc + i;
                 double val = alpha * (double)acc[t][v];
-                if (beta != 0.0) val = val + beta * ((gcdptr)(d.Cin ? d.Cin : d.C))[o];
-                ((gdptr)d.C)[o] = val;
+                if (CF) {  // C (and the beta term) stored as float
+                    if (beta != 0.0) val = val + beta * (double)((gcfptr)(d.Cin ? d.Cin : d.C))[o];
+                    ((gfptr)d.C)[o] = (float)val;
+                } else {
+                    if (beta != 0.0) val = val + beta * ((gcdptr)(d.Cin ? d.Cin : d.C))[o];
+                    ((gdptr)d.C)[o] = val;
+                }
             }
         }
 }
 
-template <bool F32, bool AF>
+template <bool F32, bool AF, bool CF = false>
 __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, GT_NTM, AF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
-template <bool F32, bool AF = false>
+// CF: C stored as float (the mixed path's aliasing-ring synthesis)
+template <bool F32, bool AF = false, bool CF = false>
 __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
@@ -1166,7 +1173,7 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ 
     const int i0 = blockIdx.x * GT_BM;
     const int n0 = blockIdx.y * GT_BNW;
     if (n0 >= d.N) return;
-    gemm_dispatch<F32, AF>(d, rows, alpha, beta, i0, n0, As, Bs);
+    gemm_dispatch<F32, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs);
 }
 
 // Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
@@ -1192,12 +1199,15 @@ __global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restric
 }
 
 
-// f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32)
+// f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32);
+// cf32: C stored as float (FP64 MFMA)
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
-                               double beta, hipStream_t st, bool f32 = false, bool af32 = false)
+                               double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (af32)
+    if (cf32)
+        hipLaunchKernelGGL((k_gemm_tall<false, false, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (af32)
         hipLaunchKernelGGL((k_gemm_tall<true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32)
         hipLaunchKernelGGL(k_gemm_tall<true>, g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -1218,7 +1228,8 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
 // its L + 1 values of m in a row, so the G of a unit (all m of one ring pair and
 // tile, which every m of the unit reads) is fetched from HBM once into that XCD's L2.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_alias(const double* __restrict__ G, double* __restrict__ R,
+template <typename T>
+__global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __restrict__ R,
                                                const RingDev* __restrict__ rings, int32_t K, int32_t K_al, int32_t L,
                                                int64_t M2, int32_t nq, int32_t nqt)
 {
@@ -1240,10 +1251,10 @@ __global__ __launch_bounds__(256) void k_alias(const double* __restrict__ G, dou
         const double sg = ((t * Rg.shift) & 1) ? -1.0 : 1.0;
         const int ma = mp < 0 ? -mp : mp;
         const double cj = mp < 0 ? -1.0 : 1.0;
-        const double* ge = G + ((int64_t)(ma * 2 + 0) * K_al + k) * M2 + 2 * q;
+        const T* ge = G + ((int64_t)(ma * 2 + 0) * K_al + k) * M2 + 2 * q;
         double er = ge[0], ei = ge[1], orr = 0.0, oi = 0.0;
         if (ma < L) {  // the odd half of m' = lmax is empty
-            const double* go = G + ((int64_t)(ma * 2 + 1) * K_al + k) * M2 + 2 * q;
+            const T* go = G + ((int64_t)(ma * 2 + 1) * K_al + k) * M2 + 2 * q;
             orr = go[0];
             oi = go[1];
         }
@@ -1259,15 +1270,15 @@ __global__ __launch_bounds__(256) void k_alias(const double* __restrict__ G, dou
     const int64_t be = ((int64_t)(m * 2 + 0) * K_al + k) * M2 + 2 * q;
     const int64_t bo = ((int64_t)(m * 2 + 1) * K_al + k) * M2 + 2 * q;
     if (eq) {
-        R[be] = -snr;
-        R[be + 1] = -sni;
-        R[bo] = -snr;
-        R[bo + 1] = -sni;
+        R[be] = (T)-snr;
+        R[be + 1] = (T)-sni;
+        R[bo] = (T)-snr;
+        R[bo + 1] = (T)-sni;
     } else {
-        R[be] = -(snr + ssr);
-        R[be + 1] = -(sni + ssi);
-        R[bo] = -(snr - ssr);
-        R[bo + 1] = -(sni - ssi);
+        R[be] = (T)-(snr + ssr);
+        R[be + 1] = (T)-(sni + ssi);
+        R[bo] = (T)-(snr - ssr);
+        R[bo + 1] = (T)-(sni - ssi);
     }
 }
 
@@ -1834,6 +1845,7 @@ struct sfb_plan {
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
+    DevBuf d_desc_syn32, d_desc_al32;  // aliasing-ring synthesis / analysis with G, R as floats (mixed)
     DevBuf d_qbase, d_lmaxn;       // per n: map index of (n, 0) and lmax_n (int32)
     std::vector<int32_t> lmaxn_h;  // lmax_n on the host
     DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
@@ -2226,7 +2238,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     //   syn: G_al = a_in LamT (first K_al ring pairs)       C0: a_out = a_in (I - C0) + b
     //   al : a_out += dOmega R_al Lam[first K_al ring pairs]
     const int64_t nbat = 2 * (L + 1);
-    std::vector<GemmDesc> ana, ana32, syn, c0, al;
+    std::vector<GemmDesc> ana, ana32, syn, c0, al, syn32, al32;
     double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
     double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
     for (int64_t b = 0; b < nbat; ++b) {
@@ -2260,6 +2272,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             sd.ldb = P->K;
             sd.ldc = P->M2;
             syn.push_back(sd);
+            // mixed path: G holds floats
+            sd.C = (double*)(P->d_G.as<float>() + b * (int64_t)P->K_al * P->M2);
+            syn32.push_back(sd);
             GemmDesc cd{};
             cd.A = a_in_slot[slot] + P->aoff[(size_t)b];
             cd.B = P->d_ic0.as<double>() + P->ic0_off[(size_t)b];
@@ -2282,6 +2297,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             ad.ldb = n;
             ad.ldc = P->M2;
             al.push_back(ad);
+            // mixed path: R holds floats
+            ad.A = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
+            al32.push_back(ad);
         }
     // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over all K ring pairs,
     // one set per a buffer (d_a, d_a1, d_a2)
@@ -2313,6 +2331,11 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     HIP_TRY(hipMemcpy(P->d_desc_syn.p, syn.data(), sizeof(GemmDesc) * syn.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->d_desc_c0.p, c0.data(), sizeof(GemmDesc) * c0.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->d_desc_al.p, al.data(), sizeof(GemmDesc) * al.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_desc_syn32.ensure(sizeof(GemmDesc) * syn32.size())) ||
+        (rc = P->d_desc_al32.ensure(sizeof(GemmDesc) * al32.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_syn32.p, syn32.data(), sizeof(GemmDesc) * syn32.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_desc_al32.p, al32.data(), sizeof(GemmDesc) * al32.size(), hipMemcpyHostToDevice));
     return SFB_OK;
 }
 
@@ -2402,7 +2425,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin,
-                      &P->d_desc_ana32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
+                      &P->d_desc_ana32, &P->d_desc_syn32, &P->d_desc_al32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -2732,7 +2755,13 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
 
 // niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a
 // (Healpix map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.
-static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
+// niter Jacobi refinements of a.  f32 (mixed path): the aliasing-ring values G and
+// their aliasing residual R are stored as floats (the synthesis accumulates in FP64
+// and rounds once; k_alias sums in FP64; the analysis of R runs on FP32 MFMA with an
+// FP64 epilogue); a, b and the dense C0 step stay FP64.  They only carry the aliasing
+// correction, so their FP32 rounding stays far below the mixed path's FP32 scatter
+// and phi-transform; the bytes they move halve.
+static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int32_t rows = 2 * nq;
@@ -2741,37 +2770,38 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st)
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
     P->a_final = P->d_a.as<double>();
-    // the Jacobi GEMMs stream FP64 a, G, R and are bandwidth-bound: FP32 operands gain
-    // nothing here, so both paths run them in FP64
-    const bool mixed = false;
     const int32_t nqt = (int32_t)nblk(nq, 256);
     const dim3 gr((unsigned)(8 * ((std::max(Kal, 1) * nqt + 7) / 8) * (L + 1)));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
-        const GemmDesc* syn = P->d_desc_syn.as<GemmDesc>() + slot * nbat;
+        const GemmDesc* syn = (f32 ? P->d_desc_syn32 : P->d_desc_syn).as<GemmDesc>() + slot * nbat;
         const GemmDesc* c0 = P->d_desc_c0.as<GemmDesc>() + slot * nbat;
-        const GemmDesc* al = P->d_desc_al.as<GemmDesc>() + slot * nbat;
+        const GemmDesc* al = (f32 ? P->d_desc_al32 : P->d_desc_al).as<GemmDesc>() + slot * nbat;
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, mixed);
+                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, false, false, f32);
                 LAUNCH_CHECK();
             }
             {
                 PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
-                hipLaunchKernelGGL(k_alias, gr, dim3(256), 0, st, P->d_G.as<double>(), P->d_R.as<double>(),
-                                   P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
+                if (f32)
+                    hipLaunchKernelGGL(k_alias<float>, gr, dim3(256), 0, st, P->d_G.as<float>(), P->d_R.as<float>(),
+                                       P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
+                else
+                    hipLaunchKernelGGL(k_alias<double>, gr, dim3(256), 0, st, P->d_G.as<double>(),
+                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
                 LAUNCH_CHECK();
             }
         }
         {
             PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st, mixed);
+            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st);
             LAUNCH_CHECK();
         }
         if (Kal > 0) {
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st, mixed);
+            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st, f32, f32);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -2795,7 +2825,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
                     P->nb[0], domega, 0.0, st, f32, f32);
         LAUNCH_CHECK();
     }
-    return jacobi_stage(P, nq, st);
+    return jacobi_stage(P, nq, st, f32);
 }
 
 // scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the direct sum on
