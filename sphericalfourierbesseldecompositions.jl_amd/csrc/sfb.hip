@@ -934,16 +934,21 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta, int64_t npix, int32_t nq,
                                                     int64_t M2, int32_t K, int32_t L,
                                                     const RingDev* __restrict__ rings, const double2* __restrict__ tw,
-                                                    R* __restrict__ X, int32_t TQ)
+                                                    R* __restrict__ X, int32_t TQMAX)
 {
     typedef typename Vec2T<R>::T R2;
     typedef typename Vec4T<R>::T R4;
     extern __shared__ double2 ps_raw[];
     R2* ps = (R2*)ps_raw;
     const int k = blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
+    // maps per block: as many as fill the work items (2 TQ ceil(nu / 2) <= blockDim.x),
+    // so small rings take many maps per block; blocks past the last map leave at once
+    const int nup = (nu + 1) / 2;
+    const int TQ = min(TQMAX, (int)blockDim.x / (2 * nup));
+    const int q0 = blockIdx.y * TQ;
+    if (q0 >= nq) return;
     const int ld = npr + 8;                       // doubles per sequence row
     const double2* twl = tw + Rg.tw_off;          // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
     R2* twi = ps;                                 // (cos, sin)(2 pi v/i), v < i
@@ -960,9 +965,7 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         D[f * ld + j] = v;
     }
     __syncthreads();
-    // Y_s[u] of sequence f and frequencies (2 up, 2 up + 1) for work item e = threadIdx.x;
-    // the host keeps 2 TQ * ceil(nu / 2) <= blockDim.x
-    const int nup = (nu + 1) / 2;
+    // Y_s[u] of sequence f and frequencies (2 up, 2 up + 1) for work item e = threadIdx.x
     const int e = threadIdx.x;
     const bool active = e < 2 * TQ * nup;
     const int up = active ? e % nup : 0, f = active ? e / nup : 0;
@@ -2691,13 +2694,26 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, direct DFT for the polar ones.
 // The polar kernel's work items (2 TQ x ceil((i/2 + 1)/2) <= 256) and LDS limit it
 // to nside <= 1024; beyond, the direct-DFT kernel (FP64 only) runs.
-static bool polar4_fits(const sfb_plan* P, size_t vbytes, int32_t* TQ4, size_t* shp)
+// k_phi_polar4 launch shape: maps per block up to TQMAX (each ring takes
+// min(TQMAX, 256 / (2 nup)) maps), the LDS of the largest block over the rings, and
+// the grid height for nq maps (set by the ring with the fewest maps per block)
+constexpr int32_t POLAR4_TQMAX = 64;
+static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t nq = 1, int32_t* gy = nullptr)
 {
     const int64_t npr_max = 4 * (P->nside - 1);
     const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
-    *TQ4 = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4, 256 / (2 * nup_max)));
-    *shp = 2 * vbytes * (size_t)(npr_max / 4 + 1) + vbytes * 2 * (size_t)*TQ4 * (size_t)(npr_max + 8);
-    return 2 * nup_max <= 256 && (int64_t)*shp <= P->lds_max;
+    if (2 * nup_max > 256) return false;
+    size_t sh = 0;
+    int64_t tqmin = POLAR4_TQMAX;
+    for (int64_t i = 1; i < P->nside; ++i) {
+        const int64_t nup = ((i / 2 + 1) + 1) / 2;
+        const int64_t tq = std::min<int64_t>(POLAR4_TQMAX, 256 / (2 * nup));
+        sh = std::max(sh, 2 * vbytes * (size_t)(i + 2) + vbytes * 2 * (size_t)tq * (size_t)(4 * i + 8));
+        tqmin = std::min(tqmin, tq);
+    }
+    *shp = sh;
+    if (gy) *gy = (int32_t)((nq + tqmin - 1) / tqmin);
+    return (int64_t)sh <= P->lds_max;
 }
 
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
@@ -2725,9 +2741,9 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
     }
     LAUNCH_CHECK();
     if (k_eq0 == 0) return SFB_OK;
-    int32_t TQ4;
+    int32_t gy4;
     size_t shp;
-    if (!polar4_fits(P, vb, &TQ4, &shp)) {
+    if (!polar4_fits(P, vb, &shp, nq, &gy4)) {
         if (f32) return fail(SFB_ERR_ARG, "internal: FP32 phi-transform beyond nside 1024");
         const int64_t npr_max = 4 * (nside - 1);
         size_t sho = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
@@ -2740,14 +2756,14 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
     }
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
+        hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), TQ4);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), POLAR4_TQMAX);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, nblk(nq, TQ4)), dim3(256), shp, st,
+        hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), TQ4);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), POLAR4_TQMAX);
     }
     LAUNCH_CHECK();
     return SFB_OK;
@@ -2882,9 +2898,8 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     // mixed path: FP32 spline values, delta maps and phi-transform when the FP32
     // phi kernels fit (nside <= 1024), FP32 MFMA for every Legendre pass
-    int32_t tq4;
     size_t shp4;
-    const bool f32 = P->precision == SFB_PREC_MIXED && polar4_fits(P, sizeof(float), &tq4, &shp4);
+    const bool f32 = P->precision == SFB_PREC_MIXED && polar4_fits(P, sizeof(float), &shp4);
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st, f32))) return rc;
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st, f32))) return rc;
     {
