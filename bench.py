@@ -63,11 +63,11 @@ def algorithmic_work(am, N, niter, f32=False, ngroup=None):
     if ngroup is None:
         ngroup = len({am.knot_x[am.knot_off[q]:am.knot_off[q + 1]].tobytes() for q in range(am.nmaps)})
     return {
-        # delta maps written once; per galaxy its weight, segment and t read once
-        "scatter": {"bytes": delta_bytes + N * (vb + 4 + vb) + (npix + 1) * 4 + npix * 8,
+        # delta maps written once; per galaxy its weight, 16-bit segment and t read once
+        "scatter": {"bytes": delta_bytes + N * (vb + 2 + vb) + (npix + 1) * 4 + npix * 8,
                     "flops": 10.0 * N * nmaps},
-        # per knot group and galaxy: r read, (segment, t) written
-        "segment": {"bytes": ngroup * N * (8 + 4 + vb), "flops": 0.0},
+        # per knot group and galaxy: r read, (16-bit segment, t) written
+        "segment": {"bytes": ngroup * N * (8 + 2 + vb), "flops": 0.0},
         "phi": {"bytes": delta_bytes + x_bytes, "flops": 0.0},
         # a = dOmega * X Lambda with N/S ring-pair symmetry: 2 flop x rows x K x sum n_b
         "analysis": {"bytes": x_bytes + a_bytes, "flops": 2.0 * M2 * K * nlm_pairs},
