@@ -785,15 +785,21 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
 //   Z_k = E_k + i O_k,  E_k = (X_k + conj(X_{M-k}))/2,  O_k = (X_k - conj(X_{M-k})) e^{2 pi i k/N}/2,
 // whose outputs are z_j = f_{2j} + i f_{2j+1}.  IDFT(Z) = conj(DFT(conj Z)): the block
 // loads conj(Z) bit-reversed and runs the forward passes of k_phi_fft.  FP64, 2 TQ
-// sequences (map, ring) per block.
+// sequences (map, ring) per block.  1-D grid over units (ring pair, tile of TQ maps),
+// tiles fastest; workgroup b runs on XCD b mod 8, and each XCD takes runs of four
+// consecutive units, which read the same 128-byte lines of X (TQ maps x 16 B each).
 __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                   int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                   const double2* __restrict__ tw, double scale,
                                                   double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM)
 {
     extern __shared__ double2 zi_raw[];
-    const int k = k0 + blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
+    const int nqt = (nq + TQ - 1) / TQ;
+    const int b = blockIdx.x, loc = b >> 3;
+    const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
+    if (unit >= (K - k0) * nqt) return;
+    const int k = k0 + unit / nqt;
+    const int q0 = (unit - (unit / nqt) * nqt) * TQ;
     const RingDev Rg = rings[k];
     const int N = Rg.npr, M = N >> 1;
     const bool eq = (k == K - 1);
@@ -858,6 +864,117 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
         out[base] = scale * z.x;
         out[base + 1] = -scale * z.y;
     }
+}
+
+// Polar ring pairs, the synthesis direction (inverse of k_phi_polar4), npr = 4i:
+//   f_n = Re sum_{m<=L} H_m e^{2 pi i m n/npr} = (1/2) sum_{b<npr} X_b e^{2 pi i b n/npr},
+//   X_b = sum_{m=b mod npr} H_m + sum_{m=-b mod npr} conj(H_m)  (H_m as in k_phi_ifft);
+// with n = 4j + s:  f_{4j+s} = (1/2) IDFT_i(V_s)_j,  V_s[u] = sum_{t<4} X_{u+ti} e^{2 pi i (u+ti) s/npr},
+// V_s Hermitian in u, so f_{4j+s} = sum_{u<=i/2} Re(V'_s[u] e^{2 pi i u j/i}) with
+// V' = V/2 at u = 0 and u = i/2 (i even), V elsewhere.  A work item is a sequence
+// (map, ring) and two outputs j; each ring takes as many maps per block as fill the
+// work items (2 TQ ceil(i/2) <= blockDim.x), up to TQMAX.  FP64.
+__global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ X, int64_t npix, int32_t nq,
+                                                     int64_t M2, int32_t K, int32_t L,
+                                                     const RingDev* __restrict__ rings,
+                                                     const double2* __restrict__ tw, double scale,
+                                                     double* __restrict__ out, int32_t TQMAX)
+{
+    extern __shared__ double2 ip_raw[];
+    const int k = blockIdx.x;
+    const RingDev Rg = rings[k];
+    const int npr = Rg.npr, ni = npr >> 2, n2 = 2 * npr, nu = ni / 2 + 1, njp = (ni + 1) / 2;
+    const int TQ = min(TQMAX, (int)blockDim.x / (2 * njp));
+    const int q0 = blockIdx.y * TQ;
+    if (q0 >= nq) return;
+    const int nf = 2 * TQ;
+    const double2* twl = tw + Rg.tw_off;  // (cos, sin)(pi kk/npr), kk < 2 npr
+    double2* twi = ip_raw;                // (cos, sin)(2 pi v/i), v < i
+    double2* Xb = twi + ni;               // [nf][npr] folded spectrum
+    double2* V = Xb + nf * npr;           // [nf][4][nu]
+    for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];
+    // fold: bin b of map ql sums its aliasing class, both signs of m, for both rings
+    // (maps fastest: a wave reads TQ consecutive maps of each X row)
+    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
+        const int ql = e % TQ, b = e / TQ;
+        const int q = q0 + ql;
+        double nr = 0.0, ni_ = 0.0, sr = 0.0, si = 0.0;
+        if (q < nq) {
+            for (int sg = 0; sg < 2; ++sg) {  // m = +b (mod npr), then m = -b (mod npr) as conj
+                const int m0 = sg ? (npr - b) % npr : b;
+                for (int m = m0; m <= L; m += npr) {
+                    const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
+                    const double2 ev = *(const double2*)(X + be);
+                    double2 od = make_double2(0.0, 0.0);
+                    if (m + 1 <= L) od = *(const double2*)(X + ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q);
+                    const double c = m ? 2.0 : 1.0;
+                    const double2 ph = twl[(int)(((int64_t)m * Rg.shift) % n2)];  // e^{i m phi_0}
+                    const double gnr = c * (ev.x + od.x), gni = c * (ev.y + od.y);
+                    const double gsr = c * (ev.x - od.x), gsi = c * (ev.y - od.y);
+                    const double hnr = gnr * ph.x - gni * ph.y, hni = gnr * ph.y + gni * ph.x;
+                    const double hsr = gsr * ph.x - gsi * ph.y, hsi = gsr * ph.y + gsi * ph.x;
+                    nr += hnr;
+                    ni_ += sg ? -hni : hni;
+                    sr += hsr;
+                    si += sg ? -hsi : hsi;
+                }
+            }
+        }
+        Xb[(2 * ql) * npr + b] = make_double2(nr, ni_);
+        Xb[(2 * ql + 1) * npr + b] = make_double2(sr, si);
+    }
+    __syncthreads();
+    // radix-4 combination, halved at u = 0 and u = i/2
+    for (int e = threadIdx.x; e < nf * 4 * nu; e += blockDim.x) {
+        const int f = e / (4 * nu), r = e - f * 4 * nu;
+        const int ss = r / nu, u = r - ss * nu;
+        const double2* xf = Xb + f * npr;
+        double vr = 0.0, vi = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int bb = u + t * ni;
+            const double2 x = xf[bb];
+            const double2 w = twl[(int)(((int64_t)2 * bb * ss) % n2)];  // e^{2 pi i bb s/npr}
+            vr += x.x * w.x - x.y * w.y;
+            vi += x.x * w.y + x.y * w.x;
+        }
+        const double hv = (u == 0 || 2 * u == ni) ? 0.5 : 1.0;
+        V[e] = make_double2(hv * vr, hv * vi);
+    }
+    __syncthreads();
+    // length-i real inverse DFTs, two outputs j per work item
+    const int e = threadIdx.x;
+    if (e >= nf * njp) return;
+    const int f = e / njp, jp = e - f * njp;
+    const int q = q0 + (f >> 1), side = f & 1;
+    if (q >= nq) return;
+    const int j0 = 2 * jp, j1 = j0 + 1;
+    const double2* vf = V + f * 4 * nu;
+    double acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int ss = 0; ss < 4; ++ss) acc[a][ss] = 0.0;
+    int i0 = 0, i1 = 0;  // u j0, u j1 (mod i)
+    for (int u = 0; u < nu; ++u) {
+        const double2 w0 = twi[i0], w1 = twi[i1];
+#pragma unroll
+        for (int ss = 0; ss < 4; ++ss) {
+            const double2 v = vf[ss * nu + u];
+            acc[0][ss] = fma(v.x, w0.x, fma(-v.y, w0.y, acc[0][ss]));
+            acc[1][ss] = fma(v.x, w1.x, fma(-v.y, w1.y, acc[1][ss]));
+        }
+        i0 += j0;
+        if (i0 >= ni) i0 -= ni;
+        i1 += j1;
+        if (i1 >= ni) i1 -= ni;
+    }
+    double* o = out + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n);
+#pragma unroll
+    for (int ss = 0; ss < 4; ++ss) o[4 * j0 + ss] = scale * acc[0][ss];
+    if (j1 < ni)
+#pragma unroll
+        for (int ss = 0; ss < 4; ++ss) o[4 * j1 + ss] = scale * acc[1][ss];
 }
 
 // Polar ring pairs (npr = 4i < 4 nside): direct DFT of the min(L, npr/2)+1
@@ -2844,9 +2961,9 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
     return jacobi_stage(P, nq, st, f32);
 }
 
-// scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the direct sum on
-// the polar ring pairs, the inverse FFT on the equatorial ones (direct everywhere when
-// the FFT's sequences do not fit in LDS)
+// scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the radix-4 inverse
+// on the polar ring pairs, the inverse FFT on the equatorial ones (the direct sum
+// k_phi_synth wherever their blocks do not fit in LDS)
 static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
@@ -2855,7 +2972,23 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
     auto ifft_lds = [&](int32_t tq) { return sizeof(double2) * (size_t)(2 * tq * M + N + 1 + 2 * tq * (L + 1)); };
     while (TQI > 0 && (int64_t)ifft_lds(TQI) > P->lds_max) --TQI;
     const int32_t k_eq0 = TQI > 0 ? (int32_t)(P->nside - 1) : K;  // first ring pair on the FFT
-    if (k_eq0 > 0) {
+    // polar ring pairs by the radix-4 inverse when its largest block fits in LDS
+    constexpr int32_t IP_TQMAX = 32;
+    size_t shq = 0;
+    int64_t tqmin = IP_TQMAX;
+    for (int64_t i = 1; i < P->nside; ++i) {
+        const int64_t njp = (i + 1) / 2, nu = i / 2 + 1;
+        const int64_t tq = std::min<int64_t>(IP_TQMAX, 256 / (2 * njp));
+        shq = std::max(shq, sizeof(double2) * (size_t)(i + 2 * tq * 4 * i + 2 * tq * 4 * nu));
+        tqmin = std::min(tqmin, tq);
+    }
+    if (TQI > 0 && k_eq0 > 0 && (int64_t)shq <= P->lds_max) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shq));
+        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)k_eq0, (unsigned)((nq + tqmin - 1) / tqmin)), dim3(256), shq,
+                           st, P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX);
+        LAUNCH_CHECK();
+    } else if (k_eq0 > 0) {
         const int32_t TQS = 8;
         const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
@@ -2867,7 +3000,8 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
     if (k_eq0 < K) {
         const size_t shi = ifft_lds(TQI);
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shi));
-        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(K - k_eq0), nblk(nq, TQI)), dim3(256), shi, st,
+        const int64_t units = (int64_t)(K - k_eq0) * nblk(nq, TQI);
+        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), shi, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM);
         LAUNCH_CHECK();
