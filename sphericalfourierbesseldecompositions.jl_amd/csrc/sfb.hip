@@ -1622,6 +1622,109 @@ __global__ __launch_bounds__(1024) void k_vjp_gather(int64_t N, int64_t npix, co
     }
 }
 
+// part[g][k] as k_vjp_gather, over the map range blockIdx.y (whole knot groups), from
+// the forward's machinery: per knot group the knots and 1/h go to LDS and every
+// galaxy's segment i and t = (r - x_i)/h are found once (as k_seg_t: the same
+// correctly rounded t); then per map of the group its records (d, c, b, a) go to LDS
+// and g = d + t(c + t(b + t a)) (the forward's value, bit for bit) and
+// g' = (c + t(2b + 3at)) / h, as the product with the tabulated 1/h (0 on the boundary
+// pseudo-records, the constant extrapolation).  Galaxies in pixel order.
+constexpr int VJ_GPT = 8;
+__global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_t cap, const double* __restrict__ r_s,
+                                                 const uint32_t* __restrict__ pix_s,
+                                                 const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
+                                                 const double* __restrict__ grh, const int32_t* __restrict__ vgs,
+                                                 const int32_t* __restrict__ vgm, const int2* __restrict__ vrange,
+                                                 const MapInfo* __restrict__ maps, const double4* __restrict__ rec4,
+                                                 const double* __restrict__ dbar, double2* __restrict__ part)
+{
+    extern __shared__ double4 vj_raw[];
+    double4* tb = vj_raw;                   // records, cap + 1
+    double* xs = (double*)(tb + cap + 1);   // knots, cap
+    double* rh = xs + cap;                  // 1/h, cap
+    const int64_t k0 = (int64_t)blockIdx.x * (1024 * VJ_GPT) + threadIdx.x;
+    double r[VJ_GPT], t[VJ_GPT], aw[VJ_GPT], ar[VJ_GPT];
+    int ii[VJ_GPT];
+    uint32_t pp[VJ_GPT];
+#pragma unroll
+    for (int u = 0; u < VJ_GPT; ++u) {
+        const int64_t k = k0 + 1024 * u;
+        r[u] = (k < N) ? r_s[k] : 0.0;
+        pp[u] = (k < N) ? pix_s[k] : 0u;
+        aw[u] = 0.0;
+        ar[u] = 0.0;
+    }
+    const int2 rg = vrange[blockIdx.y];
+    for (int g = rg.x; g < rg.y; ++g) {
+        const GroupInfo G = groups[g];
+        __syncthreads();  // the previous group's knots are no longer read
+        for (int e = threadIdx.x; e < G.nk; e += blockDim.x) {
+            xs[e] = gx[G.xoff + e];
+            rh[e] = grh[G.xoff + e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < VJ_GPT; ++u) {
+            const double rr = r[u];
+            int i;
+            double tt;
+            if (rr >= G.xfirst && rr <= G.xlast) {
+                i = min(max((int)((rr - G.xfirst) * G.invdx), 0), G.nk - 2);
+                double x0 = xs[i], x1 = xs[i + 1];
+                while (i > 0 && rr < x0) {
+                    --i;
+                    x1 = x0;
+                    x0 = xs[i];
+                }
+                while (i < G.nk - 2 && rr >= x1) {
+                    ++i;
+                    x0 = x1;
+                    x1 = xs[i + 1];
+                }
+                const double h = x1 - x0, n = rr - x0, y = rh[i];
+                const double q0 = n * y;
+                tt = fma(fma(-h, q0, n), y, q0);
+            } else if (rr < G.xfirst) {
+                i = G.nk - 1;
+                tt = -0.0;
+            } else if (rr > G.xlast) {
+                i = G.nk;
+                tt = -0.0;
+            } else {  // NaN
+                i = 0;
+                tt = rr;
+            }
+            ii[u] = i;
+            t[u] = tt;
+        }
+        for (int j = vgs[g]; j < vgs[g + 1]; ++j) {
+            const int q = vgm[j];
+            const MapInfo m = maps[q];
+            __syncthreads();  // the previous map's records are no longer read
+            for (int e = threadIdx.x; e <= m.nk; e += blockDim.x) tb[e] = rec4[m.koff + q + e];
+            __syncthreads();
+            const double* dq = dbar + (int64_t)q * npix;
+#pragma unroll
+            for (int u = 0; u < VJ_GPT; ++u) {
+                if (k0 + 1024 * u < N) {
+                    const double4 c = tb[ii[u]];  // (d, c, b, a)
+                    const double tt = t[u];
+                    const double v = fma(tt, fma(tt, fma(tt, c.w, c.z), c.y), c.x);
+                    const double dv = ii[u] < G.nk - 1 ? fma(tt, fma(tt, 3.0 * c.w, 2.0 * c.z), c.y) * rh[ii[u]] : 0.0;
+                    const double d = dq[pp[u]];
+                    aw[u] = fma(v, d, aw[u]);
+                    ar[u] = fma(dv, d, ar[u]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < VJ_GPT; ++u) {
+        const int64_t k = k0 + 1024 * u;
+        if (k < N) part[(int64_t)blockIdx.y * N + k] = make_double2(aw[u], ar[u]);
+    }
+}
+
 // dL/dw_i = inv sum_g part_w, dL/dr_i = inv w_i sum_g part_r, summed over the map
 // groups in order and written at the galaxy's input position.
 __global__ void k_vjp_reduce(int64_t N, int32_t ngroups, const double2* __restrict__ part,
@@ -1985,6 +2088,8 @@ struct sfb_plan {
     int64_t sell_total = 0;
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
     DevBuf d_pinfo, d_pwin;  // per slice lane: (pixel, count), separable window factor
+    DevBuf d_vgm, d_vgs, d_vrange;  // adjoint gather: group-ordered maps, group starts, map ranges
+    int32_t nvrange = 0;
     // knot groups (maps with bit-identical knots) and the shared segment search
     int32_t ngroup = 0, nk_max_group = 0;
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
@@ -2199,6 +2304,31 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         P->sc_nchunk_it = nchunk;
         std::vector<std::vector<int32_t>> members(groups.size());
         for (int32_t qq : qorder) members[(size_t)gid[(size_t)qq]].push_back(qq);
+        {
+            // adjoint gather: the maps group by group (vgm, group starts vgs), in ranges
+            // of whole groups of about nmaps / 32 maps (vrange: first group, end group)
+            std::vector<int32_t> vgm, vgs(1, 0);
+            for (const auto& mm : members) {
+                vgm.insert(vgm.end(), mm.begin(), mm.end());
+                vgs.push_back((int32_t)vgm.size());
+            }
+            std::vector<int2> vr;
+            const int32_t target = (int32_t)((P->nmaps + 31) / 32);
+            int32_t g0 = 0;
+            for (int32_t g = 0; g < (int32_t)members.size(); ++g)
+                if (vgs[(size_t)g + 1] - vgs[(size_t)g0] >= target || g + 1 == (int32_t)members.size()) {
+                    vr.push_back(make_int2(g0, g + 1));
+                    g0 = g + 1;
+                }
+            P->nvrange = (int32_t)vr.size();
+            if ((rc = P->d_vgm.ensure(sizeof(int32_t) * vgm.size())) ||
+                (rc = P->d_vgs.ensure(sizeof(int32_t) * vgs.size())) ||
+                (rc = P->d_vrange.ensure(sizeof(int2) * vr.size())))
+                return rc;
+            HIP_TRY(hipMemcpy(P->d_vgm.p, vgm.data(), sizeof(int32_t) * vgm.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(P->d_vgs.p, vgs.data(), sizeof(int32_t) * vgs.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(P->d_vrange.p, vr.data(), sizeof(int2) * vr.size(), hipMemcpyHostToDevice));
+        }
         for (int f = 0; f < 2; ++f) {
             const size_t step = (size_t)P->sc_nm[f];
             std::vector<ScatterBlock> per[8];
@@ -2544,7 +2674,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
-                      &P->d_pwin,
+                      &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_desc_al32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
@@ -3133,19 +3263,36 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     LAUNCH_CHECK();
     const double domega = 4.0 * kPi / (double)P->npix;
     if ((rc = phi_synthesis(P, nq, domega, st))) return rc;
-    // per-galaxy sums over the maps, in 32 map groups, then the ordered reduction
-    const int32_t MG = (nq + 31) / 32;
-    const int32_t ngroups = (nq + MG - 1) / MG;
-    if ((rc = P->d_part.ensure(sizeof(double2) * (size_t)ngroups * (size_t)N))) return rc;
-    const bool lds_tables = (int64_t)P->tbl_cap * 32 <= P->lds_max;
-    const size_t shg = lds_tables ? (size_t)P->tbl_cap * 32 : 0;
-    if (lds_tables)
-        HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shg));
-    hipLaunchKernelGGL(k_vjp_gather, dim3(nblk(N, 1024 * VG_GPT), (unsigned)ngroups), dim3(1024), shg, st, N,
-                       P->npix, P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(), P->d_maps.as<MapInfo>(),
-                       P->d_rec.as<double4>(), nq, MG, lds_tables, P->d_delta.as<double>(),
-                       P->d_part.as<double2>());
-    LAUNCH_CHECK();
+    // per-galaxy sums over the maps, in ranges of maps, then the ordered reduction
+    int32_t ngroups;
+    const size_t shv = 32 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
+    if (P->sell_ok && P->nvrange > 0 && (int64_t)shv <= P->lds_max) {
+        // knot groups: the segment search once per group and galaxy
+        ngroups = P->nvrange;
+        if ((rc = P->d_part.ensure(sizeof(double2) * (size_t)ngroups * (size_t)N))) return rc;
+        HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_it, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shv));
+        hipLaunchKernelGGL(k_vjp_it, dim3(nblk(N, 1024 * VJ_GPT), (unsigned)ngroups), dim3(1024), shv, st, N,
+                           P->npix, P->tbl_cap, P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(),
+                           P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
+                           P->d_vgs.as<int32_t>(), P->d_vgm.as<int32_t>(), P->d_vrange.as<int2>(),
+                           P->d_maps.as<MapInfo>(), P->d_rec4.as<double4>(), P->d_delta.as<double>(),
+                           P->d_part.as<double2>());
+        LAUNCH_CHECK();
+    } else {
+        const int32_t MG = (nq + 31) / 32;
+        ngroups = (nq + MG - 1) / MG;
+        if ((rc = P->d_part.ensure(sizeof(double2) * (size_t)ngroups * (size_t)N))) return rc;
+        const bool lds_tables = (int64_t)P->tbl_cap * 32 <= P->lds_max;
+        const size_t shg = lds_tables ? (size_t)P->tbl_cap * 32 : 0;
+        if (lds_tables)
+            HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)shg));
+        hipLaunchKernelGGL(k_vjp_gather, dim3(nblk(N, 1024 * VG_GPT), (unsigned)ngroups), dim3(1024), shg, st, N,
+                           P->npix, P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(), P->d_maps.as<MapInfo>(),
+                           P->d_rec.as<double4>(), nq, MG, lds_tables, P->d_delta.as<double>(),
+                           P->d_part.as<double2>());
+        LAUNCH_CHECK();
+    }
     const double inv = 1.0 / (nbar * domega);
     hipLaunchKernelGGL(k_vjp_reduce, dim3(nblk(N, 256)), dim3(256), 0, st, N, ngroups, P->d_part.as<double2>(),
                        P->d_idx.as<uint32_t>(), P->d_ws.as<double>(), inv, d_dr, d_dw);
