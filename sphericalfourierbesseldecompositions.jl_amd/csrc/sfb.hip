@@ -77,6 +77,12 @@ struct GemmDesc {
     const double* Cin;  // beta term source (same layout as C); NULL = C itself
     int32_t K, N;
     int64_t lda, ldb, ldc;
+    // split K (k_gemm_tall's SPL variants): rows k >= K1 of A and B come from
+    // A2[(k - K1) * lda2 + i] and B2[(k - K1) * ldb2 + n]
+    const double* A2;
+    const double* B2;
+    int32_t K1, pad;
+    int64_t lda2, ldb2;
 };
 
 // Descriptor pointers read from memory are generic (flat) to the compiler; they point
@@ -1189,7 +1195,7 @@ template <> struct MfmaT<true> {
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
 constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
-template <bool F32, int NT, bool AF = false, bool CF = false>
+template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
@@ -1204,13 +1210,20 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
     for (int t = 0; t < NT; ++t) acc[t] = M::zero();
     constexpr int BLD = BNW * GT_BK / 256;  // B elements per thread per k-step
     T ra[GT_ALD], rb[BLD];
+    float rf[GT_ALD];  // SPL 2: the second A's FP32 values, converted when stored to LDS
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
-            if (AF)
+            if (SPL && gk >= d.K1) {  // second A: FP64 (SPL 1) or FP32 (SPL 2)
+                const int64_t o = (int64_t)(gk - d.K1) * d.lda2 + gi;
+                if (SPL == 2)
+                    rf[u] = (gk < d.K && gi < rows) ? ((gcfptr)d.A2)[o] : 0.0f;
+                else
+                    ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A2)[o] : (T)0;
+            } else if (AF)
                 ra[u] = (gk < d.K && gi < rows) ? (T)((gcfptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
             else
                 ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
@@ -1220,7 +1233,10 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             const int e = tid + 256 * u;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
-            rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B)[(int64_t)gk * d.ldb + gn] : (T)0;
+            if (SPL && gk >= d.K1)
+                rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B2)[(int64_t)(gk - d.K1) * d.ldb2 + gn] : (T)0;
+            else
+                rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B)[(int64_t)gk * d.ldb + gn] : (T)0;
         }
     };
     fetch(0);
@@ -1228,7 +1244,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
 #pragma unroll
         for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
-            As[e >> 6][e & 63] = ra[u];
+            As[e >> 6][e & 63] = (SPL == 2 && k0 + (e >> 6) >= d.K1) ? (T)rf[u] : ra[u];
         }
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
@@ -1266,24 +1282,26 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32, bool AF, bool CF = false>
+template <bool F32, bool AF, bool CF = false, int SPL = 0>
 __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, GT_NTM, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
-// CF: C stored as float (the mixed path's aliasing-ring synthesis)
-template <bool F32, bool AF = false, bool CF = false>
-__global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+// CF: C stored as float (the mixed path's aliasing-ring synthesis); SPL: split K,
+// the second A FP64 (1) or FP32 (2).  At least 3 waves per SIMD (<= 168 VGPRs): the
+// Jacobi GEMMs are short-K and bound by the loads in flight per CU.
+template <bool F32, bool AF = false, bool CF = false, int SPL = 0>
+__global__ __launch_bounds__(256, 3) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
     typedef typename MfmaT<F32>::T T;
@@ -1293,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmDesc* __restrict__ 
     const int i0 = blockIdx.x * GT_BM;
     const int n0 = blockIdx.y * GT_BNW;
     if (n0 >= d.N) return;
-    gemm_dispatch<F32, AF, CF>(d, rows, alpha, beta, i0, n0, As, Bs);
+    gemm_dispatch<F32, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs);
 }
 
 // Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
@@ -1320,12 +1338,18 @@ __global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restric
 
 
 // f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32);
-// cf32: C stored as float (FP64 MFMA)
+// cf32: C stored as float (FP64 MFMA); spl: split K with the second A FP64 (1) or
+// FP32 (2) (FP64 MFMA)
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
-                               double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false)
+                               double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false,
+                               int spl = 0)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (cf32)
+    if (spl == 1)
+        hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (spl == 2)
+        hipLaunchKernelGGL((k_gemm_tall<false, false, false, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (cf32)
         hipLaunchKernelGGL((k_gemm_tall<false, false, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (af32)
         hipLaunchKernelGGL((k_gemm_tall<true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -2065,16 +2089,18 @@ struct sfb_plan {
     int64_t lds_max = 65536;
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
     // work
-    DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0, d_desc_c0, d_desc_al;
+    DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0;
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
-    DevBuf d_desc_syn32, d_desc_al32;  // aliasing-ring synthesis / analysis with G, R as floats (mixed)
+    DevBuf d_desc_syn32;           // aliasing-ring synthesis with G as floats (mixed path)
     DevBuf d_qbase, d_lmaxn;       // per n: map index of (n, 0) and lmax_n (int32)
     std::vector<int32_t> lmaxn_h;  // lmax_n on the host
     DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
+    std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
+    DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
     double* a_final = nullptr;     // the a buffer holding the last result
     // catalogue
     DevBuf d_keyr, d_keyr2, d_idx, d_idx2, d_pix, d_pixkey, d_pixkey2, d_rs, d_ws, d_pstart, d_sorttmp;
@@ -2470,6 +2496,21 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     }
     if ((rc = P->d_ic0.ensure(sizeof(double) * std::max<size_t>(1, ic0.size())))) return rc;
     if (!ic0.empty()) HIP_TRY(hipMemcpy(P->d_ic0.p, ic0.data(), sizeof(double) * ic0.size(), hipMemcpyHostToDevice));
+    // dOmega Lam on the aliasing ring pairs: the residual-analysis half of the merged
+    // Jacobi GEMM (its rows k < K_al of every (m, parity) block)
+    P->lamw_off.assign((size_t)(2 * (L + 1) + 1), 0);
+    for (size_t b = 0; b < (size_t)(2 * (L + 1)); ++b)
+        P->lamw_off[b + 1] = P->lamw_off[b] + (int64_t)P->K_al * P->nb[b];
+    {
+        std::vector<double> lamw((size_t)std::max<int64_t>(1, P->lamw_off.back()));
+        for (size_t b = 0; b < (size_t)(2 * (L + 1)); ++b) {
+            const int64_t n = P->nb[b];
+            for (int64_t e = 0; e < (int64_t)P->K_al * n; ++e)
+                lamw[(size_t)(P->lamw_off[b] + e)] = domega * lam[(size_t)(P->lam_off[b] + e)];
+        }
+        if ((rc = P->d_lamw.ensure(sizeof(double) * lamw.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_lamw.p, lamw.data(), sizeof(double) * lamw.size(), hipMemcpyHostToDevice));
+    }
 
     // work buffers
     P->x_size = (L + 1) * 2 * (int64_t)P->K * P->M2;
@@ -2485,10 +2526,11 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
 
     // GEMM descriptors.  First analysis: A = X0 (k = ring pair), B = Lam, C = b (d_a).
     // Iteration slot s reads a_in and writes a_out:
-    //   syn: G_al = a_in LamT (first K_al ring pairs)       C0: a_out = a_in (I - C0) + b
-    //   al : a_out += dOmega R_al Lam[first K_al ring pairs]
+    //   syn: G_al = a_in LamT (first K_al ring pairs)
+    //   cal: a_out = b + a_in (I - C0) + R_al dOmega Lam[first K_al ring pairs], one GEMM
+    //        with K split at n (A rows from a_in, then from R; B rows from I - C0, then dOmega Lam)
     const int64_t nbat = 2 * (L + 1);
-    std::vector<GemmDesc> ana, ana32, syn, c0, al, syn32, al32;
+    std::vector<GemmDesc> ana, ana32, syn, syn32, cal, cal32;
     double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
     double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
     for (int64_t b = 0; b < nbat; ++b) {
@@ -2535,21 +2577,19 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             cd.lda = P->M2;
             cd.ldb = n;
             cd.ldc = P->M2;
-            c0.push_back(cd);
-            GemmDesc ad{};
-            ad.A = P->d_R.as<double>() + b * (int64_t)P->K_al * P->M2;
-            ad.B = P->d_lam.as<double>() + P->lam_off[(size_t)b];
-            ad.C = a_out_slot[slot] + P->aoff[(size_t)b];
-            ad.Cin = nullptr;
-            ad.K = P->K_al;
-            ad.N = (P->K_al > 0) ? n : 0;
-            ad.lda = P->M2;
-            ad.ldb = n;
-            ad.ldc = P->M2;
-            al.push_back(ad);
-            // mixed path: R holds floats
-            ad.A = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
-            al32.push_back(ad);
+            // merged: a_out = b + [(I - C0) | dOmega Lam_al] [a_in ; R] (K = n + K_al)
+            GemmDesc md = cd;
+            md.K1 = n;
+            if (P->K_al > 0 && n > 0) {
+                md.A2 = P->d_R.as<double>() + b * (int64_t)P->K_al * P->M2;
+                md.B2 = P->d_lamw.as<double>() + P->lamw_off[(size_t)b];
+                md.K = n + P->K_al;
+                md.lda2 = P->M2;
+                md.ldb2 = n;
+            }
+            cal.push_back(md);
+            if (P->K_al > 0 && n > 0) md.A2 = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
+            cal32.push_back(md);  // mixed path: R holds floats
         }
     // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over all K ring pairs,
     // one set per a buffer (d_a, d_a1, d_a2)
@@ -2575,17 +2615,15 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     if ((rc = P->d_desc_ana32.ensure(sizeof(GemmDesc) * ana32.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_ana32.p, ana32.data(), sizeof(GemmDesc) * ana32.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_desc_syn.ensure(sizeof(GemmDesc) * syn.size()))) return rc;
-    if ((rc = P->d_desc_c0.ensure(sizeof(GemmDesc) * c0.size()))) return rc;
-    if ((rc = P->d_desc_al.ensure(sizeof(GemmDesc) * al.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_ana.p, ana.data(), sizeof(GemmDesc) * ana.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P->d_desc_syn.p, syn.data(), sizeof(GemmDesc) * syn.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_desc_c0.p, c0.data(), sizeof(GemmDesc) * c0.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_desc_al.p, al.data(), sizeof(GemmDesc) * al.size(), hipMemcpyHostToDevice));
-    if ((rc = P->d_desc_syn32.ensure(sizeof(GemmDesc) * syn32.size())) ||
-        (rc = P->d_desc_al32.ensure(sizeof(GemmDesc) * al32.size())))
-        return rc;
+    if ((rc = P->d_desc_syn32.ensure(sizeof(GemmDesc) * syn32.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_syn32.p, syn32.data(), sizeof(GemmDesc) * syn32.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_desc_al32.p, al32.data(), sizeof(GemmDesc) * al32.size(), hipMemcpyHostToDevice));
+    if ((rc = P->d_desc_cal.ensure(sizeof(GemmDesc) * cal.size())) ||
+        (rc = P->d_desc_cal32.ensure(sizeof(GemmDesc) * cal32.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_cal.p, cal.data(), sizeof(GemmDesc) * cal.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_desc_cal32.p, cal32.data(), sizeof(GemmDesc) * cal32.size(), hipMemcpyHostToDevice));
     return SFB_OK;
 }
 
@@ -2665,7 +2703,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     prof_collect(P);
-    DevBuf* bufs[] = {&P->d_a1, &P->d_a2, &P->d_ic0, &P->d_desc_c0, &P->d_desc_al, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
+    DevBuf* bufs[] = {&P->d_a1, &P->d_a2, &P->d_ic0, &P->d_rec, &P->d_maps, &P->d_kx, &P->d_seg, &P->d_rings, &P->d_tw, &P->d_lam, &P->d_lamT,
                       &P->d_outbase, &P->d_aoff, &P->d_desc_ana, &P->d_desc_syn, &P->d_flag, &P->d_delta,
                       &P->d_X0, &P->d_G, &P->d_R, &P->d_a, &P->d_keyr, &P->d_keyr2, &P->d_idx, &P->d_idx2,
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
@@ -2674,8 +2712,9 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
-                      &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange,
-                      &P->d_desc_ana32, &P->d_desc_syn32, &P->d_desc_al32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
+                      &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
+                      &P->d_desc_cal32,
+                      &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -3028,7 +3067,6 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int32_t rows = 2 * nq;
-    const double domega = 4.0 * kPi / (double)P->npix;
     const unsigned nbat = (unsigned)(2 * (L + 1));
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
@@ -3038,8 +3076,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
         const GemmDesc* syn = (f32 ? P->d_desc_syn32 : P->d_desc_syn).as<GemmDesc>() + slot * nbat;
-        const GemmDesc* c0 = P->d_desc_c0.as<GemmDesc>() + slot * nbat;
-        const GemmDesc* al = (f32 ? P->d_desc_al32 : P->d_desc_al).as<GemmDesc>() + slot * nbat;
+        const GemmDesc* cal = (f32 ? P->d_desc_cal32 : P->d_desc_cal).as<GemmDesc>() + slot * nbat;
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
@@ -3058,13 +3095,9 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
             }
         }
         {
-            PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-            launch_gemm(c0, nbat, rows, maxnb, 1.0, 1.0, st);
-            LAUNCH_CHECK();
-        }
-        if (Kal > 0) {
+            // a_out = b + (I - C0) a_in + dOmega Lam_al R in one GEMM over K = n + K_al
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            launch_gemm(al, nbat, rows, maxnb, domega, 1.0, st, f32, f32);
+            launch_gemm(cal, nbat, rows, maxnb, 1.0, 1.0, st, false, false, false, f32 ? 2 : 1);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -3826,7 +3859,7 @@ static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const 
         return rc;
     hipError_t e = hipMemcpy(dA.p, A, sizeof(double) * (size_t)(K * rows), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dB.p, B, sizeof(double) * (size_t)(K * N), hipMemcpyHostToDevice);
-    GemmDesc d;
+    GemmDesc d{};
     d.A = dA.as<double>();
     d.B = dB.as<double>();
     d.C = dC.as<double>();
