@@ -1219,7 +1219,9 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             const int gk = k0 + kk, gi = i0 + ii;
             if (SPL && gk >= d.K1) {  // second A: FP64 (SPL 1) or FP32 (SPL 2)
                 const int64_t o = (int64_t)(gk - d.K1) * d.lda2 + gi;
-                if (SPL == 2)
+                if (SPL == 2 && F32)  // FP32 operands: no conversion
+                    ra[u] = (gk < d.K && gi < rows) ? (T)((gcfptr)d.A2)[o] : (T)0;
+                else if (SPL == 2)
                     rf[u] = (gk < d.K && gi < rows) ? ((gcfptr)d.A2)[o] : 0.0f;
                 else
                     ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A2)[o] : (T)0;
@@ -1244,7 +1246,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
 #pragma unroll
         for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
-            As[e >> 6][e & 63] = (SPL == 2 && k0 + (e >> 6) >= d.K1) ? (T)rf[u] : ra[u];
+            As[e >> 6][e & 63] = (SPL == 2 && !F32 && k0 + (e >> 6) >= d.K1) ? (T)rf[u] : ra[u];
         }
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
@@ -1345,7 +1347,11 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
                                int spl = 0)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (spl == 1)
+    if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (f32 && af32 && cf32)
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (spl == 1)
         hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 2)
         hipLaunchKernelGGL((k_gemm_tall<false, false, false, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -1427,7 +1433,8 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
 }
 
 // anlm[base_q + m] = a^{(q)}_{l_q, m}, m = 0..l_q.
-__global__ void k_pack(const double* __restrict__ a, const int64_t* __restrict__ aoff, const MapInfo* __restrict__ maps,
+template <typename TA>
+__global__ void k_pack(const TA* __restrict__ a, const int64_t* __restrict__ aoff, const MapInfo* __restrict__ maps,
                        const int64_t* __restrict__ out_base, int32_t nq, int64_t M2, double* __restrict__ out,
                        int32_t* __restrict__ flag)
 {
@@ -1437,8 +1444,8 @@ __global__ void k_pack(const double* __restrict__ a, const int64_t* __restrict__
     const int l = maps[q].l;
     if (m > l) return;
     const int par = (l - m) & 1, j = (l - m) >> 1;
-    const double* src = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
-    double re = src[0], im = src[1];
+    const TA* src = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
+    const double re = src[0], im = src[1];
     out[2 * (out_base[q] + m)] = re;
     out[2 * (out_base[q] + m) + 1] = im;
     if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
@@ -2102,6 +2109,7 @@ struct sfb_plan {
     std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
     DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
     double* a_final = nullptr;     // the a buffer holding the last result
+    bool a_f32 = false;            // ... stored as floats (mixed path)
     // catalogue
     DevBuf d_keyr, d_keyr2, d_idx, d_idx2, d_pix, d_pixkey, d_pixkey2, d_rs, d_ws, d_pstart, d_sorttmp;
     // host-API staging
@@ -2531,6 +2539,10 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     //        with K split at n (A rows from a_in, then from R; B rows from I - C0, then dOmega Lam)
     const int64_t nbat = 2 * (L + 1);
     std::vector<GemmDesc> ana, ana32, syn, syn32, cal, cal32;
+    // mixed path: the iterate a (and b) held as floats: block offsets count floats
+    auto fa = [&](DevBuf& buf, size_t b) { return (double*)(buf.as<float>() + P->aoff[b]); };
+    DevBuf* a_in_buf[3] = {&P->d_a, &P->d_a1, &P->d_a2};
+    DevBuf* a_out_buf[3] = {&P->d_a1, &P->d_a2, &P->d_a1};
     double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
     double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
     for (int64_t b = 0; b < nbat; ++b) {
@@ -2546,8 +2558,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         d.ldb = n;
         d.ldc = P->M2;
         ana.push_back(d);
-        // mixed path: X0 holds floats, block b starts b K M2 floats in
+        // mixed path: X0 holds floats, block b starts b K M2 floats in; a written as floats
         d.A = (const double*)(P->d_X0.as<float>() + b * (int64_t)P->K * P->M2);
+        d.C = fa(P->d_a, (size_t)b);
         ana32.push_back(d);
     }
     for (int slot = 0; slot < 3; ++slot)
@@ -2564,7 +2577,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             sd.ldb = P->K;
             sd.ldc = P->M2;
             syn.push_back(sd);
-            // mixed path: G holds floats
+            // mixed path: a and G hold floats
+            sd.A = fa(*a_in_buf[slot], (size_t)b);
             sd.C = (double*)(P->d_G.as<float>() + b * (int64_t)P->K_al * P->M2);
             syn32.push_back(sd);
             GemmDesc cd{};
@@ -2588,8 +2602,12 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
                 md.ldb2 = n;
             }
             cal.push_back(md);
+            // mixed path: a_in, R, b and a_out hold floats
             if (P->K_al > 0 && n > 0) md.A2 = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
-            cal32.push_back(md);  // mixed path: R holds floats
+            md.A = fa(*a_in_buf[slot], (size_t)b);
+            md.Cin = fa(P->d_a, (size_t)b);
+            md.C = fa(*a_out_buf[slot], (size_t)b);
+            cal32.push_back(md);
         }
     // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over all K ring pairs,
     // one set per a buffer (d_a, d_a1, d_a2)
@@ -3071,6 +3089,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
     const int32_t maxnb = P->nb[0];
     const int32_t Kal = P->K_al;
     P->a_final = P->d_a.as<double>();
+    P->a_f32 = f32;
     const int32_t nqt = (int32_t)nblk(nq, 256);
     const dim3 gr((unsigned)(8 * ((std::max(Kal, 1) * nqt + 7) / 8) * (L + 1)));
     for (int it = 0; it < P->niter; ++it) {
@@ -3080,7 +3099,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, false, false, f32);
+                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, f32, f32, f32);
                 LAUNCH_CHECK();
             }
             {
@@ -3097,7 +3116,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
         {
             // a_out = b + (I - C0) a_in + dOmega Lam_al R in one GEMM over K = n + K_al
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            launch_gemm(cal, nbat, rows, maxnb, 1.0, 1.0, st, false, false, false, f32 ? 2 : 1);
+            launch_gemm(cal, nbat, rows, maxnb, 1.0, 1.0, st, f32, f32, f32, f32 ? 2 : 1);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -3118,7 +3137,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
     {
         PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
         launch_gemm((f32 ? P->d_desc_ana32 : P->d_desc_ana).as<GemmDesc>(), (unsigned)(2 * (L + 1)), 2 * nq,
-                    P->nb[0], domega, 0.0, st, f32, f32);
+                    P->nb[0], domega, 0.0, st, f32, f32, f32);
         LAUNCH_CHECK();
     }
     return jacobi_stage(P, nq, st, f32);
@@ -3201,9 +3220,14 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st, f32))) return rc;
     {
         PhaseTimer timer(P, SFB_PHASE_PACK, st);
-        hipLaunchKernelGGL(k_pack, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
-                           P->a_final, P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
-                           P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
+        if (P->a_f32)
+            hipLaunchKernelGGL(k_pack<float>, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
+                               (const float*)P->a_final, P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
+                               P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
+        else
+            hipLaunchKernelGGL(k_pack<double>, dim3(nblk(P->nmaps, 256), (unsigned)(P->lmax + 1)), dim3(256), 0, st,
+                               P->a_final, P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(),
+                               P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
         LAUNCH_CHECK();
     }
     int32_t flag[2] = {0, 0};
