@@ -39,9 +39,9 @@ extern "C" {
 #define SFB_ERR_NOMEM 6      /* host allocation failed */
 
 #define SFB_PREC_FP64 0      /* plain path: reference method, FP64 throughout */
-#define SFB_PREC_MIXED 1     /* FP32 spline values, delta maps and phi-transform, first analysis
-                                on FP32 MFMA; FP64 setup, segment search, lambda tables, GEMM
-                                epilogues and the Jacobi iterations */
+#define SFB_PREC_MIXED 1     /* FP32 spline values, delta maps, phi-transform and every Legendre
+                                pass (FP32 MFMA; FP32 iterate and aliasing buffers); FP64 setup,
+                                segment search, lambda / C0 tables, GEMM epilogues and aliasing sums */
 
 #define SFB_WIN_DENSE 0      /* win_a = W[npix, lmax+1, nmax] */
 #define SFB_WIN_SEPARABLE 1  /* win_a = mask[npix], win_b = w_ln[lmax+1, nmax] */

@@ -268,26 +268,28 @@ __global__ void k_sell_keys(int64_t npix, const int32_t* __restrict__ pstart, ui
 
 // slice widths 64 * (largest count among the slice's pixels) and the inverse
 // pixel permutation; one thread per slice
-__global__ void k_sell_slices(int64_t npix, int32_t nslice, const uint32_t* __restrict__ perm,
-                              const int32_t* __restrict__ pstart, int64_t* __restrict__ width,
-                              uint32_t* __restrict__ invperm, int2* __restrict__ pinfo)
+// One wavefront per slice, one lane per slice row: the pixel, its inverse position,
+// its (pixel, galaxy count) and the slice width 64 * (largest count in the slice).
+__global__ __launch_bounds__(256) void k_sell_slices(int64_t npix, int32_t nslice, const uint32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ pstart, int64_t* __restrict__ width,
+                                                     uint32_t* __restrict__ invperm, int2* __restrict__ pinfo)
 {
-    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslice) return;
-    const int64_t e0 = 64 * (int64_t)s, e1 = min(npix, e0 + 64);
-    int32_t m = 0;
-    for (int64_t e = e0; e < e0 + 64; ++e) {
-        if (e >= e1) {  // lanes past the last pixel: no pixel, no galaxies
-            pinfo[e] = make_int2(0, 0);
-            continue;
-        }
+    const int32_t s = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= nslice) return;  // whole wavefronts leave together
+    const int64_t e = 64 * (int64_t)s + lane;
+    int32_t c = 0;
+    if (e < npix) {
         const uint32_t p = perm[e];
         invperm[p] = (uint32_t)e;
-        const int32_t c = pstart[p + 1] - pstart[p];
+        c = pstart[p + 1] - pstart[p];
         pinfo[e] = make_int2((int)p, c);
-        m = max(m, c);
+    } else {  // lanes past the last pixel: no pixel, no galaxies
+        pinfo[e] = make_int2(0, 0);
     }
-    width[s] = 64 * (int64_t)m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, 64));
+    if (lane == 0) width[s] = 64 * (int64_t)c;
 }
 
 // separable window: the pixel factor of every slice lane, win_a[pixel] (0 past npix)
@@ -2842,7 +2844,8 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp4, P->d_skey.as<uint32_t>(), P->d_skey2.as<uint32_t>(),
                                           P->d_sval.as<uint32_t>(), P->d_perm.as<uint32_t>(), (unsigned)P->npix, 0,
                                           16 + wbits, st));
-        hipLaunchKernelGGL(k_sell_slices, dim3(nblk(P->sell_nslice, 256)), dim3(256), 0, st, P->npix, P->sell_nslice,
+        hipLaunchKernelGGL(k_sell_slices, dim3(nblk(64 * (int64_t)P->sell_nslice, 256)), dim3(256), 0, st, P->npix,
+                           P->sell_nslice,
                            P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_swidth.as<int64_t>(),
                            P->d_invperm.as<uint32_t>(), P->d_pinfo.as<int2>());
         LAUNCH_CHECK();
