@@ -626,6 +626,10 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     }
 }
 
+template <typename R> struct Vec2T;
+template <> struct Vec2T<double> { typedef double2 T; };
+template <> struct Vec2T<float> { typedef float2 T; };
+
 // Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
 // every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
 // each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
@@ -646,44 +650,51 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
             const R fsr = asr * phx + asi * phy, fsi = asi * phx - asr * phy;
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
+            typedef typename Vec2T<R>::T R2;  // (re, im) pairs start at even offsets
+            R2 ve, vo;
             if (eq) {
-                X[be] = fnr;
-                X[be + 1] = fni;
-                X[bo] = fnr;
-                X[bo + 1] = fni;
+                ve.x = fnr; ve.y = fni;
+                vo = ve;
             } else {
-                X[be] = fnr + fsr;
-                X[be + 1] = fni + fsi;
-                X[bo] = fnr - fsr;
-                X[bo + 1] = fni - fsi;
+                ve.x = fnr + fsr; ve.y = fni + fsi;
+                vo.x = fnr - fsr; vo.y = fni - fsi;
             }
+            *(R2*)(X + be) = ve;
+            *(R2*)(X + bo) = vo;
         }
     }
 }
 
-template <typename R> struct Vec2T;
-template <> struct Vec2T<double> { typedef double2 T; };
-template <> struct Vec2T<float> { typedef float2 T; };
+
+// FFT sequences in LDS are padded by one element per 256 bytes (16 complex FP64,
+// 32 complex FP32): element x of a sequence sits at fpad(x) = x + (x >> FFT_PAD_SH),
+// and a sequence takes fft_plen(M) slots.  Unpadded, the bit-reversed stores put
+// the 16 lanes of an LDS cycle on one bank group, as do the first butterfly stages.
+constexpr bool FFT_PAD = true;
+template <typename R> struct FftPad { static constexpr int SH = sizeof(R) == 8 ? 4 : 5; };
+template <typename R> __device__ __forceinline__ int fpad(int x) { return FFT_PAD ? x + (x >> FftPad<R>::SH) : x; }
+static inline int64_t fft_plen(int64_t M, size_t vbytes) { return FFT_PAD ? M + (M >> (vbytes == 8 ? 4 : 5)) : M; }
 
 // In-place forward DFT (e^{-i}) of nf sequences of length M = N/2 held bit-reversed
-// in LDS: the radix-2 stages fused in pairs (one radix-4 pass of 4 loads / 3
-// twiddles / 4 butterflies per thread item between barriers), then a last radix-2
-// stage when log2 M is odd.  twr[kk] = (cos, sin)(pi kk / N), kk <= N.
+// (and padded, stride Mp) in LDS: the radix-2 stages fused in pairs (one radix-4 pass
+// of 4 loads / 3 twiddles / 4 butterflies per thread item between barriers), then a
+// last radix-2 stage when log2 M is odd.  twr[kk] = (cos, sin)(pi kk / N), kk <= N.
 template <typename R>
-__device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int M, int N, int logM,
+__device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int M, int Mp, int N, int logM,
                                            const typename Vec2T<R>::T* twr)
 {
     typedef typename Vec2T<R>::T R2;
     int st = 1;
     for (; st + 1 <= logM; st += 2) {
         const int half = 1 << (st - 1);
-        const int tstep = N / half;  // stage st twiddle step; stage st+1 uses tstep/2
-        const int q4 = M >> 2;
-        for (int e = threadIdx.x; e < nf * q4; e += blockDim.x) {
-            const int f = e / q4, b = e - f * q4;
-            const int grp = b / half, pos = b - grp * half;
-            const int a0 = f * M + grp * 4 * half + pos;
-            const int a1 = a0 + half, a2 = a0 + 2 * half, a3 = a0 + 3 * half;
+        const int tstep = N >> (st - 1);  // stage st twiddle step; stage st+1 uses tstep/2
+        const int lq4 = logM - 2;         // items per sequence: M/4 = 2^lq4
+        for (int e = threadIdx.x; e < (nf << lq4); e += blockDim.x) {
+            const int f = e >> lq4, b = e & ((1 << lq4) - 1);
+            const int grp = b >> (st - 1), pos = b & (half - 1);
+            const int l0 = grp * 4 * half + pos;
+            const int a0 = f * Mp + fpad<R>(l0), a1 = f * Mp + fpad<R>(l0 + half);
+            const int a2 = f * Mp + fpad<R>(l0 + 2 * half), a3 = f * Mp + fpad<R>(l0 + 3 * half);
             const R2 w1 = twr[pos * tstep];
             const R2 w2 = twr[pos * (tstep >> 1)];
             const R2 w3 = twr[(pos + half) * (tstep >> 1)];
@@ -705,11 +716,12 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
     }
     if (st == logM) {  // one radix-2 stage left
         const int half = 1 << (st - 1);
-        const int tstep = N / half;
-        for (int e = threadIdx.x; e < nf * (M >> 1); e += blockDim.x) {
-            const int f = e / (M >> 1), b = e - f * (M >> 1);
-            const int grp = b / half, pos = b - grp * half;
-            const int i0 = f * M + grp * 2 * half + pos, i1 = i0 + half;
+        const int tstep = N >> (st - 1);
+        for (int e = threadIdx.x; e < (nf << (logM - 1)); e += blockDim.x) {
+            const int f = e >> (logM - 1), b = e & ((1 << (logM - 1)) - 1);
+            const int grp = b >> (st - 1), pos = b & (half - 1);
+            const int l0 = grp * 2 * half + pos;
+            const int i0 = f * Mp + fpad<R>(l0), i1 = f * Mp + fpad<R>(l0 + half);
             const R2 w = twr[pos * tstep];
             const R2 a = zs[i0], c = zs[i1];
             const R cr = c.x * w.x + c.y * w.y;
@@ -738,43 +750,45 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
     const int k = k0 + blockIdx.x;
     const int q0 = blockIdx.y * TQ;
     const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1;
+    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
+    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1, 2 or 4)
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
-    R2* twr = zs + nf * M;                // kk <= N staged in LDS
+    R2* twr = zs + nf * Mp;               // kk <= N staged in LDS
     for (int e = threadIdx.x; e <= N; e += blockDim.x) {
         const double2 w = twg[e];
         twr[e].x = (R)w.x;
         twr[e].y = (R)w.y;
     }
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e / M, jj = e - f * M;
+        const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
         const int ql = f >> 1, side = f & 1;
         const int q = q0 + ql;
         R re = 0, im = 0;
-        if (q < nq && !(side && eq)) {
+        if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
             const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
-            re = delta[base];
-            im = delta[base + 1];
+            const R2 v = *(const R2*)(delta + base);
+            re = v.x;
+            im = v.y;
         }
         const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-        zs[f * M + rj].x = re;
-        zs[f * M + rj].y = im;
+        zs[f * Mp + fpad<R>(rj)].x = re;
+        zs[f * Mp + fpad<R>(rj)].y = im;
     }
     __syncthreads();
-    fft_passes<R>(zs, nf, M, N, logM, twr);
+    fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
     // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M
     const int nm = min(L, M);
     for (int e = threadIdx.x; e < TQ * (nm + 1); e += blockDim.x) {
-        const int m = e / TQ, ql = e - m * TQ;  // maps fastest: contiguous X stores
+        const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X stores
         const int q = q0 + ql;
         if (q >= nq) continue;
         R xr[2], xi[2];
         for (int side = 0; side < 2; ++side) {
-            const R2* z = zs + (2 * ql + side) * M;
-            const R2 zm = z[m & (M - 1)];
-            const R2 zc = z[(M - m) & (M - 1)];
+            const R2* z = zs + (2 * ql + side) * Mp;
+            const R2 zm = z[fpad<R>(m & (M - 1))];
+            const R2 zc = z[fpad<R>((M - m) & (M - 1))];
             const R er = (R)0.5 * (zm.x + zc.x), ei = (R)0.5 * (zm.y - zc.y);
             const R orr = (R)0.5 * (zm.y + zc.y), oi = (R)-0.5 * (zm.x - zc.x);
             const R2 w = twr[2 * m];
@@ -809,16 +823,17 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
     const int k = k0 + unit / nqt;
     const int q0 = (unit - (unit / nqt) * nqt) * TQ;
     const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1;
+    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<double>::SH);
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
-    double2* zs = zi_raw;                 // nf x M
-    double2* twr = zs + nf * M;           // N + 1 twiddles
+    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1, 2 or 4)
+    double2* zs = zi_raw;                 // nf x Mp (padded sequences)
+    double2* twr = zs + nf * Mp;          // N + 1 twiddles
     double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     for (int e = threadIdx.x; e <= N; e += blockDim.x) twr[e] = twg[e];
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
-        const int m = e / TQ, ql = e - m * TQ;  // maps fastest: contiguous X loads
+        const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X loads
         const int q = q0 + ql;
         double er = 0, ei = 0, orr = 0, oi = 0;
         if (q < nq) {
@@ -839,7 +854,7 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e / M, kk = e - f * M;
+        const int f = e >> logM, kk = e & (M - 1);  // M = 2^logM
         const double2* h = hs + f * (L + 1);
         auto spec = [&](int b) -> double2 {  // X_b, 0 <= b <= M
             if (b == 0) return make_double2(2.0 * h[0].x, 0.0);
@@ -858,16 +873,16 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
         const double orr = dr * w.x - di * w.y, oi = dr * w.y + di * w.x;
         // Z = E + i O; conj(Z) goes in bit-reversed
         const int rk = (int)(__brev((unsigned)kk) >> (32 - logM));
-        zs[f * M + rk] = make_double2(er - oi, -(ei + orr));
+        zs[f * Mp + fpad<double>(rk)] = make_double2(er - oi, -(ei + orr));
     }
     __syncthreads();
-    fft_passes<double>(zs, nf, M, N, logM, twr);
+    fft_passes<double>(zs, nf, M, Mp, N, logM, twr);
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e / M, j = e - f * M;
+        const int f = e >> logM, j = e & (M - 1);
         const int ql = f >> 1, side = f & 1;
         const int q = q0 + ql;
         if (q >= nq || (side && eq)) continue;
-        const double2 z = zs[e];  // conj(z_j)
+        const double2 z = zs[f * Mp + fpad<double>(j)];  // conj(z_j)
         const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * j;
         out[base] = scale * z.x;
         out[base + 1] = -scale * z.y;
@@ -3034,7 +3049,7 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
     const size_t vb = f32 ? sizeof(float) : sizeof(double);
     // FP32 sequences take half the LDS: twice as many per block at the same occupancy
     const int32_t tqf = f32 ? 2 * P->TQF : P->TQF;
-    const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)M + 2 * (size_t)M + 1);
+    const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(256), shf, st,
@@ -3154,7 +3169,9 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int64_t N = 4 * P->nside, M = N / 2;
     int32_t TQI = 2;
-    auto ifft_lds = [&](int32_t tq) { return sizeof(double2) * (size_t)(2 * tq * M + N + 1 + 2 * tq * (L + 1)); };
+    auto ifft_lds = [&](int32_t tq) {
+        return sizeof(double2) * (size_t)(2 * tq * fft_plen(M, sizeof(double)) + N + 1 + 2 * tq * (L + 1));
+    };
     while (TQI > 0 && (int64_t)ifft_lds(TQI) > P->lds_max) --TQI;
     const int32_t k_eq0 = TQI > 0 ? (int32_t)(P->nside - 1) : K;  // first ring pair on the FFT
     // polar ring pairs by the radix-4 inverse when its largest block fits in LDS
