@@ -1390,62 +1390,101 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
 // carried by the dense per-m matrix C0; this kernel writes minus the m' != m
 // terms in E/O form for the first K_al ring pairs (the rings with npr <= 2 lmax).
 // G holds the parity halves Ge, Go per ring pair: north = Ge + Go, south = Ge - Go.
-// Grid: one block per (ring pair k, tile of 256 maps, m), XCD-aware: workgroup b
-// runs on XCD b mod 8, and XCD x takes the (k, tile) units x, x + 8, ..., each with
-// its L + 1 values of m in a row, so the G of a unit (all m of one ring pair and
-// tile, which every m of the unit reads) is fetched from HBM once into that XCD's L2.
+// Summed by residue class: for ring pair k, residue r < npr and map q,
+//   F_r = sum_{m' = r (mod npr), |m'| <= L} (-1)^{((m'-r)/npr) shift} Gt_{m'}   (Gt_{-m} = conj G_m)
+// once, then every member m >= 0 of the class gets
+//   R_m = -npr ((-1)^{((m-r)/npr) shift} F_r - Gt_m),
+// the sum over m' != m ((-1)^{a-b} = (-1)^a (-1)^b).  A class of c members costs 2c loads
+// instead of c^2 direct ones (c = 71 on the 4-pixel rings at lmax 140); classes of at most
+// three members (the long rings) take the direct sums, which are then no more loads.
+// Grid: one block per (ring pair k, tile of 256 maps, residue r < min(lmax + 1, max npr)),
+// XCD-aware: workgroup b runs on XCD b mod 8, and XCD x takes the (k, tile) units x, x + 8,
+// ..., each with all its residues in a row, so the G of a unit is fetched from HBM once into
+// that XCD's L2.  Residues past the ring's npr (or past lmax) leave at once.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __restrict__ R,
-                                               const RingDev* __restrict__ rings, int32_t K, int32_t K_al, int32_t L,
-                                               int64_t M2, int32_t nq, int32_t nqt)
+                                               const RingDev* __restrict__ rings, int32_t K, int32_t K_al,
+                                               int32_t L, int64_t M2, int32_t nq, int32_t nqt, int32_t rmax)
 {
     const int b = blockIdx.x, xcd = b & 7, loc = b >> 3;
-    const int unit = (loc / (L + 1)) * 8 + xcd;
-    const int m = loc - (loc / (L + 1)) * (L + 1);
+    const int unit = (loc / rmax) * 8 + xcd;
+    const int r = loc - (loc / rmax) * rmax;
     if (unit >= K_al * nqt) return;
     const int k = unit / nqt;
-    const int q = (unit - k * nqt) * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr;
+    if (r >= npr || r > L) return;  // r > L: no member m >= 0
+    const int q = (unit - k * nqt) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
     const bool eq = (k == K - 1);
-    double snr = 0, sni = 0, ssr = 0, ssi = 0;
-    const int mstart = m - ((m + L) / npr) * npr;
-    for (int mp = mstart; mp <= L; mp += npr) {
-        if (mp == m) continue;
-        const int t = (mp - m) / npr;
-        const double sg = ((t * Rg.shift) & 1) ? -1.0 : 1.0;
+    // north = E + O, south = E - O of member m' (conjugated for m' < 0)
+    auto member = [&](int mp, double& nr, double& ni, double& sr, double& si) {
         const int ma = mp < 0 ? -mp : mp;
-        const double cj = mp < 0 ? -1.0 : 1.0;
         const T* ge = G + ((int64_t)(ma * 2 + 0) * K_al + k) * M2 + 2 * q;
-        double er = ge[0], ei = ge[1], orr = 0.0, oi = 0.0;
+        const double er = ge[0], ei = ge[1];
+        double orr = 0.0, oi = 0.0;
         if (ma < L) {  // the odd half of m' = lmax is empty
             const T* go = G + ((int64_t)(ma * 2 + 1) * K_al + k) * M2 + 2 * q;
             orr = go[0];
             oi = go[1];
         }
-        snr += sg * (er + orr);
-        sni += sg * cj * (ei + oi);
-        ssr += sg * (er - orr);
-        ssi += sg * cj * (ei - oi);
+        const double cj = mp < 0 ? -1.0 : 1.0;
+        nr = er + orr;
+        ni = cj * (ei + oi);
+        sr = er - orr;
+        si = cj * (ei - oi);
+    };
+    const int m0 = r - ((r + L) / npr) * npr;  // smallest member >= -L
+    auto put = [&](int m, double snr, double sni, double ssr, double ssi) {
+        const int64_t be = ((int64_t)(m * 2 + 0) * K_al + k) * M2 + 2 * q;
+        const int64_t bo = ((int64_t)(m * 2 + 1) * K_al + k) * M2 + 2 * q;
+        if (eq) {
+            R[be] = (T)-snr;
+            R[be + 1] = (T)-sni;
+            R[bo] = (T)-snr;
+            R[bo + 1] = (T)-sni;
+        } else {
+            R[be] = (T)-(snr + ssr);
+            R[be + 1] = (T)-(sni + ssi);
+            R[bo] = (T)-(snr - ssr);
+            R[bo + 1] = (T)-(sni - ssi);
+        }
+    };
+    const int c = (L - m0) / npr + 1;  // members in [-L, L]
+    const int cp = (L - r) / npr + 1;  // members in [0, L]
+    if (cp * (c - 1) <= c + cp) {      // short classes (long rings): direct sums
+        for (int m = r; m <= L; m += npr) {
+            double snr = 0, sni = 0, ssr = 0, ssi = 0;
+            for (int mp = m0; mp <= L; mp += npr) {
+                if (mp == m) continue;
+                const double sg = ((((mp - m) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
+                double nr, ni, sr, si;
+                member(mp, nr, ni, sr, si);
+                snr += sg * nr;
+                sni += sg * ni;
+                ssr += sg * sr;
+                ssi += sg * si;
+            }
+            put(m, npr * snr, npr * sni, npr * ssr, npr * ssi);
+        }
+        return;
     }
-    snr *= npr;
-    sni *= npr;
-    ssr *= npr;
-    ssi *= npr;
-    const int64_t be = ((int64_t)(m * 2 + 0) * K_al + k) * M2 + 2 * q;
-    const int64_t bo = ((int64_t)(m * 2 + 1) * K_al + k) * M2 + 2 * q;
-    if (eq) {
-        R[be] = (T)-snr;
-        R[be + 1] = (T)-sni;
-        R[bo] = (T)-snr;
-        R[bo + 1] = (T)-sni;
-    } else {
-        R[be] = (T)-(snr + ssr);
-        R[be + 1] = (T)-(sni + ssi);
-        R[bo] = (T)-(snr - ssr);
-        R[bo + 1] = (T)-(sni - ssi);
+    double fnr = 0, fni = 0, fsr = 0, fsi = 0;
+    for (int mp = m0; mp <= L; mp += npr) {
+        const double sg = ((((mp - r) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
+        double nr, ni, sr, si;
+        member(mp, nr, ni, sr, si);
+        fnr += sg * nr;
+        fni += sg * ni;
+        fsr += sg * sr;
+        fsi += sg * si;
+    }
+    for (int m = r; m <= L; m += npr) {  // members m >= 0
+        const double sg = ((((m - r) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
+        double nr, ni, sr, si;
+        member(m, nr, ni, sr, si);
+        put(m, npr * (sg * fnr - nr), npr * (sg * fni - ni), npr * (sg * fsr - sr), npr * (sg * fsi - si));
     }
 }
 
@@ -2122,6 +2161,7 @@ struct sfb_plan {
     DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
+    int32_t alias_rmax = 0;        // min(lmax + 1, the largest npr among them)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
     std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
     DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
@@ -2492,8 +2532,11 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
 
     // ring pairs that alias: npr <= 2 lmax (they are the first K_al pairs)
     P->K_al = 0;
+    P->alias_rmax = 0;
     for (int32_t k = 0; k < P->K; ++k)
         if (rd[(size_t)k].npr <= 2 * L) P->K_al = k + 1;
+    for (int32_t k = 0; k < P->K_al; ++k) P->alias_rmax = std::max(P->alias_rmax, (int32_t)rd[(size_t)k].npr);
+    P->alias_rmax = std::min(P->alias_rmax, (int32_t)(L + 1));  // residues past lmax hold no m >= 0
     // I - C0 per (m, parity): C0[j][j'] = dOmega sum_rings npr lambda_j lambda_j'
     // (north and south rings give the same product within one parity)
     const double domega = 4.0 * 3.141592653589793 / (double)P->npix;
@@ -3109,7 +3152,9 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
     P->a_final = P->d_a.as<double>();
     P->a_f32 = f32;
     const int32_t nqt = (int32_t)nblk(nq, 256);
-    const dim3 gr((unsigned)(8 * ((std::max(Kal, 1) * nqt + 7) / 8) * (L + 1)));
+    // residues per aliasing ring pair: up to the largest npr among them
+    const int32_t rmax = P->alias_rmax;
+    const dim3 gr((unsigned)(8 * ((std::max(Kal, 1) * nqt + 7) / 8) * std::max(rmax, 1)));
     for (int it = 0; it < P->niter; ++it) {
         const int slot = (it == 0) ? 0 : ((it & 1) ? 1 : 2);
         const GemmDesc* syn = (f32 ? P->d_desc_syn32 : P->d_desc_syn).as<GemmDesc>() + slot * nbat;
@@ -3123,11 +3168,13 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
             {
                 PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
                 if (f32)
-                    hipLaunchKernelGGL(k_alias<float>, gr, dim3(256), 0, st, P->d_G.as<float>(), P->d_R.as<float>(),
-                                       P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
+                    hipLaunchKernelGGL(k_alias<float>, gr, dim3(256), 0, st, P->d_G.as<float>(),
+                                       P->d_R.as<float>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt,
+                                       rmax);
                 else
                     hipLaunchKernelGGL(k_alias<double>, gr, dim3(256), 0, st, P->d_G.as<double>(),
-                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt);
+                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt,
+                                       rmax);
                 LAUNCH_CHECK();
             }
         }
