@@ -1212,7 +1212,7 @@ template <> struct MfmaT<true> {
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
 constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
-template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0>
+template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0, bool BF = false>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
@@ -1252,7 +1252,11 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
             const int e = tid + 256 * u;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
-            if (SPL && gk >= d.K1)
+            if (BF && SPL && gk >= d.K1)  // BF: B and B2 stored as float (F32 operands)
+                rb[u] = (gk < d.K && gn < d.N) ? (T)((gcfptr)d.B2)[(int64_t)(gk - d.K1) * d.ldb2 + gn] : (T)0;
+            else if (BF)
+                rb[u] = (gk < d.K && gn < d.N) ? (T)((gcfptr)d.B)[(int64_t)gk * d.ldb + gn] : (T)0;
+            else if (SPL && gk >= d.K1)
                 rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B2)[(int64_t)(gk - d.K1) * d.ldb2 + gn] : (T)0;
             else
                 rb[u] = (gk < d.K && gn < d.N) ? (T)((gcdptr)d.B)[(int64_t)gk * d.ldb + gn] : (T)0;
@@ -1301,25 +1305,25 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32, bool AF, bool CF = false, int SPL = 0>
+template <bool F32, bool AF, bool CF = false, int SPL = 0, bool BF = false>
 __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
 // CF: C stored as float (the mixed path's aliasing-ring synthesis); SPL: split K,
 // the second A FP64 (1) or FP32 (2).  At least 3 waves per SIMD (<= 168 VGPRs): the
 // Jacobi GEMMs are short-K and bound by the loads in flight per CU.
-template <bool F32, bool AF = false, bool CF = false, int SPL = 0>
+template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false>
 __global__ __launch_bounds__(256, 3) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
@@ -1330,7 +1334,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_tall(const GemmDesc* __restrict
     const int i0 = blockIdx.x * GT_BM;
     const int n0 = blockIdx.y * GT_BNW;
     if (n0 >= d.N) return;
-    gemm_dispatch<F32, AF, CF, SPL>(d, rows, alpha, beta, i0, n0, As, Bs);
+    gemm_dispatch<F32, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs);
 }
 
 // Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
@@ -1358,16 +1362,17 @@ __global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restric
 
 // f32: operands rounded to FP32 on the MFMA; af32: A is stored as float (implies f32);
 // cf32: C stored as float (FP64 MFMA); spl: split K with the second A FP64 (1) or
-// FP32 (2) (FP64 MFMA)
+// FP32 (2) (FP64 MFMA).  With f32, af32 and cf32 all set (the mixed path) B and B2 are
+// FP32 tables too: the MFMA rounds them to FP32 either way, so they take half the bytes.
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
                                double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false,
                                int spl = 0)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
     if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
-        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32)
-        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 1)
         hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 2)
@@ -2165,6 +2170,7 @@ struct sfb_plan {
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
     std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
     DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
+    DevBuf d_lam32, d_lamT32, d_ic032, d_lamw32;  // FP32 copies of the GEMM tables (mixed path)
     double* a_final = nullptr;     // the a buffer holding the last result
     bool a_f32 = false;            // ... stored as floats (mixed path)
     // catalogue
@@ -2564,6 +2570,17 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     }
     if ((rc = P->d_ic0.ensure(sizeof(double) * std::max<size_t>(1, ic0.size())))) return rc;
     if (!ic0.empty()) HIP_TRY(hipMemcpy(P->d_ic0.p, ic0.data(), sizeof(double) * ic0.size(), hipMemcpyHostToDevice));
+    // FP32 copies for the mixed path's GEMMs (their MFMA operands are FP32)
+    auto upload32 = [&](DevBuf& buf, const std::vector<double>& v) -> int {
+        std::vector<float> f(v.begin(), v.end());
+        if (f.empty()) f.push_back(0.0f);
+        int e = buf.ensure(sizeof(float) * f.size());
+        if (e) return e;
+        HIP_TRY(hipMemcpy(buf.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+        return SFB_OK;
+    };
+    if ((rc = upload32(P->d_lam32, lam)) || (rc = upload32(P->d_lamT32, lamT)) || (rc = upload32(P->d_ic032, ic0)))
+        return rc;
     // dOmega Lam on the aliasing ring pairs: the residual-analysis half of the merged
     // Jacobi GEMM (its rows k < K_al of every (m, parity) block)
     P->lamw_off.assign((size_t)(2 * (L + 1) + 1), 0);
@@ -2578,6 +2595,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         }
         if ((rc = P->d_lamw.ensure(sizeof(double) * lamw.size()))) return rc;
         HIP_TRY(hipMemcpy(P->d_lamw.p, lamw.data(), sizeof(double) * lamw.size(), hipMemcpyHostToDevice));
+        if ((rc = upload32(P->d_lamw32, lamw))) return rc;
     }
 
     // work buffers
@@ -2620,6 +2638,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         ana.push_back(d);
         // mixed path: X0 holds floats, block b starts b K M2 floats in; a written as floats
         d.A = (const double*)(P->d_X0.as<float>() + b * (int64_t)P->K * P->M2);
+        d.B = (const double*)(P->d_lam32.as<float>() + P->lam_off[(size_t)b]);
         d.C = fa(P->d_a, (size_t)b);
         ana32.push_back(d);
     }
@@ -2639,6 +2658,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             syn.push_back(sd);
             // mixed path: a and G hold floats
             sd.A = fa(*a_in_buf[slot], (size_t)b);
+            sd.B = (const double*)(P->d_lamT32.as<float>() + P->lam_off[(size_t)b]);
             sd.C = (double*)(P->d_G.as<float>() + b * (int64_t)P->K_al * P->M2);
             syn32.push_back(sd);
             GemmDesc cd{};
@@ -2663,7 +2683,11 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             }
             cal.push_back(md);
             // mixed path: a_in, R, b and a_out hold floats
-            if (P->K_al > 0 && n > 0) md.A2 = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
+            if (P->K_al > 0 && n > 0) {
+                md.A2 = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
+                md.B2 = (const double*)(P->d_lamw32.as<float>() + P->lamw_off[(size_t)b]);
+            }
+            md.B = (const double*)(P->d_ic032.as<float>() + P->ic0_off[(size_t)b]);
             md.A = fa(*a_in_buf[slot], (size_t)b);
             md.Cin = fa(P->d_a, (size_t)b);
             md.C = fa(*a_out_buf[slot], (size_t)b);
@@ -2791,7 +2815,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
-                      &P->d_desc_cal32,
+                      &P->d_desc_cal32, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
