@@ -1409,8 +1409,9 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __restrict__ R,
-                                               const RingDev* __restrict__ rings, int32_t K, int32_t K_al,
-                                               int32_t L, int64_t M2, int32_t nq, int32_t nqt, int32_t rmax)
+                                               const RingDev* __restrict__ rings, const int32_t* __restrict__ almax,
+                                               int32_t K, int32_t K_al, int32_t L, int64_t M2, int32_t nq, int32_t nqt,
+                                               int32_t rmax)
 {
     const int b = blockIdx.x, xcd = b & 7, loc = b >> 3;
     const int unit = (loc / rmax) * 8 + xcd;
@@ -1419,7 +1420,8 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
     const int k = unit / nqt;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr;
-    if (r >= npr || r > L) return;  // r > L: no member m >= 0
+    const int Lk = almax[k];          // G and R of this ring pair matter for m <= Lk only
+    if (r >= npr || r > Lk) return;  // r > Lk: no member m >= 0
     const int q = (unit - k * nqt) * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const bool eq = (k == K - 1);
@@ -1440,7 +1442,7 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
         sr = er - orr;
         si = cj * (ei - oi);
     };
-    const int m0 = r - ((r + L) / npr) * npr;  // smallest member >= -L
+    const int m0 = r - ((r + Lk) / npr) * npr;  // smallest member >= -Lk
     auto put = [&](int m, double snr, double sni, double ssr, double ssi) {
         const int64_t be = ((int64_t)(m * 2 + 0) * K_al + k) * M2 + 2 * q;
         const int64_t bo = ((int64_t)(m * 2 + 1) * K_al + k) * M2 + 2 * q;
@@ -1456,12 +1458,12 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
             R[bo + 1] = (T)-(sni - ssi);
         }
     };
-    const int c = (L - m0) / npr + 1;  // members in [-L, L]
-    const int cp = (L - r) / npr + 1;  // members in [0, L]
+    const int c = (Lk - m0) / npr + 1;  // members in [-Lk, Lk]
+    const int cp = (Lk - r) / npr + 1;  // members in [0, Lk]
     if (cp * (c - 1) <= c + cp) {      // short classes (long rings): direct sums
-        for (int m = r; m <= L; m += npr) {
+        for (int m = r; m <= Lk; m += npr) {
             double snr = 0, sni = 0, ssr = 0, ssi = 0;
-            for (int mp = m0; mp <= L; mp += npr) {
+            for (int mp = m0; mp <= Lk; mp += npr) {
                 if (mp == m) continue;
                 const double sg = ((((mp - m) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
                 double nr, ni, sr, si;
@@ -1476,7 +1478,7 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
         return;
     }
     double fnr = 0, fni = 0, fsr = 0, fsi = 0;
-    for (int mp = m0; mp <= L; mp += npr) {
+    for (int mp = m0; mp <= Lk; mp += npr) {
         const double sg = ((((mp - r) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
         double nr, ni, sr, si;
         member(mp, nr, ni, sr, si);
@@ -1485,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_alias(const T* __restrict__ G, T* __res
         fsr += sg * sr;
         fsi += sg * si;
     }
-    for (int m = r; m <= L; m += npr) {  // members m >= 0
+    for (int m = r; m <= Lk; m += npr) {  // members m >= 0
         const double sg = ((((m - r) / npr) * Rg.shift) & 1) ? -1.0 : 1.0;
         double nr, ni, sr, si;
         member(m, nr, ni, sr, si);
@@ -2150,6 +2152,8 @@ struct sfb_plan {
     std::vector<MapInfo> maps;
     std::vector<int64_t> out_base, aoff, lam_off;
     std::vector<int32_t> nb;  // per (m, par): number of l
+    std::vector<int32_t> kcut;  // per m: first ring pair whose Lam_lm matters (non-decreasing in m)
+    DevBuf d_almax;             // per aliasing ring pair k: the largest m with kcut[m] <= k (-1: none)
     int64_t a_size = 0, x_size = 0, lam_size = 0;
     // tables
     DevBuf d_rec;   // per knot (x, y, ypp, a): the adjoint's spline value and derivative
@@ -2166,7 +2170,7 @@ struct sfb_plan {
     DevBuf d_gtab, d_wB, d_winT, d_rr, d_desc_tmp, h_fa, h_fb, h_fo;  // window / field transforms
     DevBuf d_part, h_cot, h_dr, h_dw;  // adjoint partial sums and host-API staging
     int32_t K_al = 0;              // ring pairs whose rings alias (npr <= 2 lmax)
-    int32_t alias_rmax = 0;        // min(lmax + 1, the largest npr among them)
+    int32_t alias_rmax = 0;        // residues per aliasing ring pair: max over k of min(npr, almax + 1)
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
     std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
     DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
@@ -2538,11 +2542,49 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
 
     // ring pairs that alias: npr <= 2 lmax (they are the first K_al pairs)
     P->K_al = 0;
-    P->alias_rmax = 0;
     for (int32_t k = 0; k < P->K; ++k)
         if (rd[(size_t)k].npr <= 2 * L) P->K_al = k + 1;
-    for (int32_t k = 0; k < P->K_al; ++k) P->alias_rmax = std::max(P->alias_rmax, (int32_t)rd[(size_t)k].npr);
-    P->alias_rmax = std::min(P->alias_rmax, (int32_t)(L + 1));  // residues past lmax hold no m >= 0
+    // Ring cut per m.  Lam_lm(theta) falls like sin^m theta, so at high m the polar ring
+    // pairs carry nothing: kcut[m] is the first ring pair at which the largest |Lam_lm| over
+    // l (both parities) exceeds 1e-24 of its largest over all ring pairs, then made
+    // non-decreasing in m by a suffix minimum (which only lowers a cut).  The Legendre
+    // GEMMs of the analysis and of the Jacobi steps skip the pairs below the cut; their
+    // terms are below 1e-24 of the largest.  On the aliasing ring pairs this also bounds
+    // m: pair k needs G and R only for m <= almax[k] (every such m has kcut[m] <= k).
+    P->kcut.assign((size_t)(L + 1), P->K);
+    for (int64_t m = 0; m <= L; ++m) {
+        double mx = 0.0;
+        std::vector<double> rowmax((size_t)P->K, 0.0);
+        for (int par = 0; par < 2; ++par) {
+            const size_t b = (size_t)(m * 2 + par);
+            const int64_t n = P->nb[b];
+            for (int32_t k = 0; k < P->K; ++k)
+                for (int64_t j = 0; j < n; ++j)
+                    rowmax[(size_t)k] = std::max(rowmax[(size_t)k], std::fabs(lam[(size_t)(P->lam_off[b] + k * n + j)]));
+        }
+        for (int32_t k = 0; k < P->K; ++k) mx = std::max(mx, rowmax[(size_t)k]);
+        int32_t kc = P->K;
+        for (int32_t k = 0; k < P->K; ++k)
+            if (rowmax[(size_t)k] > 1e-24 * mx) {
+                kc = k;
+                break;
+            }
+        P->kcut[(size_t)m] = kc;
+    }
+    for (int64_t m = L - 1; m >= 0; --m) P->kcut[(size_t)m] = std::min(P->kcut[(size_t)m], P->kcut[(size_t)m + 1]);
+    {
+        std::vector<int32_t> almax((size_t)std::max<int32_t>(1, P->K_al), -1);
+        for (int32_t k = 0; k < P->K_al; ++k)
+            for (int64_t m = 0; m <= L; ++m)
+                if (P->kcut[(size_t)m] <= k) almax[(size_t)k] = (int32_t)m;
+        // residues per aliasing ring pair: those of m = 0..almax[k] modulo npr
+        P->alias_rmax = 0;
+        for (int32_t k = 0; k < P->K_al; ++k)
+            P->alias_rmax = std::max(P->alias_rmax, std::min((int32_t)rd[(size_t)k].npr, almax[(size_t)k] + 1));
+        if ((rc = P->d_almax.ensure(sizeof(int32_t) * almax.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_almax.p, almax.data(), sizeof(int32_t) * almax.size(), hipMemcpyHostToDevice));
+    }
+
     // I - C0 per (m, parity): C0[j][j'] = dOmega sum_rings npr lambda_j lambda_j'
     // (north and south rings give the same product within one parity)
     const double domega = 4.0 * 3.141592653589793 / (double)P->npix;
@@ -2623,43 +2665,47 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     DevBuf* a_out_buf[3] = {&P->d_a1, &P->d_a2, &P->d_a1};
     double* a_in_slot[3] = {P->d_a.as<double>(), P->d_a1.as<double>(), P->d_a2.as<double>()};
     double* a_out_slot[3] = {P->d_a1.as<double>(), P->d_a2.as<double>(), P->d_a1.as<double>()};
+    // ring pairs below kcut[m] are skipped: the analysis starts at ring pair kc, the
+    // aliasing-ring synthesis and residual analysis at kal = min(kc, K_al)
     for (int64_t b = 0; b < nbat; ++b) {
         int32_t n = P->nb[(size_t)b];
+        const int64_t kc = P->kcut[(size_t)(b / 2)];
         GemmDesc d{};
-        d.A = P->d_X0.as<double>() + b * (int64_t)P->K * P->M2;
-        d.B = P->d_lam.as<double>() + P->lam_off[(size_t)b];
+        d.A = P->d_X0.as<double>() + (b * (int64_t)P->K + kc) * P->M2;
+        d.B = P->d_lam.as<double>() + P->lam_off[(size_t)b] + kc * n;
         d.C = P->d_a.as<double>() + P->aoff[(size_t)b];
         d.Cin = nullptr;
-        d.K = P->K;
+        d.K = (int32_t)(P->K - kc);
         d.N = n;
         d.lda = P->M2;
         d.ldb = n;
         d.ldc = P->M2;
         ana.push_back(d);
         // mixed path: X0 holds floats, block b starts b K M2 floats in; a written as floats
-        d.A = (const double*)(P->d_X0.as<float>() + b * (int64_t)P->K * P->M2);
-        d.B = (const double*)(P->d_lam32.as<float>() + P->lam_off[(size_t)b]);
+        d.A = (const double*)(P->d_X0.as<float>() + (b * (int64_t)P->K + kc) * P->M2);
+        d.B = (const double*)(P->d_lam32.as<float>() + P->lam_off[(size_t)b] + kc * n);
         d.C = fa(P->d_a, (size_t)b);
         ana32.push_back(d);
     }
     for (int slot = 0; slot < 3; ++slot)
         for (int64_t b = 0; b < nbat; ++b) {
             int32_t n = P->nb[(size_t)b];
+            const int64_t kal = std::min<int64_t>(P->kcut[(size_t)(b / 2)], P->K_al);
             GemmDesc sd{};
             sd.A = a_in_slot[slot] + P->aoff[(size_t)b];
-            sd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
-            sd.C = P->d_G.as<double>() + b * (int64_t)P->K_al * P->M2;
+            sd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b] + kal;
+            sd.C = P->d_G.as<double>() + (b * (int64_t)P->K_al + kal) * P->M2;
             sd.Cin = nullptr;
             sd.K = n;
-            sd.N = (n > 0) ? P->K_al : 0;
+            sd.N = (n > 0) ? (int32_t)(P->K_al - kal) : 0;
             sd.lda = P->M2;
             sd.ldb = P->K;
             sd.ldc = P->M2;
             syn.push_back(sd);
             // mixed path: a and G hold floats
             sd.A = fa(*a_in_buf[slot], (size_t)b);
-            sd.B = (const double*)(P->d_lamT32.as<float>() + P->lam_off[(size_t)b]);
-            sd.C = (double*)(P->d_G.as<float>() + b * (int64_t)P->K_al * P->M2);
+            sd.B = (const double*)(P->d_lamT32.as<float>() + P->lam_off[(size_t)b] + kal);
+            sd.C = (double*)(P->d_G.as<float>() + (b * (int64_t)P->K_al + kal) * P->M2);
             syn32.push_back(sd);
             GemmDesc cd{};
             cd.A = a_in_slot[slot] + P->aoff[(size_t)b];
@@ -2671,21 +2717,21 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             cd.lda = P->M2;
             cd.ldb = n;
             cd.ldc = P->M2;
-            // merged: a_out = b + [(I - C0) | dOmega Lam_al] [a_in ; R] (K = n + K_al)
+            // merged: a_out = b + [(I - C0) | dOmega Lam_al] [a_in ; R] (K = n + K_al - kal)
             GemmDesc md = cd;
             md.K1 = n;
-            if (P->K_al > 0 && n > 0) {
-                md.A2 = P->d_R.as<double>() + b * (int64_t)P->K_al * P->M2;
-                md.B2 = P->d_lamw.as<double>() + P->lamw_off[(size_t)b];
-                md.K = n + P->K_al;
+            if (P->K_al > kal && n > 0) {
+                md.A2 = P->d_R.as<double>() + (b * (int64_t)P->K_al + kal) * P->M2;
+                md.B2 = P->d_lamw.as<double>() + P->lamw_off[(size_t)b] + kal * n;
+                md.K = (int32_t)(n + P->K_al - kal);
                 md.lda2 = P->M2;
                 md.ldb2 = n;
             }
             cal.push_back(md);
             // mixed path: a_in, R, b and a_out hold floats
-            if (P->K_al > 0 && n > 0) {
-                md.A2 = (const double*)(P->d_R.as<float>() + b * (int64_t)P->K_al * P->M2);
-                md.B2 = (const double*)(P->d_lamw32.as<float>() + P->lamw_off[(size_t)b]);
+            if (P->K_al > kal && n > 0) {
+                md.A2 = (const double*)(P->d_R.as<float>() + (b * (int64_t)P->K_al + kal) * P->M2);
+                md.B2 = (const double*)(P->d_lamw32.as<float>() + P->lamw_off[(size_t)b] + kal * n);
             }
             md.B = (const double*)(P->d_ic032.as<float>() + P->ic0_off[(size_t)b]);
             md.A = fa(*a_in_buf[slot], (size_t)b);
@@ -2815,7 +2861,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
-                      &P->d_desc_cal32, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
+                      &P->d_desc_cal32, &P->d_almax, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
@@ -3193,12 +3239,12 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
                 PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
                 if (f32)
                     hipLaunchKernelGGL(k_alias<float>, gr, dim3(256), 0, st, P->d_G.as<float>(),
-                                       P->d_R.as<float>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt,
-                                       rmax);
+                                       P->d_R.as<float>(), P->d_rings.as<RingDev>(), P->d_almax.as<int32_t>(), K,
+                                       Kal, L, P->M2, nq, nqt, rmax);
                 else
                     hipLaunchKernelGGL(k_alias<double>, gr, dim3(256), 0, st, P->d_G.as<double>(),
-                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), K, Kal, L, P->M2, nq, nqt,
-                                       rmax);
+                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), P->d_almax.as<int32_t>(), K,
+                                       Kal, L, P->M2, nq, nqt, rmax);
                 LAUNCH_CHECK();
             }
         }
