@@ -738,6 +738,8 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
 // bit-reversed load, then the radix-2 stages fused in pairs (one radix-4 pass of
 // 4 loads / 3 twiddles / 4 butterflies per thread item between barriers).  R is
 // the value type (double: plain path; float: mixed path).
+constexpr int PH_UL = 4;  // phi-transform sample loads in flight per thread
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
@@ -761,20 +763,29 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
         twr[e].x = (R)w.x;
         twr[e].y = (R)w.y;
     }
-    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
-        const int ql = f >> 1, side = f & 1;
-        const int q = q0 + ql;
-        R re = 0, im = 0;
-        if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
-            const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
-            const R2 v = *(const R2*)(delta + base);
-            re = v.x;
-            im = v.y;
+    // sample pairs (start at even pixels), PH_UL loads in flight per thread before their
+    // bit-reversed LDS stores
+    for (int e0 = threadIdx.x; e0 < nf * M; e0 += PH_UL * blockDim.x) {
+        R2 v[PH_UL];
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
+            const int q = q0 + (f >> 1), side = f & 1;
+            v[u].x = 0;
+            v[u].y = 0;
+            if (e < nf * M && q < nq && !(side && eq))
+                v[u] = *(const R2*)(delta + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
         }
-        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-        zs[f * Mp + fpad<R>(rj)].x = re;
-        zs[f * Mp + fpad<R>(rj)].y = im;
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < nf * M) {
+                const int f = e >> logM, jj = e & (M - 1);
+                const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+                zs[f * Mp + fpad<R>(rj)] = v[u];
+            }
+        }
     }
     __syncthreads();
     fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
@@ -1097,12 +1108,27 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         twi[e].x = (R)twl[8 * e].x;
         twi[e].y = (R)twl[8 * e].y;
     }
-    for (int e = threadIdx.x; e < 2 * TQ * npr; e += blockDim.x) {
-        const int f = e / npr, j = e - f * npr;
-        const int q = q0 + (f >> 1);
-        R v = 0;
-        if (q < nq) v = delta[(int64_t)q * npix + ((f & 1) ? Rg.start_s : Rg.start_n) + j];
-        D[f * ld + j] = v;
+    // samples in fours (npr = 4i and every polar ring starts at a multiple of 4), PH_UL
+    // loads in flight per thread before their LDS stores
+    const int nv = npr >> 2, nvt = 2 * TQ * nv;
+    for (int e0 = threadIdx.x; e0 < nvt; e0 += PH_UL * blockDim.x) {
+        R4 v[PH_UL];
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int f = e / nv, j4 = e - f * nv;
+            const int q = q0 + (f >> 1);
+            v[u].x = v[u].y = v[u].z = v[u].w = 0;
+            if (e < nvt && q < nq) v[u] = *(const R4*)(delta + (int64_t)q * npix + ((f & 1) ? Rg.start_s : Rg.start_n) + 4 * j4);
+        }
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < nvt) {
+                const int f = e / nv, j4 = e - f * nv;
+                *(R4*)(D + f * ld + 4 * j4) = v[u];
+            }
+        }
     }
     __syncthreads();
     // Y_s[u] of sequence f and frequencies (2 up, 2 up + 1) for work item e = threadIdx.x
