@@ -744,7 +744,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                  const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
-                                                 int32_t k0, int32_t logM)
+                                                 int32_t k0, int32_t logM, const int32_t* __restrict__ mall)
 {
     typedef typename Vec2T<R>::T R2;
     extern __shared__ double2 zs_raw[];
@@ -789,8 +789,10 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
     }
     __syncthreads();
     fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
-    // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M
-    const int nm = min(L, M);
+    // X_m = E_m + e^{-2 pi i m/N} O_m, E/O the DFTs of the even/odd samples, m = 0..M;
+    // stored for m <= Lk (mall: the m that the analysis reads on this ring pair)
+    const int Lk = mall ? mall[k] : L;
+    const int nm = min(Lk, M);
     for (int e = threadIdx.x; e < TQ * (nm + 1); e += blockDim.x) {
         const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X stores
         const int q = q0 + ql;
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
             xr[side] = er + (w.x * orr + w.y * oi);
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
-        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twg, L, m, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twg, Lk, m, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -824,7 +826,8 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
 __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                   int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                   const double2* __restrict__ tw, double scale,
-                                                  double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM)
+                                                  double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM,
+                                                  const int32_t* __restrict__ mall)
 {
     extern __shared__ double2 zi_raw[];
     const int nqt = (nq + TQ - 1) / TQ;
@@ -843,11 +846,12 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
     double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     for (int e = threadIdx.x; e <= N; e += blockDim.x) twr[e] = twg[e];
+    const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
         const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X loads
         const int q = q0 + ql;
         double er = 0, ei = 0, orr = 0, oi = 0;
-        if (q < nq) {
+        if (q < nq && m <= Lk) {
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             er = X[be];
             ei = X[be + 1];
@@ -912,7 +916,8 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
                                                      int64_t M2, int32_t K, int32_t L,
                                                      const RingDev* __restrict__ rings,
                                                      const double2* __restrict__ tw, double scale,
-                                                     double* __restrict__ out, int32_t TQMAX)
+                                                     double* __restrict__ out, int32_t TQMAX,
+                                                     const int32_t* __restrict__ mall)
 {
     extern __shared__ double2 ip_raw[];
     const int k = blockIdx.x;
@@ -927,6 +932,7 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     double2* Xb = twi + ni;               // [nf][npr] folded spectrum
     double2* V = Xb + nf * npr;           // [nf][4][nu]
     for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];
+    const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     // fold: bin b of map ql sums its aliasing class, both signs of m, for both rings
     // (maps fastest: a wave reads TQ consecutive maps of each X row)
     for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
@@ -936,7 +942,7 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
         if (q < nq) {
             for (int sg = 0; sg < 2; ++sg) {  // m = +b (mod npr), then m = -b (mod npr) as conj
                 const int m0 = sg ? (npr - b) % npr : b;
-                for (int m = m0; m <= L; m += npr) {
+                for (int m = m0; m <= Lk; m += npr) {
                     const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
                     const double2 ev = *(const double2*)(X + be);
                     double2 od = make_double2(0.0, 0.0);
@@ -1085,7 +1091,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta, int64_t npix, int32_t nq,
                                                     int64_t M2, int32_t K, int32_t L,
                                                     const RingDev* __restrict__ rings, const double2* __restrict__ tw,
-                                                    R* __restrict__ X, int32_t TQMAX)
+                                                    R* __restrict__ X, int32_t TQMAX, const int32_t* __restrict__ mall)
 {
     typedef typename Vec2T<R>::T R2;
     typedef typename Vec4T<R>::T R4;
@@ -1176,8 +1182,10 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         }
     }
     __syncthreads();
-    // X_kf for both rings of every map, then the folded stores (maps fastest)
-    const int nkf = min(L, npr / 2) + 1;
+    // X_kf for both rings of every map, then the folded stores (maps fastest), for
+    // m <= Lk (mall: the m that the analysis reads on this ring pair)
+    const int Lk = mall ? mall[k] : L;
+    const int nkf = min(Lk, npr / 2) + 1;
     for (int e2 = threadIdx.x; e2 < TQ * nkf; e2 += blockDim.x) {
         const int ql = e2 % TQ, kf = e2 / TQ;
         const int q = q0 + ql;
@@ -1202,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
             xr[side] = sr;
             xi[side] = si;
         }
-        store_folded<R>(X, M2, K, k, false, npr, Rg.shift, twl, L, kf, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<R>(X, M2, K, k, false, npr, Rg.shift, twl, Lk, kf, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -1607,7 +1615,8 @@ __global__ void k_unpack_cot(const double* __restrict__ cot, const int64_t* __re
 // Horner in z = e^{i phi_j}.
 __global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                    int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                   double scale, double* __restrict__ out, int32_t TQ)
+                                                   double scale, double* __restrict__ out, int32_t TQ,
+                                                   const int32_t* __restrict__ mall)
 {
     extern __shared__ double2 gs[];  // [TQ][side][L+1]
     const int k = blockIdx.x;
@@ -1616,11 +1625,12 @@ __global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X,
     const int npr = R.npr;
     const bool eq = (k == K - 1);
     const int nside = eq ? 1 : 2;
+    const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
         const int m = e / TQ, ql = e - m * TQ;
         const int q = q0 + ql;
         double er = 0, ei = 0, orr = 0, oi = 0;
-        if (q < nq) {
+        if (q < nq && m <= Lk) {
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             er = X[be];
             ei = X[be + 1];
@@ -2179,7 +2189,7 @@ struct sfb_plan {
     std::vector<int64_t> out_base, aoff, lam_off;
     std::vector<int32_t> nb;  // per (m, par): number of l
     std::vector<int32_t> kcut;  // per m: first ring pair whose Lam_lm matters (non-decreasing in m)
-    DevBuf d_almax;             // per aliasing ring pair k: the largest m with kcut[m] <= k (-1: none)
+    DevBuf d_mall;              // per ring pair k: the largest m with kcut[m] <= k (-1: none)
     int64_t a_size = 0, x_size = 0, lam_size = 0;
     // tables
     DevBuf d_rec;   // per knot (x, y, ypp, a): the adjoint's spline value and derivative
@@ -2576,7 +2586,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     // non-decreasing in m by a suffix minimum (which only lowers a cut).  The Legendre
     // GEMMs of the analysis and of the Jacobi steps skip the pairs below the cut; their
     // terms are below 1e-24 of the largest.  On the aliasing ring pairs this also bounds
-    // m: pair k needs G and R only for m <= almax[k] (every such m has kcut[m] <= k).
+    // m: pair k needs G and R only for m <= mall[k] (every such m has kcut[m] <= k), and
+    // the phi-transform need not write (nor its inverse read) X of pair k beyond mall[k].
     P->kcut.assign((size_t)(L + 1), P->K);
     for (int64_t m = 0; m <= L; ++m) {
         double mx = 0.0;
@@ -2599,16 +2610,16 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     }
     for (int64_t m = L - 1; m >= 0; --m) P->kcut[(size_t)m] = std::min(P->kcut[(size_t)m], P->kcut[(size_t)m + 1]);
     {
-        std::vector<int32_t> almax((size_t)std::max<int32_t>(1, P->K_al), -1);
-        for (int32_t k = 0; k < P->K_al; ++k)
+        std::vector<int32_t> almax((size_t)P->K, -1);
+        for (int32_t k = 0; k < P->K; ++k)
             for (int64_t m = 0; m <= L; ++m)
                 if (P->kcut[(size_t)m] <= k) almax[(size_t)k] = (int32_t)m;
         // residues per aliasing ring pair: those of m = 0..almax[k] modulo npr
         P->alias_rmax = 0;
         for (int32_t k = 0; k < P->K_al; ++k)
             P->alias_rmax = std::max(P->alias_rmax, std::min((int32_t)rd[(size_t)k].npr, almax[(size_t)k] + 1));
-        if ((rc = P->d_almax.ensure(sizeof(int32_t) * almax.size()))) return rc;
-        HIP_TRY(hipMemcpy(P->d_almax.p, almax.data(), sizeof(int32_t) * almax.size(), hipMemcpyHostToDevice));
+        if ((rc = P->d_mall.ensure(sizeof(int32_t) * almax.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_mall.p, almax.data(), sizeof(int32_t) * almax.size(), hipMemcpyHostToDevice));
     }
 
     // I - C0 per (m, parity): C0[j][j'] = dOmega sum_rings npr lambda_j lambda_j'
@@ -2765,19 +2776,20 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             md.C = fa(*a_out_buf[slot], (size_t)b);
             cal32.push_back(md);
         }
-    // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over all K ring pairs,
-    // one set per a buffer (d_a, d_a1, d_a2)
+    // full synthesis for the adjoint: X0[(m, par)][k] = a LamT over the ring pairs
+    // k >= kcut[m], one set per a buffer (d_a, d_a1, d_a2)
     std::vector<GemmDesc> synf;
     for (int slot = 0; slot < 3; ++slot)
         for (int64_t b = 0; b < nbat; ++b) {
             int32_t n = P->nb[(size_t)b];
+            const int64_t kc = P->kcut[(size_t)(b / 2)];  // ring pairs from kc on
             GemmDesc fd{};
             fd.A = a_in_slot[slot] + P->aoff[(size_t)b];
-            fd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b];
-            fd.C = P->d_X0.as<double>() + b * (int64_t)P->K * P->M2;
+            fd.B = P->d_lamT.as<double>() + P->lam_off[(size_t)b] + kc;
+            fd.C = P->d_X0.as<double>() + (b * (int64_t)P->K + kc) * P->M2;
             fd.Cin = nullptr;
             fd.K = n;
-            fd.N = (n > 0) ? P->K : 0;
+            fd.N = (n > 0) ? (int32_t)(P->K - kc) : 0;
             fd.lda = P->M2;
             fd.ldb = P->K;
             fd.ldc = P->M2;
@@ -2887,7 +2899,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
-                      &P->d_desc_cal32, &P->d_almax, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
+                      &P->d_desc_cal32, &P->d_mall, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
     for (DevBuf* b : bufs) b->release();
@@ -3179,8 +3191,11 @@ static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t n
 
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
 // for the polar ones.  f32: delta and X0 hold floats (mixed path).
-static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
+// cut: X of ring pair k written for m <= mall[k] only (what the analysis reads);
+// otherwise for every m <= lmax.
+static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool cut = true)
 {
+    const int32_t* mall = cut ? P->d_mall.as<int32_t>() : nullptr;
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int64_t nside = P->nside;
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
@@ -3193,12 +3208,12 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(256), shf, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, P->TQF)), dim3(256), shf, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQF, k_eq0, P->logM);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQF, k_eq0, P->logM, mall);
     }
     LAUNCH_CHECK();
     if (k_eq0 == 0) return SFB_OK;
@@ -3219,12 +3234,12 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32)
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), POLAR4_TQMAX);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), POLAR4_TQMAX, mall);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), POLAR4_TQMAX);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), POLAR4_TQMAX, mall);
     }
     LAUNCH_CHECK();
     return SFB_OK;
@@ -3265,11 +3280,11 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
                 PhaseTimer timer(P, SFB_PHASE_RESIDUAL, st);
                 if (f32)
                     hipLaunchKernelGGL(k_alias<float>, gr, dim3(256), 0, st, P->d_G.as<float>(),
-                                       P->d_R.as<float>(), P->d_rings.as<RingDev>(), P->d_almax.as<int32_t>(), K,
+                                       P->d_R.as<float>(), P->d_rings.as<RingDev>(), P->d_mall.as<int32_t>(), K,
                                        Kal, L, P->M2, nq, nqt, rmax);
                 else
                     hipLaunchKernelGGL(k_alias<double>, gr, dim3(256), 0, st, P->d_G.as<double>(),
-                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), P->d_almax.as<int32_t>(), K,
+                                       P->d_R.as<double>(), P->d_rings.as<RingDev>(), P->d_mall.as<int32_t>(), K,
                                        Kal, L, P->M2, nq, nqt, rmax);
                 LAUNCH_CHECK();
             }
@@ -3307,6 +3322,7 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
 // scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the radix-4 inverse
 // on the polar ring pairs, the inverse FFT on the equatorial ones (the direct sum
 // k_phi_synth wherever their blocks do not fit in LDS)
+// X0 (from the full synthesis, written for m <= mall[k] on ring pair k) -> maps
 static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
@@ -3331,7 +3347,7 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shq));
         hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)k_eq0, (unsigned)((nq + tqmin - 1) / tqmin)), dim3(256), shq,
                            st, P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX);
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX, P->d_mall.as<int32_t>());
         LAUNCH_CHECK();
     } else if (k_eq0 > 0) {
         const int32_t TQS = 8;
@@ -3339,7 +3355,7 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
         hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)k_eq0, nblk(nq, TQS)), dim3(256), shs, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), scale,
-                           P->d_delta.as<double>(), TQS);
+                           P->d_delta.as<double>(), TQS, P->d_mall.as<int32_t>());
         LAUNCH_CHECK();
     }
     if (k_eq0 < K) {
@@ -3348,7 +3364,8 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
         const int64_t units = (int64_t)(K - k_eq0) * nblk(nq, TQI);
         hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), shi, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM);
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM,
+                           P->d_mall.as<int32_t>());
         LAUNCH_CHECK();
     }
     return SFB_OK;
@@ -4078,7 +4095,7 @@ extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, 
     HIP_TRY(hipSetDevice(P->device));
     HIP_TRY(hipMemcpyAsync(P->d_delta.p, maps, sizeof(double) * (size_t)(nmap * P->npix), hipMemcpyHostToDevice,
                            P->stream));
-    int rc = phi_transform(P, (int32_t)nmap, P->stream, false);
+    int rc = phi_transform(P, (int32_t)nmap, P->stream, false, false);
     if (rc) return rc;
     // out[(b * K + k) * 2 nmap + i]
     HIP_TRY(hipMemcpy2DAsync(x_out, sizeof(double) * 2 * (size_t)nmap, P->d_X0.p, sizeof(double) * (size_t)P->M2,
