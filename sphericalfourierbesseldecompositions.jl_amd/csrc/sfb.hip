@@ -633,9 +633,10 @@ template <> struct Vec2T<float> { typedef float2 T; };
 // Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
 // every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
 // each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
-template <typename R>
+// twr[kk] = (cos, sin)(pi kk/npr) for every index (m shift) mod 2 npr that occurs.
+template <typename R, typename TW>
 __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int32_t K, int32_t k, bool eq,
-                                             int npr, int shift, const double2* __restrict__ twr, int L, int kf,
+                                             int npr, int shift, const TW* __restrict__ twr, int L, int kf,
                                              int q, R xnr, R xni, R xsr, R xsi)
 {
     const int n2 = 2 * npr;
@@ -643,7 +644,7 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
         if (fam == 1 && (kf == 0 || 2 * kf == npr)) break;
         const double cj = fam ? -1.0 : 1.0;
         for (int m = fam ? npr - kf : kf; m <= L; m += npr) {
-            const double2 ph64 = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
+            const TW ph64 = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
             const R phx = (R)ph64.x, phy = (R)ph64.y;
             const R anr = xnr, ani = (R)cj * xni, asr = xsr, asi = (R)cj * xsi;
             const R fnr = anr * phx + ani * phy, fni = ani * phx - anr * phy;
@@ -758,10 +759,21 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
     const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1, 2 or 4)
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     R2* twr = zs + nf * Mp;               // kk <= N staged in LDS
-    for (int e = threadIdx.x; e <= N; e += blockDim.x) {
-        const double2 w = twg[e];
-        twr[e].x = (R)w.x;
-        twr[e].y = (R)w.y;
+    for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
+        double2 w[PH_UL];
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e <= N) w[u] = twg[e];
+        }
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e <= N) {
+                twr[e].x = (R)w[u].x;
+                twr[e].y = (R)w[u].y;
+            }
+        }
     }
     // sample pairs (start at even pixels), PH_UL loads in flight per thread before their
     // bit-reversed LDS stores
@@ -808,7 +820,8 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
             xr[side] = er + (w.x * orr + w.y * oi);
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
-        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twg, Lk, m, q, xr[0], xi[0], xr[1], xi[1]);
+        // ring phases from the LDS table: (m' shift) mod 2N <= L <= N for every m' <= L
+        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -845,7 +858,15 @@ __global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, 
     double2* twr = zs + nf * Mp;          // N + 1 twiddles
     double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
-    for (int e = threadIdx.x; e <= N; e += blockDim.x) twr[e] = twg[e];
+    for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
+        double2 w[PH_UL];
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u)
+            if (e0 + u * (int)blockDim.x <= N) w[u] = twg[e0 + u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < PH_UL; ++u)
+            if (e0 + u * (int)blockDim.x <= N) twr[e0 + u * blockDim.x] = w[u];
+    }
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
         const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X loads
