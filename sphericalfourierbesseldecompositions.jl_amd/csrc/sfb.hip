@@ -775,29 +775,20 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
             }
         }
     }
-    // sample pairs (start at even pixels), PH_UL loads in flight per thread before their
-    // bit-reversed LDS stores
-    for (int e0 = threadIdx.x; e0 < nf * M; e0 += PH_UL * blockDim.x) {
-        R2 v[PH_UL];
-#pragma unroll
-        for (int u = 0; u < PH_UL; ++u) {
-            const int e = e0 + u * blockDim.x;
-            const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
-            const int q = q0 + (f >> 1), side = f & 1;
-            v[u].x = 0;
-            v[u].y = 0;
-            if (e < nf * M && q < nq && !(side && eq))
-                v[u] = *(const R2*)(delta + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
+    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+        const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
+        const int ql = f >> 1, side = f & 1;
+        const int q = q0 + ql;
+        R re = 0, im = 0;
+        if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
+            const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj;
+            const R2 v = *(const R2*)(delta + base);
+            re = v.x;
+            im = v.y;
         }
-#pragma unroll
-        for (int u = 0; u < PH_UL; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < nf * M) {
-                const int f = e >> logM, jj = e & (M - 1);
-                const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-                zs[f * Mp + fpad<R>(rj)] = v[u];
-            }
-        }
+        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+        zs[f * Mp + fpad<R>(rj)].x = re;
+        zs[f * Mp + fpad<R>(rj)].y = im;
     }
     __syncthreads();
     fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
