@@ -363,11 +363,13 @@ struct GroupInfo {
 // (checked against IEEE division on every distinct segment of the tables).
 // TT = float rounds t once for the FP32 evaluation of the mixed path.
 // One block: group blockIdx.y on slots [blockIdx.x * per_blk, ...): the group's knots
-// and 1/h go to LDS once, then each thread searches four slots at a time.
+// and 1/h go to LDS once, then each thread searches four slots at a time.  1024-thread
+// blocks: the LDS tables (up to 2 x 8 x 2404 bytes at kmax 0.1) then hold two blocks
+// of 16 waves per CU instead of four of 4 (0.54 -> 0.40 ms FP64 for 195 groups).
 // i is 16-bit: the sliced layout is only used when a map's records fit in LDS
 // (nk + 1 <= 5120 FP64 records).
 template <typename TT>
-__global__ __launch_bounds__(256) void k_seg_t(int64_t T, int64_t per_blk, const double* __restrict__ sr,
+__global__ __launch_bounds__(1024) void k_seg_t(int64_t T, int64_t per_blk, const double* __restrict__ sr,
                                                const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
                                                const double* __restrict__ grh, uint16_t* __restrict__ it_i,
                                                TT* __restrict__ it_t)
@@ -3138,11 +3140,11 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                    P->d_swf.as<float>());
                 LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(256), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(1024), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
                                P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
                                P->d_it_i.as<uint16_t>(), P->d_it_t.as<float>());
         } else {
-            hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(256), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+            hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(1024), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
                                P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
                                P->d_it_i.as<uint16_t>(), P->d_it_t.as<double>());
         }
