@@ -744,7 +744,7 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
 constexpr int PH_UL = 4;  // phi-transform sample loads in flight per thread
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
+__global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                  const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
                                                  int32_t k0, int32_t logM, const int32_t* __restrict__ mall)
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(256) void k_phi_fft(const R* __restrict__ delta, in
 // sequences (map, ring) per block.  1-D grid over units (ring pair, tile of TQ maps),
 // tiles fastest; workgroup b runs on XCD b mod 8, and each XCD takes runs of four
 // consecutive units, which read the same 128-byte lines of X (TQ maps x 16 B each).
-__global__ __launch_bounds__(256) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
+__global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                   int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                   const double2* __restrict__ tw, double scale,
                                                   double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM,
@@ -3215,7 +3215,9 @@ static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t n
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
 // for the polar ones.  f32: delta and X0 hold floats (mixed path).
 // cut: X of ring pair k written for m <= mall[k] only (what the analysis reads);
-// otherwise for every m <= lmax.
+// otherwise for every m <= lmax.  The FP64 FFT runs 512 threads per block (one
+// radix-4 butterfly per thread and pass at four maps: 1.65 -> 1.36 ms at nside 128),
+// the FP32 one 256 (512 measured slower).
 static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool cut = true)
 {
     const int32_t* mall = cut ? P->d_mall.as<int32_t>() : nullptr;
@@ -3234,7 +3236,7 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
                            P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(256), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(512), shf, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall);
     }
@@ -3385,7 +3387,7 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
         const size_t shi = ifft_lds(TQI);
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shi));
         const int64_t units = (int64_t)(K - k_eq0) * nblk(nq, TQI);
-        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), shi, st,
+        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(512), shi, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM,
                            P->d_mall.as<int32_t>());
