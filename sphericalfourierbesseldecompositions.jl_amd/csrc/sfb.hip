@@ -3272,12 +3272,12 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
 
 // niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a
 // (Healpix map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.
-// niter Jacobi refinements of a.  f32 (mixed path): the aliasing-ring values G and
-// their aliasing residual R are stored as floats (the synthesis accumulates in FP64
-// and rounds once; k_alias sums in FP64; the analysis of R runs on FP32 MFMA with an
-// FP64 epilogue); a, b and the dense C0 step stay FP64.  They only carry the aliasing
-// correction, so their FP32 rounding stays far below the mixed path's FP32 scatter
-// and phi-transform; the bytes they move halve.
+// niter Jacobi refinements of a.  f32 (mixed path): the iterate a (and b), the
+// aliasing-ring values G and their aliasing residual R are stored as floats; every
+// GEMM runs on FP32 MFMA with FP32 tables and an FP64 epilogue (rounded once on the
+// store), and k_alias sums in FP64.  Their FP32 rounding stays far below that of the
+// mixed path's FP32 scatter and phi-transform; the bytes they move halve.  Ring pairs
+// below kcut[m] are skipped by the descriptors, and k_alias works on m <= mall[k].
 static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
