@@ -948,8 +948,11 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     // fold: bin b of map ql sums its aliasing class, both signs of m, for both rings
-    // (maps fastest: a wave reads TQ consecutive maps of each X row)
-    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
+    // (maps fastest: a wave reads TQ consecutive maps of each X row).  The spectrum of a
+    // real ring is Hermitian, X_{npr-b} = conj(X_b): bins b <= npr/2 are summed and
+    // mirrored, so each X value is read once rather than once per sign.
+    const int nbh = npr / 2 + 1;
+    for (int e = threadIdx.x; e < TQ * nbh; e += blockDim.x) {
         const int ql = e % TQ, b = e / TQ;
         const int q = q0 + ql;
         double nr = 0.0, ni_ = 0.0, sr = 0.0, si = 0.0;
@@ -976,6 +979,10 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
         }
         Xb[(2 * ql) * npr + b] = make_double2(nr, ni_);
         Xb[(2 * ql + 1) * npr + b] = make_double2(sr, si);
+        if (b > 0 && 2 * b < npr) {
+            Xb[(2 * ql) * npr + npr - b] = make_double2(nr, -ni_);
+            Xb[(2 * ql + 1) * npr + npr - b] = make_double2(sr, -si);
+        }
     }
     __syncthreads();
     // radix-4 combination, halved at u = 0 and u = i/2
