@@ -444,28 +444,26 @@ template <typename R> struct Vec4T;
 template <> struct Vec4T<double> { typedef double4 T; };
 template <> struct Vec4T<float> { typedef float4 T; };
 
-// Records in LDS.  An FP64 record is two 16-byte halves; stored in order, every
-// record's first half would sit in an even 16-byte bank group, so each of the two
-// reads of a random gather would spread over half the bank groups.  Odd records
-// are stored with their halves swapped: the read of (d, c) at 32 i + 16 (i & 1)
-// then lands on every bank group alike, and so does the read of (b, a).
-// FP32 records are one 16-byte read and stay in order.
-__device__ __forceinline__ void lds_rec_put(double4* sm, int k, double4 v)
+// Records in LDS.  An FP64 record is two 16-byte halves, kept in two dense arrays of a
+// map's table of ts records: (d, c) of record i at 16 i, (b, a) at 16 (ts + i).  Either
+// array read at random records spreads over every bank group (as an in-order array of
+// 32-byte records would not: its first halves all sit in even 16-byte groups), and a
+// gather's two addresses are 16 i plus a per-table constant (a shift and two adds per
+// map).  FP32 records are one 16-byte read and stay in order.
+__device__ __forceinline__ void lds_rec_put(double4* sm, int ts, int k, double4 v)
 {
     double2* s2 = (double2*)sm;
-    const int a = 2 * k + (k & 1);
-    s2[a] = make_double2(v.x, v.y);
-    s2[a ^ 1] = make_double2(v.z, v.w);
+    s2[k] = make_double2(v.x, v.y);
+    s2[ts + k] = make_double2(v.z, v.w);
 }
-__device__ __forceinline__ double4 lds_rec_get(const double4* sm, int i)
+__device__ __forceinline__ double4 lds_rec_get(const double4* sm, int ts, int i)
 {
     const double2* s2 = (const double2*)sm;
-    const int a = 2 * i + (i & 1);
-    const double2 lo = s2[a], hi = s2[a ^ 1];
+    const double2 lo = s2[i], hi = s2[ts + i];
     return make_double4(lo.x, lo.y, hi.x, hi.y);
 }
-__device__ __forceinline__ void lds_rec_put(float4* sm, int k, float4 v) { sm[k] = v; }
-__device__ __forceinline__ float4 lds_rec_get(const float4* sm, int i) { return sm[i]; }
+__device__ __forceinline__ void lds_rec_put(float4* sm, int, int k, float4 v) { sm[k] = v; }
+__device__ __forceinline__ float4 lds_rec_get(const float4* sm, int, int i) { return sm[i]; }
 
 // Galaxy rows of the sliced layout: row k holds slots 64 k .. 64 k + 63, one per
 // pixel (lane) of its slice, so slice s covers rows soff[s]/64 .. soff[s+1]/64 - 1.
@@ -476,17 +474,34 @@ template <bool UNIT, typename R> struct RowLoad {
     R t[4], w[4];
 };
 
+// Row and slot indices are 32-bit (the host keeps 8 T below 2^32): the slot arrays are
+// read as raw buffers (a wave-uniform descriptor and a 32-bit per-lane byte offset), which
+// spares the 64-bit address arithmetic of every load.
+constexpr int BUF_DWORD3 = 0x00020000;  // buffer descriptor word 3 for gfx9-family parts
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const void* p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xffffffff, BUF_DWORD3);
+}
+template <typename R> __device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t o);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t o)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, o * 8u, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t o)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o * 4u, 0, 0));
+}
+
 template <bool UNIT, typename R>
-__device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, const uint16_t* __restrict__ gi,
-                                          const R* __restrict__ gt, const R* __restrict__ sw, int64_t r,
-                                          int64_t rlast, int lane)
+__device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_rsrc_t gi, __amdgpu_buffer_rsrc_t gt,
+                                          __amdgpu_buffer_rsrc_t sw, int r, int rlast, int lane)
 {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const int64_t o = min(r + u, rlast) * 64 + lane;  // rows past the run re-read its last row
-        L.i[u] = gi[o];
-        L.t[u] = gt[o];
-        if (!UNIT) L.w[u] = sw[o];
+        const uint32_t o = (uint32_t)(min(r + u, rlast) * 64 + lane);  // rows past the run re-read its last row
+        L.i[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(gi, o * 2u, 0, 0);
+        L.t[u] = buf_load<R>(gt, o);
+        if (!UNIT) L.w[u] = buf_load<R>(sw, o);
     }
 }
 
@@ -532,13 +547,14 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         const MapInfo m = maps[q];
         if (k == 0) pad = m.pad;
         if (has[k])
-            for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) lds_rec_put(sm4 + k * tstride, j, rec4[m.koff + q + j]);
+            for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) lds_rec_put(sm4 + k * tstride, tstride, j, rec4[m.koff + q + j]);
         tab[k] = sm4 + (has[k] ? k : 0) * tstride;
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
     }
-    const uint16_t* gi = it_i + (int64_t)pad * T;
-    const R* gt = it_t + (int64_t)pad * T;
+    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * T);
+    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * T);
+    const __amdgpu_buffer_rsrc_t gw = slot_rsrc(sw);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int c0 = B.chunk * spc, c1 = min(nslice, c0 + spc);
     const int per_w = (c1 - c0 + nwave - 1) / nwave;
@@ -568,17 +584,17 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
 #pragma unroll
     for (int k = 0; k < NM; ++k) acc[k] = 0;
 
-    const int64_t rlast = (soff[s_end] >> 6) - 1;
-    int64_t r = soff[s] >> 6;             // next row to sum
-    int64_t first = r;                    // first row of slice s
-    int64_t bound = soff[s + 1] >> 6;     // first row after slice s
-    int64_t nbound = soff[min(s + 2, nslice)] >> 6;  // first row after slice s + 1
+    const int rlast = (int)(soff[s_end] >> 6) - 1;
+    int r = (int)(soff[s] >> 6);                       // next row to sum
+    int first = r;                                     // first row of slice s
+    int bound = (int)(soff[s + 1] >> 6);               // first row after slice s
+    int nbound = (int)(soff[min(s + 2, nslice)] >> 6);  // first row after slice s + 1
     SliceSt cs, ns;
     load_st(s, cs);
     load_dense_w(cs);
     load_st(s + 1, ns);
     // slices that end at row rr (empty ones included) are written; s moves on
-    auto close_slices = [&](int64_t rr) {
+    auto close_slices = [&](int rr) {
         while (bound == rr && s < s_end) {
             if ((int64_t)s * 64 + lane < npix) {
 #pragma unroll
@@ -596,21 +612,21 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
             load_st(s + 1, ns);
             first = bound;
             bound = nbound;
-            nbound = soff[min(s + 2, nslice)] >> 6;
+            nbound = (int)(soff[min(s + 2, nslice)] >> 6);
         }
     };
     close_slices(r);
     if (r > rlast) return;
 
     RowLoad<UNIT, R> cur, nxt;
-    load_rows<UNIT, R>(cur, gi, gt, sw, r, rlast, lane);
+    load_rows<UNIT, R>(cur, gi, gt, gw, r, rlast, lane);
     for (; r <= rlast; r += 4) {
-        load_rows<UNIT, R>(nxt, gi, gt, sw, r + 4, rlast, lane);
+        load_rows<UNIT, R>(nxt, gi, gt, gw, r + 4, rlast, lane);
         R4 c[4][NM];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int k = 0; k < NM; ++k) c[u][k] = lds_rec_get(tab[k], cur.i[u]);  // (d, c, b, a)
+            for (int k = 0; k < NM; ++k) c[u][k] = lds_rec_get(tab[k], tstride, cur.i[u]);  // (d, c, b, a)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
@@ -2243,6 +2259,7 @@ struct sfb_plan {
     int64_t ncat = 0;
     // sliced (SELL) catalogue layout; sell_ok when a map's records fit in LDS
     bool sell_ok = false, sell_empty = false;
+    bool sell_fits = true;  // 8 x slots < 2^32: the sliced scatter's 32-bit slot offsets hold
     int32_t sell_nslice = 0;
     int64_t sell_total = 0;
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
@@ -3053,6 +3070,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
                                hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         P->sell_total = total;
+        P->sell_fits = total < ((int64_t)1 << 29);
         if ((rc = P->d_sr.ensure(8 * (size_t)std::max<int64_t>(total, 1))) ||
             (rc = P->d_sw.ensure(8 * (size_t)std::max<int64_t>(total, 1))))
             return rc;
@@ -3124,7 +3142,7 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 {
     double domega = 4.0 * kPi / (double)P->npix;
     double inv = 1.0 / (nbar * domega);
-    if (P->sell_ok && !P->sell_empty) {
+    if (P->sell_ok && !P->sell_empty && P->sell_fits) {
         // shared segment search per knot group, then three fma per map and galaxy
         const int64_t T = std::max<int64_t>(P->sell_total, 1);
         int rc;
