@@ -272,7 +272,8 @@ __global__ void k_sell_keys(int64_t npix, const int32_t* __restrict__ pstart, ui
 // its (pixel, galaxy count) and the slice width 64 * (largest count in the slice).
 __global__ __launch_bounds__(256) void k_sell_slices(int64_t npix, int32_t nslice, const uint32_t* __restrict__ perm,
                                                      const int32_t* __restrict__ pstart, int64_t* __restrict__ width,
-                                                     uint32_t* __restrict__ invperm, int2* __restrict__ pinfo)
+                                                     uint32_t* __restrict__ invperm, int2* __restrict__ pinfo,
+                                                     int32_t* __restrict__ smin)
 {
     const int32_t s = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
@@ -287,9 +288,16 @@ __global__ __launch_bounds__(256) void k_sell_slices(int64_t npix, int32_t nslic
     } else {  // lanes past the last pixel: no pixel, no galaxies
         pinfo[e] = make_int2(0, 0);
     }
+    int32_t cmin = c;  // the fullest and the emptiest lane of the slice
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, 64));
-    if (lane == 0) width[s] = 64 * (int64_t)c;
+    for (int o = 32; o > 0; o >>= 1) {
+        c = max(c, __shfl_xor(c, o, 64));
+        cmin = min(cmin, __shfl_xor(cmin, o, 64));
+    }
+    if (lane == 0) {
+        width[s] = 64 * (int64_t)c;
+        smin[s] = cmin;
+    }
 }
 
 // separable window: the pixel factor of every slice lane, win_a[pixel] (0 past npix)
@@ -478,30 +486,34 @@ template <bool UNIT, typename R> struct RowLoad {
 // read as raw buffers (a wave-uniform descriptor and a 32-bit per-lane byte offset), which
 // spares the 64-bit address arithmetic of every load.
 constexpr int BUF_DWORD3 = 0x00020000;  // buffer descriptor word 3 for gfx9-family parts
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const void* p)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const void* p, uint32_t bytes)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xffffffff, BUF_DWORD3);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, BUF_DWORD3);
 }
-template <typename R> __device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t o);
-template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t o)
+// element o + 64 u: per-lane byte offset plus the row step as the scalar offset
+template <typename R> __device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t o, int u);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t o, int u)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, o * 8u, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, o * 8u, 512 * u, 0));
 }
-template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t o)
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t o, int u)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o * 4u, 0, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o * 4u, 256 * u, 0));
 }
 
+// Rows r .. r + 3: one per-lane offset and the row steps as immediate offsets.  Rows past
+// the wave's run read its neighbours' slots (not used); past the last slot the buffer's
+// range check returns zeros.
 template <bool UNIT, typename R>
 __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_rsrc_t gi, __amdgpu_buffer_rsrc_t gt,
-                                          __amdgpu_buffer_rsrc_t sw, int r, int rlast, int lane)
+                                          __amdgpu_buffer_rsrc_t sw, int r, int lane)
 {
+    const uint32_t o = (uint32_t)(r * 64 + lane);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const uint32_t o = (uint32_t)(min(r + u, rlast) * 64 + lane);  // rows past the run re-read its last row
-        L.i[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(gi, o * 2u, 0, 0);
-        L.t[u] = buf_load<R>(gt, o);
-        if (!UNIT) L.w[u] = buf_load<R>(sw, o);
+        L.i[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(gi, o * 2u, 128 * u, 0);
+        L.t[u] = buf_load<R>(gt, o, u);
+        if (!UNIT) L.w[u] = buf_load<R>(sw, o, u);
     }
 }
 
@@ -521,6 +533,7 @@ template <bool DENSE, int NM, bool UNIT, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
                                                      int32_t tstride, const int2* __restrict__ pinfo,
+                                                     const int32_t* __restrict__ smin,
                                                      const double* __restrict__ pwin,
                                                      const int64_t* __restrict__ soff, const R* __restrict__ sw,
                                                      const uint16_t* __restrict__ it_i, const R* __restrict__ it_t,
@@ -552,10 +565,12 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
     }
-    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * T);
-    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * T);
-    const __amdgpu_buffer_rsrc_t gw = slot_rsrc(sw);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * T, (uint32_t)(2 * T));
+    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * T, (uint32_t)(sizeof(R) * T));
+    const __amdgpu_buffer_rsrc_t gw = slot_rsrc(sw, (uint32_t)(sizeof(R) * T));
+    // wave-uniform slice bookkeeping (scalar registers and loads)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nwave = blockDim.x >> 6;
     const int c0 = B.chunk * spc, c1 = min(nslice, c0 + spc);
     const int per_w = (c1 - c0 + nwave - 1) / nwave;
     const int s_end = min(c1, c0 + (wave + 1) * per_w);
@@ -567,11 +582,13 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     // slice ahead so that closing a slice only stores.
     struct SliceSt {
         int2 pc;
+        int minc;  // the slice's smallest lane count: rows below it need no padding mask
         double w[NM];
     };
     auto load_st = [&](int sl, SliceSt& st) {
         const int64_t e = (int64_t)min(sl, nslice - 1) * 64 + lane;
         st.pc = pinfo[e];
+        st.minc = __builtin_amdgcn_readfirstlane(smin[min(sl, nslice - 1)]);
 #pragma unroll
         for (int k = 0; k < NM; ++k) st.w[k] = DENSE ? 0.0 : pwin[e];
     };
@@ -619,9 +636,9 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     if (r > rlast) return;
 
     RowLoad<UNIT, R> cur, nxt;
-    load_rows<UNIT, R>(cur, gi, gt, gw, r, rlast, lane);
+    load_rows<UNIT, R>(cur, gi, gt, gw, r, lane);
     for (; r <= rlast; r += 4) {
-        load_rows<UNIT, R>(nxt, gi, gt, gw, r + 4, rlast, lane);
+        load_rows<UNIT, R>(nxt, gi, gt, gw, r + 4, lane);
         R4 c[4][NM];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -631,13 +648,16 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
             const R t = cur.t[u];
-            const bool live = !UNIT || r + u - first < cs.pc.y;
+            R v[NM];
 #pragma unroll
-            for (int k = 0; k < NM; ++k) {
-                const R v = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
-                if (UNIT) acc[k] = acc[k] + (live ? v : (R)0);
-                else acc[k] = acc[k] + cur.w[u] * v;
+            for (int k = 0; k < NM; ++k) v[k] = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
+            if (UNIT && r + u - first >= cs.minc) {  // wave-uniform: some lane is past its galaxies
+                const bool live = r + u - first < cs.pc.y;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) v[k] = live ? v[k] : (R)0;
             }
+#pragma unroll
+            for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + v[k] : acc[k] + cur.w[u] * v[k];
             close_slices(r + u + 1);
         }
         cur = nxt;
@@ -2263,6 +2283,7 @@ struct sfb_plan {
     int32_t sell_nslice = 0;
     int64_t sell_total = 0;
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
+    DevBuf d_smin;  // per slice: the smallest galaxy count of its lanes
     DevBuf d_pinfo, d_pwin;  // per slice lane: (pixel, count), separable window factor
     DevBuf d_vgm, d_vgs, d_vrange;  // adjoint gather: group-ordered maps, group starts, map ranges
     int32_t nvrange = 0;
@@ -2942,7 +2963,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_pix, &P->d_pixkey, &P->d_pixkey2, &P->d_rs, &P->d_ws, &P->d_pstart, &P->d_sorttmp,
                       &P->h_rtp, &P->h_w, &P->h_wa, &P->h_wb, &P->h_out,
                       &P->d_desc_synf, &P->d_part, &P->h_cot, &P->h_dr, &P->h_dw,
-                      &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_soff,
+                      &P->d_skey, &P->d_skey2, &P->d_sval, &P->d_perm, &P->d_invperm, &P->d_swidth, &P->d_smin, &P->d_soff,
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
@@ -3023,6 +3044,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
             (rc = P->d_invperm.ensure(4 * (size_t)P->npix)) ||
             (rc = P->d_pinfo.ensure(sizeof(int2) * 64 * (size_t)P->sell_nslice)) ||
             (rc = P->d_swidth.ensure(8 * (size_t)P->sell_nslice)) ||
+            (rc = P->d_smin.ensure(4 * (size_t)P->sell_nslice)) ||
             (rc = P->d_soff.ensure(8 * (size_t)(P->sell_nslice + 1))))
             return rc;
         HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp4, P->d_skey.as<uint32_t>(), P->d_skey2.as<uint32_t>(),
@@ -3060,7 +3082,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         hipLaunchKernelGGL(k_sell_slices, dim3(nblk(64 * (int64_t)P->sell_nslice, 256)), dim3(256), 0, st, P->npix,
                            P->sell_nslice,
                            P->d_perm.as<uint32_t>(), P->d_pstart.as<int32_t>(), P->d_swidth.as<int64_t>(),
-                           P->d_invperm.as<uint32_t>(), P->d_pinfo.as<int2>());
+                           P->d_invperm.as<uint32_t>(), P->d_pinfo.as<int2>(), P->d_smin.as<int32_t>());
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, P->sell_nslice, P->d_swidth.as<int64_t>(),
                            P->d_soff.as<int64_t>());
@@ -3102,7 +3124,8 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
     auto go = [&](auto kern) -> int {
         HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T, blk, per_xcd, tstride,
-                           P->d_pinfo.as<int2>(), P->d_pwin.as<double>(), P->d_soff.as<int64_t>(), sw,
+                           P->d_pinfo.as<int2>(), P->d_smin.as<int32_t>(), P->d_pwin.as<double>(),
+                           P->d_soff.as<int64_t>(), sw,
                            P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
                            d_wb, P->lmax, P->d_delta.as<R>());
         LAUNCH_CHECK();
