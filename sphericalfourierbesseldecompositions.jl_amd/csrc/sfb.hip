@@ -1817,7 +1817,7 @@ __global__ __launch_bounds__(1024) void k_vjp_gather(int64_t N, int64_t npix, co
 // and g = d + t(c + t(b + t a)) (the forward's value, bit for bit) and
 // g' = (c + t(2b + 3at)) / h, as the product with the tabulated 1/h (0 on the boundary
 // pseudo-records, the constant extrapolation).  Galaxies in pixel order.
-constexpr int VJ_GPT = 8;
+constexpr int VJ_GPT = 6;  // galaxies per thread: 8 spilled once the early gathers held their values
 __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_t cap, const double* __restrict__ r_s,
                                                  const uint32_t* __restrict__ pix_s,
                                                  const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
@@ -1885,13 +1885,18 @@ __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_
             ii[u] = i;
             t[u] = tt;
         }
+        // Per map: its delta-bar gathers are issued first (they do not depend on LDS), so
+        // they are in flight while the map's records are staged.
         for (int j = vgs[g]; j < vgs[g + 1]; ++j) {
             const int q = vgm[j];
             const MapInfo m = maps[q];
+            const double* dq = dbar + (int64_t)q * npix;
+            double dd[VJ_GPT];
+#pragma unroll
+            for (int u = 0; u < VJ_GPT; ++u) dd[u] = (k0 + 1024 * u < N) ? dq[pp[u]] : 0.0;
             __syncthreads();  // the previous map's records are no longer read
             for (int e = threadIdx.x; e <= m.nk; e += blockDim.x) tb[e] = rec4[m.koff + q + e];
             __syncthreads();
-            const double* dq = dbar + (int64_t)q * npix;
 #pragma unroll
             for (int u = 0; u < VJ_GPT; ++u) {
                 if (k0 + 1024 * u < N) {
@@ -1899,9 +1904,8 @@ __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_
                     const double tt = t[u];
                     const double v = fma(tt, fma(tt, fma(tt, c.w, c.z), c.y), c.x);
                     const double dv = ii[u] < G.nk - 1 ? fma(tt, fma(tt, 3.0 * c.w, 2.0 * c.z), c.y) * rh[ii[u]] : 0.0;
-                    const double d = dq[pp[u]];
-                    aw[u] = fma(v, d, aw[u]);
-                    ar[u] = fma(dv, d, ar[u]);
+                    aw[u] = fma(v, dd[u], aw[u]);
+                    ar[u] = fma(dv, dd[u], ar[u]);
                 }
             }
         }
