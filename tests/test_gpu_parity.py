@@ -94,6 +94,33 @@ def test_delta_map_crowded_pixels_bitexact():
             assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
 
 
+def test_delta_map_large_tables_per_pixel_bitexact():
+    """Knot tables too large for LDS (6000 knots: the sliced scatter needs (nk + 1) x 32 bytes
+    of records per map in LDS, at most 160 KB) take the per-pixel scatter kernel; its delta
+    maps are bit-exact too, with and without weights, radii inside and outside the knots."""
+    nk, nside, lmax = 6000, 8, 3
+    x = np.linspace(500.0, 1000.0, nk)
+    scales = 20.0 + 5.0 * np.arange(lmax + 1)
+    ys = [np.sin(x / s) for s in scales]
+    ypps = [-y / s ** 2 for y, s in zip(ys, scales)]
+    am = S.AnlmModes.from_tables(500.0, 1000.0, nside, np.array([lmax]), np.arange(lmax + 2) * nk,
+                                 np.tile(x, lmax + 1), np.concatenate(ys), np.concatenate(ypps))
+    rng = np.random.default_rng(31)
+    win = S.SeparableArray(rng.uniform(0.5, 1.0, 12 * nside * nside), rng.uniform(0.1, 1.0, (lmax + 1, 1)),
+                           "mask", "w_ln")
+    cat = uniform_shell_catalog(20000, 450.0, 1050.0, seed=32)
+    w = rng.uniform(0.5, 1.5, cat.shape[1])
+    nbar = cat.shape[1] / shell_volume(450.0, 1050.0)
+    plan = S.Plan(am, niter=3)
+    om = _oracle_modes_from(am)
+    for q in range(lmax + 1):
+        for ww in (w, None):
+            say("large-table delta map", q, "weights" if ww is not None else "unit")
+            g = plan.delta_map(cat, nbar, win, ww, q)
+            c = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), ww, q)
+            assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
+
+
 def test_delta_map_weights_dense_bitexact(c1):
     am, win, cat, nbar = c1
     rng = np.random.default_rng(5)
