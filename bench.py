@@ -130,7 +130,9 @@ def main():
     t0 = time.perf_counter()
     plan = S.Plan(am, niter=niter, precision="fp64")
     t_plan = time.perf_counter() - t0
-    log(f"plan: {t_plan:.2f}s, window: {t_win:.2f}s")
+    tiles, tune_ms = plan.tiles()
+    tiles["search_ms"] = tune_ms
+    log(f"plan: {t_plan:.2f}s (phi tile search {tune_ms:.0f} ms: {tiles}), window: {t_win:.2f}s")
 
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
@@ -284,6 +286,7 @@ def main():
         "errors_vs_oracle": errs,
         "phases_ms": {k: v[0] / max(calls, 1) for k, v in phases.items()},
         "setup_s": {"AnlmModes": t_modes, "plan": t_plan},
+        "phi_tiles": tiles,
         "adjoint_fp64_ms": adj_dt * 1e3,
         "batched_mocks": batch,
         "roofline": roof,

@@ -27,8 +27,10 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -2247,6 +2249,9 @@ struct sfb_plan {
     int64_t M2 = 0;      // 2 * nmaps
     int32_t niter = 3, precision = 0, TQ = 8;
     int32_t TQF = 1, TQF32 = 2, TQP = 2, logM = 1;  // phi-transform tiles (FFT FP64 / FP32, polar)
+    int32_t fft_nt = 512, fft_nt32 = 256;           // FFT threads per block (FP64 / FP32)
+    int32_t p4_tq = 64, p4_tq32 = 64;               // polar maps per block, at most (FP64 / FP32)
+    double tune_ms = 0.0;                           // time the plan-time tile search took
     std::vector<MapInfo> maps;
     std::vector<int64_t> out_base, aoff, lam_off;
     std::vector<int32_t> nb;  // per (m, par): number of l
@@ -2879,6 +2884,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
 
 extern "C" const char* sfb_version(void) { return "sfb_mi355x 0.1 (gfx950)"; }
 
+static int autotune_phi(sfb_plan* P);  // plan-time phi-transform tile search (defined below)
+
 extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int64_t nmax, const int64_t* lmax_n,
                                const int64_t* knot_off, const double* knot_x, const double* knot_y,
                                const double* knot_ypp, int32_t niter, int32_t precision, int32_t device)
@@ -2947,6 +2954,7 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         return fail(SFB_ERR_DEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     int rc = plan_tables(P, lmax_n, knot_off, knot_x, knot_y, knot_ypp);
+    if (!rc) rc = autotune_phi(P);
     if (rc) {
         sfb_plan_destroy(P);
         return rc;
@@ -3245,17 +3253,17 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 // k_phi_polar4 launch shape: maps per block up to TQMAX (each ring takes
 // min(TQMAX, 256 / (2 nup)) maps), the LDS of the largest block over the rings, and
 // the grid height for nq maps (set by the ring with the fewest maps per block)
-constexpr int32_t POLAR4_TQMAX = 64;
 static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t nq = 1, int32_t* gy = nullptr)
 {
+    const int32_t tqmax = vbytes == 8 ? P->p4_tq : P->p4_tq32;  // maps per block, at most
     const int64_t npr_max = 4 * (P->nside - 1);
     const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
     if (2 * nup_max > 256) return false;
     size_t sh = 0;
-    int64_t tqmin = POLAR4_TQMAX;
+    int64_t tqmin = tqmax;
     for (int64_t i = 1; i < P->nside; ++i) {
         const int64_t nup = ((i / 2 + 1) + 1) / 2;
-        const int64_t tq = std::min<int64_t>(POLAR4_TQMAX, 256 / (2 * nup));
+        const int64_t tq = std::min<int64_t>(tqmax, 256 / (2 * nup));
         sh = std::max(sh, 2 * vbytes * (size_t)(i + 2) + vbytes * 2 * (size_t)tq * (size_t)(4 * i + 8));
         tqmin = std::min(tqmin, tq);
     }
@@ -3267,9 +3275,8 @@ static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t n
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
 // for the polar ones.  f32: delta and X0 hold floats (mixed path).
 // cut: X of ring pair k written for m <= mall[k] only (what the analysis reads);
-// otherwise for every m <= lmax.  The FP64 FFT runs 512 threads per block (one
-// radix-4 butterfly per thread and pass at four maps: 1.65 -> 1.36 ms at nside 128),
-// the FP32 one 256 (512 measured slower).
+// otherwise for every m <= lmax.  Tile shapes (maps and threads per FFT block, maps per
+// polar block) come from the plan's tile search (autotune_phi); they change no arithmetic.
 static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool cut = true)
 {
     const int32_t* mall = cut ? P->d_mall.as<int32_t>() : nullptr;
@@ -3283,12 +3290,12 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(256), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(P->fft_nt32), shf, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(512), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(P->fft_nt), shf, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall);
     }
@@ -3311,12 +3318,12 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), POLAR4_TQMAX, mall);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->p4_tq32, mall);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), POLAR4_TQMAX, mall);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->p4_tq, mall);
     }
     LAUNCH_CHECK();
     return SFB_OK;
@@ -3324,6 +3331,93 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
 
 // niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a
 // (Healpix map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.
+// Plan-time tile search for the phi-transform.  Candidates: maps per FFT block (powers of
+// two whose LDS fits) x 256 / 512 threads, for FP64 and for FP32, then up to 32 / 64 maps per
+// polar block.  Each runs the whole transform over every map of the plan's (zeroed) map
+// buffer, once to warm up and three times timed on the plan's stream; the fastest stays.
+// Tile shapes change no arithmetic (each ring sequence is transformed alike), so the search
+// moves only the time.  SFB_TUNE=0 in the environment keeps the fixed defaults.
+static int autotune_phi(sfb_plan* P)
+{
+    const char* env = std::getenv("SFB_TUNE");
+    if (env && env[0] == '0') return SFB_OK;
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t st = P->stream;
+    const int32_t nq = (int32_t)P->nmaps;
+    const int64_t nd = P->nmaps * P->npix;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nd, 256)), dim3(256), 0, st, P->d_delta.as<double>(), nd, 0.0);
+    LAUNCH_CHECK();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(SFB_ERR_DEVICE, "hipEventCreate failed");
+    }
+    int rc = SFB_OK;
+    auto timed = [&](bool f32, float* best) -> int {  // min over 3 runs, after one warm-up
+        int r;
+        if ((r = phi_transform(P, nq, st, f32))) return r;
+        *best = 1e30f;
+        for (int it = 0; it < 3; ++it) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, st) != hipSuccess) return fail(SFB_ERR_DEVICE, "hipEventRecord failed");
+            if ((r = phi_transform(P, nq, st, f32))) return r;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                return fail(SFB_ERR_DEVICE, "tile search timing failed");
+            *best = std::min(*best, ms);
+        }
+        return SFB_OK;
+    };
+    const int64_t M = 2 * P->nside;
+    for (int f = 0; f < 2 && !rc; ++f) {
+        const size_t vb = f ? 4 : 8;
+        int32_t& tq = f ? P->TQF32 : P->TQF;
+        int32_t& nt = f ? P->fft_nt32 : P->fft_nt;
+        int32_t& p4 = f ? P->p4_tq32 : P->p4_tq;
+        // FFT tiles (the polar kernel at its default meanwhile)
+        int32_t btq = tq, bnt = nt;
+        float bms = 1e30f;
+        for (int32_t cq = 1; cq <= 8 && !rc; cq *= 2) {
+            const size_t sh = 2 * vb * (2 * (size_t)cq * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
+            if ((int64_t)sh > P->lds_max) break;
+            for (int32_t cn = 256; cn <= 512 && !rc; cn *= 2) {
+                tq = cq;
+                nt = cn;
+                float ms = 0.0f;
+                if ((rc = timed(f == 1, &ms))) break;
+                if (ms < bms) {
+                    bms = ms;
+                    btq = cq;
+                    bnt = cn;
+                }
+            }
+        }
+        tq = btq;
+        nt = bnt;
+        // polar tiles
+        int32_t bp = p4;
+        bms = 1e30f;
+        for (int32_t cp : {32, 64}) {
+            if (rc) break;
+            p4 = cp;
+            size_t shp;
+            if (!polar4_fits(P, vb, &shp)) continue;
+            float ms = 0.0f;
+            if ((rc = timed(f == 1, &ms))) break;
+            if (ms < bms) {
+                bms = ms;
+                bp = cp;
+            }
+        }
+        p4 = bp;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return rc;
+}
+
 // niter Jacobi refinements of a.  f32 (mixed path): the iterate a (and b), the
 // aliasing-ring values G and their aliasing residual R are stored as floats; every
 // GEMM runs on FP32 MFMA with FP32 tables and an FP64 epilogue (rounded once on the
@@ -4179,6 +4273,19 @@ extern "C" int sfb_phi_transform(sfb_plan* P, int64_t nmap, const double* maps, 
                              sizeof(double) * 2 * (size_t)nmap, (size_t)(2 * (P->lmax + 1) * P->K),
                              hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
+    return SFB_OK;
+}
+
+extern "C" int sfb_plan_tiles(const sfb_plan* P, int32_t* tiles, double* tune_ms)
+{
+    if (!P || !tiles) return fail(SFB_ERR_ARG, "plan or tiles is NULL");
+    tiles[0] = P->TQF;
+    tiles[1] = P->fft_nt;
+    tiles[2] = P->TQF32;
+    tiles[3] = P->fft_nt32;
+    tiles[4] = P->p4_tq;
+    tiles[5] = P->p4_tq32;
+    if (tune_ms) *tune_ms = P->tune_ms;
     return SFB_OK;
 }
 

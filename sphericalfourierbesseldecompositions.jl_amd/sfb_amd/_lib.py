@@ -66,6 +66,7 @@ SIGNATURES = {
     "sfb_phi_transform": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_plan_profile": (ctypes.c_int, [_P, _i32]),
     "sfb_plan_phase_times": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sfb_plan_tiles": (ctypes.c_int, [_P, _P, _P]),
 }
 
 PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment"]

@@ -62,6 +62,15 @@ class Plan:
         L.check(L.lib().sfb_plan_phase_times(self._h, L.ptr(ms), L.ptr(n), ctypes.byref(calls)))
         return {ph: (float(ms[i]), int(n[i])) for i, ph in enumerate(L.PHASES)}, calls.value
 
+    def tiles(self):
+        """The phi-transform tile shapes the plan's tile search chose, and its time (ms)."""
+        t = np.zeros(6, dtype=np.int32)
+        ms = ctypes.c_double()
+        L.check(L.lib().sfb_plan_tiles(self._h, L.ptr(t), ctypes.byref(ms)))
+        keys = ("fft_maps_fp64", "fft_threads_fp64", "fft_maps_fp32", "fft_threads_fp32", "polar_maps_fp64",
+                "polar_maps_fp32")
+        return dict(zip(keys, (int(x) for x in t))), ms.value
+
     # -- helpers --------------------------------------------------------
     def _window(self, win):
         am = self.amodes
