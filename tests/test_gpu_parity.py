@@ -149,6 +149,22 @@ def test_map2alm_vs_oracle(nside, lmax, niter):
         assert err < 1e-12, err
 
 
+def test_tile_search_leaves_results_unchanged(c1, monkeypatch):
+    """The plan's phi-transform tile search picks launch shapes only: a plan built with it
+    and one built with SFB_TUNE=0 (fixed defaults) return bit-identical a_nlm, FP64 and mixed."""
+    am, win, cat, nbar = c1
+    tuned = S.Plan(am, niter=3)
+    monkeypatch.setenv("SFB_TUNE", "0")
+    fixed = S.Plan(am, niter=3)
+    say("tiles", tuned.tiles(), fixed.tiles())
+    for prec in ("fp64", "mixed"):
+        tuned.set_precision(prec)
+        fixed.set_precision(prec)
+        a = tuned.cat2amln(cat, nbar, win, None)
+        b = fixed.cat2amln(cat, nbar, win, None)
+        assert np.array_equal(a, b), prec
+
+
 def test_cat2amln_c1_fp64_vs_oracle(c1):
     am, win, cat, nbar = c1
     say("cat2amln C1 fp64")
