@@ -969,14 +969,21 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
                                                      const RingDev* __restrict__ rings,
                                                      const double2* __restrict__ tw, double scale,
                                                      double* __restrict__ out, int32_t TQMAX,
-                                                     const int32_t* __restrict__ mall)
+                                                     const int32_t* __restrict__ mall, int32_t nring,
+                                                     int32_t ntile)
 {
     extern __shared__ double2 ip_raw[];
-    const int k = blockIdx.x;
+    // 1-D grid over units (ring pair, map tile), tiles fastest; workgroup b runs on XCD
+    // b mod 8 and each XCD takes runs of four consecutive units, so neighbouring tiles,
+    // which share the 128-byte lines of X, are fetched into one L2
+    const int b = blockIdx.x, loc = b >> 3;
+    const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
+    if (unit >= nring * ntile) return;
+    const int k = unit / ntile;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr, ni = npr >> 2, n2 = 2 * npr, nu = ni / 2 + 1, njp = (ni + 1) / 2;
     const int TQ = min(TQMAX, (int)blockDim.x / (2 * njp));
-    const int q0 = blockIdx.y * TQ;
+    const int q0 = (unit - k * ntile) * TQ;
     if (q0 >= nq) return;
     const int nf = 2 * TQ;
     const double2* twl = tw + Rg.tw_off;  // (cos, sin)(pi kk/npr), kk < 2 npr
@@ -3516,9 +3523,12 @@ static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
     }
     if (TQI > 0 && k_eq0 > 0 && (int64_t)shq <= P->lds_max) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shq));
-        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)k_eq0, (unsigned)((nq + tqmin - 1) / tqmin)), dim3(256), shq,
-                           st, P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX, P->d_mall.as<int32_t>());
+        const int32_t ntile = (int32_t)((nq + tqmin - 1) / tqmin);
+        const int64_t units = (int64_t)k_eq0 * ntile;
+        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), shq, st,
+                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX, P->d_mall.as<int32_t>(),
+                           k_eq0, ntile);
         LAUNCH_CHECK();
     } else if (k_eq0 > 0) {
         const int32_t TQS = 8;
