@@ -182,7 +182,7 @@ def main():
     log(f"[adjoint fp64] {adj_dt * 1e3:.2f} ms/step")
 
     # batched mocks in the same geometry, generated on the device, one
-    # sfb_cat2amln_batch_device call over all of them (FP64)
+    # sfb_cat2amln_batch_device call over all of them (FP64, then mixed)
     batch = None
     if args.mocks > 0:
         rng = np.random.default_rng(3)
@@ -196,15 +196,24 @@ def main():
         del u
         nbars = N_b / (4 * np.pi / 3 * (rmax ** 3 - rmin ** 3))
         d_mout = torch.empty(args.mocks * 2 * nlm, dtype=torch.float64, device=dev)
-        plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_mout)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_mout)
-        torch.cuda.synchronize()
-        bdt = time.perf_counter() - t
-        batch = {"mocks": args.mocks, "galaxies": tot, "ms": bdt * 1e3, "ms_per_mock": bdt * 1e3 / args.mocks,
-                 "Mgal_s": tot / bdt / 1e6, "path": "fp64"}
-        log(f"[batch] {args.mocks} mocks, {tot} galaxies: {bdt * 1e3:.1f} ms ({tot / bdt / 1e6:.1f} Mgal/s)")
+        batch = {}
+        for prec in ("fp64", "mixed"):
+            plan.set_precision(prec)
+            plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln,
+                                       d_mout)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plan.cat2amln_batch_device(args.mocks, N_b, d_mock, None, nbars, L.SFB_WIN_SEPARABLE, d_mask, d_wln,
+                                       d_mout)
+            torch.cuda.synchronize()
+            bdt = time.perf_counter() - t
+            b = {"mocks": args.mocks, "galaxies": tot, "ms": bdt * 1e3, "ms_per_mock": bdt * 1e3 / args.mocks,
+                 "Mgal_s": tot / bdt / 1e6, "path": prec}
+            if prec == "fp64":
+                batch = b
+            else:
+                batch["mixed"] = b
+            log(f"[batch {prec}] {args.mocks} mocks, {tot} galaxies: {bdt * 1e3:.1f} ms ({tot / bdt / 1e6:.1f} Mgal/s)")
         del d_mock, d_mout
 
     head = "mixed" if "mixed" in results else "fp64"
