@@ -818,26 +818,41 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
             }
         }
     }
-    for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
-        const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
-        const int ql = f >> 1, side = f & 1;
-        const int q = q0 + ql;
-        R re = 0, im = 0;
-        if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
-            const int64_t pix = (side ? Rg.start_s : Rg.start_n) + 2 * jj;
-            if (perm) {  // slot-order maps: both pixels through the permutation
-                const uint2 sl = *(const uint2*)(perm + pix);
-                re = delta[(int64_t)q * npix + sl.x];
-                im = delta[(int64_t)q * npix + sl.y];
-            } else {
-                const R2 v = *(const R2*)(delta + (int64_t)q * npix + pix);
+    if (perm) {
+        // slot-order maps: per (side, pixel pair) the two slots are read once and the pair
+        // gathered for each of the block's TQ maps
+        for (int e = threadIdx.x; e < 2 * M; e += blockDim.x) {
+            const int side = e >> logM, jj = e & (M - 1);  // M = 2^logM
+            const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+            uint2 sl = make_uint2(0u, 0u);
+            const bool live = !(side && eq);  // sample pairs start at even pixels
+            if (live) sl = *(const uint2*)(perm + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
+            for (int ql = 0; ql < TQ; ++ql) {
+                const int q = q0 + ql;
+                R re = 0, im = 0;
+                if (live && q < nq) {
+                    re = delta[(int64_t)q * npix + sl.x];
+                    im = delta[(int64_t)q * npix + sl.y];
+                }
+                zs[(2 * ql + side) * Mp + fpad<R>(rj)].x = re;
+                zs[(2 * ql + side) * Mp + fpad<R>(rj)].y = im;
+            }
+        }
+    } else {
+        for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
+            const int f = e >> logM, jj = e & (M - 1);  // M = 2^logM
+            const int ql = f >> 1, side = f & 1;
+            const int q = q0 + ql;
+            R re = 0, im = 0;
+            if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
+                const R2 v = *(const R2*)(delta + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
                 re = v.x;
                 im = v.y;
             }
+            const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
+            zs[f * Mp + fpad<R>(rj)].x = re;
+            zs[f * Mp + fpad<R>(rj)].y = im;
         }
-        const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
-        zs[f * Mp + fpad<R>(rj)].x = re;
-        zs[f * Mp + fpad<R>(rj)].y = im;
     }
     __syncthreads();
     fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
@@ -1195,34 +1210,45 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
     // samples in fours (npr = 4i and every polar ring starts at a multiple of 4), PH_UL
     // loads in flight per thread before their LDS stores
     const int nv = npr >> 2, nvt = 2 * TQ * nv;
-    for (int e0 = threadIdx.x; e0 < nvt; e0 += PH_UL * blockDim.x) {
-        R4 v[PH_UL];
-#pragma unroll
-        for (int u = 0; u < PH_UL; ++u) {
-            const int e = e0 + u * blockDim.x;
-            const int f = e / nv, j4 = e - f * nv;
-            const int q = q0 + (f >> 1);
-            v[u].x = v[u].y = v[u].z = v[u].w = 0;
-            if (e < nvt && q < nq) {
-                const int64_t pix = ((f & 1) ? Rg.start_s : Rg.start_n) + 4 * j4;
-                if (perm) {  // slot-order maps: the four pixels through the permutation
-                    const uint4 sl = *(const uint4*)(perm + pix);
+    if (perm) {
+        // slot-order maps: per (side, four pixels) the slots are read once and the four
+        // samples gathered for each of the block's TQ maps
+        for (int e = threadIdx.x; e < 2 * nv; e += blockDim.x) {
+            const int side = e / nv, j4 = e - side * nv;
+            const uint4 sl = *(const uint4*)(perm + (side ? Rg.start_s : Rg.start_n) + 4 * j4);
+            for (int ql = 0; ql < TQ; ++ql) {
+                const int q = q0 + ql;
+                R4 v;
+                v.x = v.y = v.z = v.w = 0;
+                if (q < nq) {
                     const R* dq = delta + (int64_t)q * npix;
-                    v[u].x = dq[sl.x];
-                    v[u].y = dq[sl.y];
-                    v[u].z = dq[sl.z];
-                    v[u].w = dq[sl.w];
-                } else {
-                    v[u] = *(const R4*)(delta + (int64_t)q * npix + pix);
+                    v.x = dq[sl.x];
+                    v.y = dq[sl.y];
+                    v.z = dq[sl.z];
+                    v.w = dq[sl.w];
                 }
+                *(R4*)(D + (2 * ql + side) * ld + 4 * j4) = v;
             }
         }
+    } else {
+        for (int e0 = threadIdx.x; e0 < nvt; e0 += PH_UL * blockDim.x) {
+            R4 v[PH_UL];
 #pragma unroll
-        for (int u = 0; u < PH_UL; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < nvt) {
+            for (int u = 0; u < PH_UL; ++u) {
+                const int e = e0 + u * blockDim.x;
                 const int f = e / nv, j4 = e - f * nv;
-                *(R4*)(D + f * ld + 4 * j4) = v[u];
+                const int q = q0 + (f >> 1);
+                v[u].x = v[u].y = v[u].z = v[u].w = 0;
+                if (e < nvt && q < nq)
+                    v[u] = *(const R4*)(delta + (int64_t)q * npix + ((f & 1) ? Rg.start_s : Rg.start_n) + 4 * j4);
+            }
+#pragma unroll
+            for (int u = 0; u < PH_UL; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < nvt) {
+                    const int f = e / nv, j4 = e - f * nv;
+                    *(R4*)(D + f * ld + 4 * j4) = v[u];
+                }
             }
         }
     }
