@@ -3239,11 +3239,10 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     double inv = 1.0 / (nbar * domega);
     P->delta_perm = nullptr;
     if (P->sell_ok && !P->sell_empty && P->sell_fits) {
-        // FP64 delta in slot order when the slices tile the pixels exactly (npix a multiple
-        // of 64): each slice end stores one contiguous row per map (4.2 -> 3.5 ms at C2) and
-        // the phi-transform gathers through the permutation (+0.2 ms).  On the mixed path the
-        // FP32 gathers cost what the stores save, so its maps stay in pixel order.
-        if (!f32 && 64 * (int64_t)P->sell_nslice == P->npix) P->delta_perm = P->d_invperm.as<uint32_t>();
+        // delta in slot order when the slices tile the pixels exactly (npix a multiple of 64):
+        // each slice end stores one contiguous row per map (FP64 4.2 -> 3.5 ms, FP32 2.35 ->
+        // 2.10 ms at C2) and the phi-transform gathers through the permutation (+0.14 ms).
+        if (64 * (int64_t)P->sell_nslice == P->npix) P->delta_perm = P->d_invperm.as<uint32_t>();
         // shared segment search per knot group, then three fma per map and galaxy
         const int64_t T = std::max<int64_t>(P->sell_total, 1);
         int rc;
@@ -3413,6 +3412,36 @@ static int autotune_phi(sfb_plan* P)
     const int64_t nd = P->nmaps * P->npix;
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nd, 256)), dim3(256), 0, st, P->d_delta.as<double>(), nd, 0.0);
     LAUNCH_CHECK();
+    // The sliced scatter leaves maps in slot order (pixels permuted inside windows of
+    // SELL_SIGMA); time the tiles on that access pattern, with a fixed pseudo-random
+    // permutation inside the same windows.
+    DevBuf d_perm;
+    struct Release {  // the permutation buffer is freed on every return
+        DevBuf& b;
+        sfb_plan* p;
+        ~Release()
+        {
+            p->delta_perm = nullptr;
+            b.release();
+        }
+    } release_perm{d_perm, P};
+    if (P->sell_ok && P->npix % 64 == 0) {
+        std::vector<uint32_t> perm((size_t)P->npix);
+        uint64_t x = 0x9E3779B97F4A7C15ull;
+        for (int64_t w0 = 0; w0 < P->npix; w0 += SELL_SIGMA) {
+            const int64_t w1 = std::min<int64_t>(P->npix, w0 + SELL_SIGMA);
+            for (int64_t p = w0; p < w1; ++p) perm[(size_t)p] = (uint32_t)p;
+            for (int64_t p = w1 - 1; p > w0; --p) {  // Fisher-Yates with a 64-bit LCG
+                x = x * 6364136223846793005ull + 1442695040888963407ull;
+                const int64_t j = w0 + (int64_t)((x >> 33) % (uint64_t)(p - w0 + 1));
+                std::swap(perm[(size_t)p], perm[(size_t)j]);
+            }
+        }
+        int rc0 = d_perm.ensure(sizeof(uint32_t) * perm.size());
+        if (rc0) return rc0;
+        HIP_TRY(hipMemcpy(d_perm.p, perm.data(), sizeof(uint32_t) * perm.size(), hipMemcpyHostToDevice));
+        P->delta_perm = d_perm.as<uint32_t>();
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -3480,6 +3509,7 @@ static int autotune_phi(sfb_plan* P)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    HIP_TRY(hipStreamSynchronize(st));  // the search's work is done before d_perm is freed
     P->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
 }
