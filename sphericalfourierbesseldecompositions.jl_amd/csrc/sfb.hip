@@ -788,13 +788,19 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                  const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
                                                  int32_t k0, int32_t logM, const int32_t* __restrict__ mall,
-                                                 const uint32_t* __restrict__ perm)
+                                                 const uint32_t* __restrict__ perm, int32_t nring, int32_t ntile)
 {
     typedef typename Vec2T<R>::T R2;
     extern __shared__ double2 zs_raw[];
     R2* zs = (R2*)zs_raw;
-    const int k = k0 + blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
+    // 1-D grid over units (map tile, ring pair), ring pairs fastest; workgroup b runs on
+    // XCD b mod 8 and each XCD takes runs of four consecutive units: neighbouring ring
+    // pairs of one tile read the same lines of slot-order maps into one L2
+    const int b = blockIdx.x, loc = b >> 3;
+    const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
+    if (unit >= nring * ntile) return;
+    const int k = k0 + unit % nring;
+    const int q0 = (unit / nring) * TQ;
     const RingDev Rg = rings[k];
     const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
     const bool eq = (k == K - 1);
@@ -1184,20 +1190,24 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
                                                     int64_t M2, int32_t K, int32_t L,
                                                     const RingDev* __restrict__ rings, const double2* __restrict__ tw,
                                                     R* __restrict__ X, int32_t TQMAX, const int32_t* __restrict__ mall,
-                                                    const uint32_t* __restrict__ perm)
+                                                    const uint32_t* __restrict__ perm, int32_t nring, int32_t ntile)
 {
     typedef typename Vec2T<R>::T R2;
     typedef typename Vec4T<R>::T R4;
     extern __shared__ double2 ps_raw[];
     R2* ps = (R2*)ps_raw;
-    const int k = blockIdx.x;
+    // 1-D XCD-grouped grid over (map tile, ring pair) as k_phi_fft
+    const int b = blockIdx.x, loc = b >> 3;
+    const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
+    if (unit >= nring * ntile) return;
+    const int k = unit % nring;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
     // maps per block: as many as fill the work items (2 TQ ceil(nu / 2) <= blockDim.x),
     // so small rings take many maps per block; blocks past the last map leave at once
     const int nup = (nu + 1) / 2;
     const int TQ = min(TQMAX, (int)blockDim.x / (2 * nup));
-    const int q0 = blockIdx.y * TQ;
+    const int q0 = (unit / nring) * TQ;
     if (q0 >= nq) return;
     const int ld = npr + 8;                       // doubles per sequence row
     const double2* twl = tw + Rg.tw_off;          // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
@@ -3354,16 +3364,19 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     // FP32 sequences take half the LDS: twice as many per block at the same occupancy
     const int32_t tqf = f32 ? P->TQF32 : P->TQF;
     const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
+    // 1-D grid over (map tile, ring pair) units, rounded up to whole runs of 4 units on 8 XCDs
+    const int32_t nring_f = K - k_eq0, ntile_f = (int32_t)nblk(nq, tqf);
+    const dim3 grid_f((unsigned)(32 * (((int64_t)nring_f * ntile_f + 31) / 32)));
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<float>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(P->fft_nt32), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<float>, grid_f, dim3(P->fft_nt32), shf, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall, P->delta_perm);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<double>, dim3((unsigned)(K - k_eq0), nblk(nq, tqf)), dim3(P->fft_nt), shf, st,
+        hipLaunchKernelGGL(k_phi_fft<double>, grid_f, dim3(P->fft_nt), shf, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall, P->delta_perm);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     }
     LAUNCH_CHECK();
     if (k_eq0 == 0) return SFB_OK;
@@ -3380,16 +3393,17 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
         LAUNCH_CHECK();
         return SFB_OK;
     }
+    const dim3 grid_p((unsigned)(32 * (((int64_t)k_eq0 * gy4 + 31) / 32)));
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<float>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
+        hipLaunchKernelGGL(k_phi_polar4<float>, grid_p, dim3(256), shp, st,
                            P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->p4_tq32, mall, P->delta_perm);
+                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->p4_tq32, mall, P->delta_perm, k_eq0, gy4);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<double>, dim3((unsigned)k_eq0, (unsigned)gy4), dim3(256), shp, st,
+        hipLaunchKernelGGL(k_phi_polar4<double>, grid_p, dim3(256), shp, st,
                            P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->p4_tq, mall, P->delta_perm);
+                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->p4_tq, mall, P->delta_perm, k_eq0, gy4);
     }
     LAUNCH_CHECK();
     return SFB_OK;
