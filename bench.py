@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--mocks", type=int, default=64,
                     help="batched-mock measurement: catalogues with N_b ~ Poisson(mu_b), mu_b ~ U[1e5, 1e6] "
                          "(0 skips it)")
+    ap.add_argument("--no-host-path", action="store_true",
+                    help="skip the timing of the host-array entry point (sfb_cat2amln)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 512 MiB (torch.neg, FP64) before the steps, as the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -164,6 +166,22 @@ def main():
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     for prec in precs:
         results[prec] = run(prec)
+
+    # the same call from host arrays (sfb_cat2amln: catalogue upload and a_nlm download
+    # over PCIe inside the call), per precision; reported beside `value`, never as it
+    # (the catalogue in the reference's layout, 3 x N column-major, which the wrapper passes as is)
+    host_path = {}
+    cat_cm = np.asfortranarray(cat)
+    for prec in ([] if args.no_host_path else precs):
+        plan.set_precision(prec)
+        for _ in range(max(1, args.warmup)):
+            plan.cat2amln(cat_cm, nbar, win, None)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plan.cat2amln(cat_cm, nbar, win, None)
+        hdt = (time.perf_counter() - t) / args.steps
+        host_path[prec] = {"ms": hdt * 1e3, "Mgal_s": N / hdt / 1e6}
+        log(f"[{prec} host arrays] {hdt * 1e3:.2f} ms/step ({N / hdt / 1e6:.1f} Mgal/s)")
 
     # adjoint (sfb_cat2amln_vjp_device, FP64) on the same catalogue: dL/dr, dL/dw for a
     # random cotangent, inputs resident in HBM
@@ -300,6 +318,7 @@ def main():
         "setup_s": {"AnlmModes": t_modes, "plan": t_plan},
         "phi_tiles": tiles,
         "adjoint_fp64_ms": adj_dt * 1e3,
+        "host_arrays_path": host_path,
         "batched_mocks": batch,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -794,10 +794,10 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
     extern __shared__ double2 zs_raw[];
     R2* zs = (R2*)zs_raw;
     // 1-D grid over units (map tile, ring pair), ring pairs fastest; workgroup b runs on
-    // XCD b mod 8 and each XCD takes runs of four consecutive units: neighbouring ring
+    // XCD b mod 8 and each XCD takes runs of eight consecutive units: neighbouring ring
     // pairs of one tile read the same lines of slot-order maps into one L2
     const int b = blockIdx.x, loc = b >> 3;
-    const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
+    const int unit = (((loc >> 3) << 3) + (b & 7)) * 8 + (loc & 7);
     if (unit >= nring * ntile) return;
     const int k = k0 + unit % nring;
     const int q0 = (unit / nring) * TQ;
@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
     typedef typename Vec4T<R>::T R4;
     extern __shared__ double2 ps_raw[];
     R2* ps = (R2*)ps_raw;
-    // 1-D XCD-grouped grid over (map tile, ring pair) as k_phi_fft
+    // 1-D grid over (map tile, ring pair) units as k_phi_fft, in runs of four per XCD
     const int b = blockIdx.x, loc = b >> 3;
     const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
     if (unit >= nring * ntile) return;
@@ -3364,9 +3364,10 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     // FP32 sequences take half the LDS: twice as many per block at the same occupancy
     const int32_t tqf = f32 ? P->TQF32 : P->TQF;
     const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
-    // 1-D grid over (map tile, ring pair) units, rounded up to whole runs of 4 units on 8 XCDs
+    // 1-D grid over (map tile, ring pair) units, rounded up to whole runs of 8 units on 8 XCDs
+    // (runs of 2, 4 and 8 were timed: FP32 0.92, 0.87, 0.82 ms; the polar rings keep runs of 4)
     const int32_t nring_f = K - k_eq0, ntile_f = (int32_t)nblk(nq, tqf);
-    const dim3 grid_f((unsigned)(32 * (((int64_t)nring_f * ntile_f + 31) / 32)));
+    const dim3 grid_f((unsigned)(64 * (((int64_t)nring_f * ntile_f + 63) / 64)));
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<float>, grid_f, dim3(P->fft_nt32), shf, st,
