@@ -121,6 +121,35 @@ def test_delta_map_large_tables_per_pixel_bitexact():
             assert np.array_equal(g, c), f"map {q}: max diff {np.max(np.abs(g - c))}"
 
 
+def test_delta_map_tied_radii_and_knot_radii_bitexact(c1):
+    """Galaxies that share one radius inside one pixel (their weighted sum runs in input order,
+    the order a stable sort by r leaves equal radii in), interleaved in the input with the rest,
+    and galaxies exactly on spline knots (segment boundaries, the first and the last knot of a
+    map, where the segment search and t = 0 or the boundary record decide): delta maps
+    bit-exact against the oracle, with distinct weights so that any reordering would show."""
+    am, win, _, nbar = c1
+    rng = np.random.default_rng(41)
+    q = am.nmaps // 3
+    kx = am.knot_x[am.knot_off[q]:am.knot_off[q + 1]]
+    parts = []
+    for _ in range(40):  # 40 pixels, 50 galaxies each at one radius
+        r0, th0, ph0 = rng.uniform(500.0, 1000.0), np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi)
+        parts.append(np.stack([np.full(50, r0), np.full(50, th0), np.full(50, ph0)]))
+    nk = kx.size  # every knot of map q, twice, at random angles
+    parts.append(np.stack([np.tile(kx, 2), np.arccos(rng.uniform(-1, 1, 2 * nk)), rng.uniform(0, 2 * np.pi, 2 * nk)]))
+    parts.append(uniform_shell_catalog(4000, 500.0, 1000.0, seed=42))
+    cat = np.concatenate(parts, axis=1)
+    cat = cat[:, rng.permutation(cat.shape[1])]
+    w = rng.uniform(0.5, 1.5, cat.shape[1])
+    plan = S.get_plan(am)
+    om = _oracle_modes_from(am)
+    for qq in (0, q, am.nmaps - 1):
+        say("tied / knot radii delta map", qq)
+        g = plan.delta_map(cat, nbar, win, w, qq)
+        c = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), w, qq)
+        assert np.array_equal(g, c), f"map {qq}: max diff {np.max(np.abs(g - c))}"
+
+
 def test_delta_map_weights_dense_bitexact(c1):
     am, win, cat, nbar = c1
     rng = np.random.default_rng(5)
