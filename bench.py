@@ -97,7 +97,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the (n,l) maps would shard "
-              "across GPUs as described in DESIGN.md, but no multi-GPU path is built.", file=sys.stderr)
+              "across GPUs (each GPU a share of the maps, the sorted catalogue broadcast once, a_nlm "
+              "gathered), but no multi-GPU path is built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
