@@ -805,7 +805,7 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
     const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
-    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1, 2 or 4)
+    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1 to 8, the tile search's candidates)
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     R2* twr = zs + nf * Mp;               // kk <= N staged in LDS
     for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
@@ -914,7 +914,7 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, 
     const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<double>::SH);
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
-    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1, 2 or 4)
+    const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1 or 2)
     double2* zs = zi_raw;                 // nf x Mp (padded sequences)
     double2* twr = zs + nf * Mp;          // N + 1 twiddles
     double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
