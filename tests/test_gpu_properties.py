@@ -6,6 +6,7 @@ run in full, plus the reference's error behaviour.
   order, so shuffling a catalogue with distinct radii gives bit-identical a_nlm.
 * Affine structure: cat2amln(w) = A(S(w) - W) with S linear, so
   f(2w) - 2 f(w) + f(empty catalogue) = 0 up to rounding.
+* The device entry point (inputs in HBM) against the host one: bit-identical.
 * A bounded oracle sample: the first maps of the same call against the oracle.
 * Errors: a NaN weight gives AssertionError (src/cat2anlm.jl:308), a NaN angle is
   rejected (Healpix.jl's floor(Int, NaN) throws), lmax > 4 nside is refused.
@@ -52,6 +53,58 @@ def test_affine_in_weights(c2):
     f0 = plan.cat2amln(np.zeros((3, 0)), nbar, win, [])
     resid = f2 - 2 * f1 + f0
     assert np.max(np.abs(resid)) < 1e-11 * np.max(np.abs(f2))
+
+
+def test_device_entry_matches_host_entry(c2):
+    """sfb_cat2amln_device (catalogue, window and output already in HBM: the call bench.py
+    times) returns the a_nlm of sfb_cat2amln from host arrays bit for bit, on both paths, with
+    unit and given weights, on the plan's stream and on a caller's stream.  Device memory and
+    the stream come from the HIP runtime the library itself links (no second runtime)."""
+    import ctypes
+    from sfb_amd import _lib as L
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    H2D, D2H = 1, 2
+    am, win, cat, nbar, plan = c2
+    N = cat.shape[1]
+    w = np.random.default_rng(13).uniform(0.5, 1.5, N)
+    host_in = [np.ascontiguousarray(cat.T), w, np.ascontiguousarray(win.mask, dtype=np.float64),
+               np.ascontiguousarray(win.w_ln.reshape(-1, order="F"))]
+    nout = 2 * S.getnlmsize(am)
+    ptrs, stream = [], ctypes.c_void_p()
+
+    def dmalloc(nbytes):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) == 0
+        ptrs.append(p)
+        return p.value
+
+    def copy(dst, src, nbytes, kind):
+        assert hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes), kind) == 0
+
+    try:
+        d_rtp, d_w, d_mask, d_wln = (dmalloc(a.nbytes) for a in host_in)
+        for d, a in zip((d_rtp, d_w, d_mask, d_wln), host_in):
+            copy(d, a.ctypes.data, a.nbytes, H2D)
+        d_out = dmalloc(8 * nout)
+        assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+        for prec in ("fp64", "mixed"):
+            plan.set_precision(prec)
+            for ww, dw in ((None, None), (w, d_w)):
+                host = plan.cat2amln(cat, nbar, win, ww)
+                for st in (None, stream.value):
+                    say("device entry", prec, "unit" if ww is None else "weights",
+                        "plan stream" if st is None else "caller stream")
+                    out = np.full(nout, np.nan)
+                    copy(d_out, out.ctypes.data, out.nbytes, H2D)
+                    plan.cat2amln_device(N, d_rtp, dw, nbar, L.SFB_WIN_SEPARABLE, d_mask, d_wln, d_out, stream=st)
+                    copy(out.ctypes.data, d_out, out.nbytes, D2H)
+                    assert np.array_equal(out[0::2] + 1j * out[1::2], host), (prec, ww is None, st)
+    finally:
+        plan.set_precision("fp64")
+        if stream.value:
+            hip.hipStreamDestroy(stream)
+        for p in ptrs:
+            hip.hipFree(p)
 
 
 def test_c2_sample_against_oracle(c2):
