@@ -3412,12 +3412,11 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     return SFB_OK;
 }
 
-// niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a
-// (Healpix map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.
 // Plan-time tile search for the phi-transform.  Candidates: maps per FFT block (powers of
 // two whose LDS fits) x 256 / 512 threads, for FP64 and for FP32, then up to 32 / 64 maps per
-// polar block.  Each runs the whole transform over every map of the plan's (zeroed) map
-// buffer, once to warm up and three times timed on the plan's stream; the fastest stays.
+// polar block.  Each runs the whole transform over the first TUNE_MAPS maps of the plan's
+// (zeroed) map buffer (every map alike; 1024 maps give the grid thousands of blocks per
+// ring-pair class), once to warm up and three times timed on the plan's stream; the fastest stays.
 // Tile shapes change no arithmetic (each ring sequence is transformed alike), so the search
 // moves only the time.  SFB_TUNE=0 in the environment keeps the fixed defaults.
 static int autotune_phi(sfb_plan* P)
@@ -3426,8 +3425,9 @@ static int autotune_phi(sfb_plan* P)
     if (env && env[0] == '0') return SFB_OK;
     const auto t_start = std::chrono::steady_clock::now();
     hipStream_t st = P->stream;
-    const int32_t nq = (int32_t)P->nmaps;
-    const int64_t nd = P->nmaps * P->npix;
+    constexpr int32_t TUNE_MAPS = 1024;
+    const int32_t nq = (int32_t)std::min<int64_t>(P->nmaps, TUNE_MAPS);
+    const int64_t nd = (int64_t)nq * P->npix;
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nd, 256)), dim3(256), 0, st, P->d_delta.as<double>(), nd, 0.0);
     LAUNCH_CHECK();
     // The sliced scatter leaves maps in slot order (pixels permuted inside windows of
@@ -3532,8 +3532,9 @@ static int autotune_phi(sfb_plan* P)
     return rc;
 }
 
-// niter Jacobi refinements of a.  f32 (mixed path): the iterate a (and b), the
-// aliasing-ring values G and their aliasing residual R are stored as floats; every
+// niter Jacobi refinements a <- b + (I - A S) a, b and the first a in d_a (Healpix
+// map2alm! iteration, src/healpix_helpers.jl:55); leaves a_final.  f32 (mixed path): the
+// iterate a (and b), the aliasing-ring values G and their aliasing residual R are stored as floats; every
 // GEMM runs on FP32 MFMA with FP32 tables and an FP64 epilogue (rounded once on the
 // store), and k_alias sums in FP64.  Their FP32 rounding stays far below that of the
 // mixed path's FP32 scatter and phi-transform; the bytes they move halve.  Ring pairs
