@@ -168,8 +168,11 @@ int sfb_amln2clnn(sfb_plan* p, int64_t nlnn, const int64_t* lnn, const double* a
 /* Vector-Jacobian product of cat2amln at (rθϕ, w, nbar): given dL_danlm (2*nlmsize
  * doubles, (dL/dRe a, dL/dIm a) interleaved in a_nlm order), returns dL/dr_i and
  * dL/dw_i (N each, input order) and dL/dnbar.  dL/dθ and dL/dφ are zero (the
- * pixel assignment is piecewise constant) and the window does not enter (the
- * transform is affine in it).  g'(r) is the spline derivative of src/Splines.jl:367-377
+ * pixel assignment is piecewise constant).  The window cotangent is not returned:
+ * a is affine in the window with dL/dW[p, l, n] = -dbar_{nl}(p), the synthesised
+ * cotangent map this function forms internally, but no entry point hands it out.
+ * A non-finite angle is an error (SFB_ERR_ARG), as in the forward call.
+ * g'(r) is the spline derivative of src/Splines.jl:367-377
  * (derivative(gnl, n, l, r), src/GNLs.jl:110).  No counterpart in the reference,
  * which has no derivative of cat2amln; it backs the ChainRulesCore.rrule of the
  * Julia shim.  Any of dL_dr, dL_dw, dL_dnbar may be NULL. */
@@ -217,7 +220,8 @@ int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const doub
 #define SFB_PHASE_RESIDUAL 5   /* ring-Fourier residual with aliasing (niter launches) */
 #define SFB_PHASE_PACK 6       /* a_nlm packing and the non-finite check */
 #define SFB_PHASE_SEGMENT 7    /* per knot group and galaxy: spline segment and t (shared by the group's maps) */
-#define SFB_NPHASE 8
+#define SFB_PHASE_MONOPOLE 8   /* mixed precision: the l = 0 maps' scatter, phi-transform and Legendre steps in FP64 */
+#define SFB_NPHASE 9
 
 /* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and
  * clears the totals.  No reference counterpart (the reference prints @time). */
@@ -259,6 +263,15 @@ int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const doubl
 /* The same GEMM through the FP32-operand MFMA kernel used for the Jacobi corrections
  * of SFB_PREC_MIXED (operands rounded to FP32, FP64 accumulation across k-tiles). */
 int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
+
+/* Launch shapes of the ring transforms for a geometry, computed on the host (no device):
+ * out[0] polar ring pairs on the forward radix-4 kernel (nside - 1), out[1] its LDS bytes
+ * per block (FP64), out[2] its fewest maps per block; out[3] polar ring pairs on the
+ * inverse radix-4 kernel, out[4] ring pairs on the inverse direct sum, out[5] the inverse
+ * radix-4 LDS bytes, out[6] its fewest maps per block, out[7] maps per inverse-FFT block.
+ * SFB_ERR_ARG when the forward blocks do not fit in lds_bytes.  No reference counterpart:
+ * it lets a host-only test check the shapes for every nside. */
+int sfb_ring_kernel_shapes(int64_t nside, int64_t lmax, int64_t lds_bytes, int64_t* out);
 
 /* The per-ring phi-transform of nmap maps [nmap][npix]: x_out[((m*2+par)*K + k)*2*nmap + 2q+c],
  * K = 2 nside ring pairs, par 0 = F_north + F_south, 1 = F_north - F_south,

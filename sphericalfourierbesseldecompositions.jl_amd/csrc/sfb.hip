@@ -12,7 +12,7 @@
 //                 segment i and t = (r - x_i)/h, correctly rounded
 //   k_scatter_it  delta[q][p] = (sum_{i in p} w_i g_q(r_i)) * 1/(nbar dOmega) - W_q(p)
 //                 with the reference's spline formula and rounding (cat2anlm.jl:284-291)
-//   k_phi_fft / k_phi_polar
+//   k_phi_fft / k_phi_polar4
 //                 per ring pair (north k, south 4 nside-2-k): F_m = sum_j delta_j e^{-i m phi_j},
 //                 stored as E = F_N + F_S and O = F_N - F_S for m = 0..lmax
 //   Legendre      a_{l m} = dOmega sum_k lambda_lm(theta_k) (E or O by parity of l+m):
@@ -224,17 +224,20 @@ __device__ __forceinline__ double spline_dev(const MapInfo& mi, const double* __
     return fma(t, fma(t, fma(t, s[2], s[3]), s[4]), s[5]);
 }
 
-// delta[q][p] for every map q (blockIdx.y) and pixel p.
+// delta[q][p] for every map q (blockIdx.y) and pixel p.  With a map list, row y of
+// delta is map qlist[y] (the mixed path's FP64 monopole maps).
 template <bool DENSE, typename R = double>
 __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __restrict__ pstart,
                                                  const double* __restrict__ r_s, const double* __restrict__ w_s,
                                                  const MapInfo* __restrict__ maps, const double* __restrict__ kx,
                                                  const double* __restrict__ seg, double inv,
                                                  const double* __restrict__ win_a, const double* __restrict__ win_b,
-                                                 int64_t lmax, int32_t q0, R* __restrict__ delta)
+                                                 int64_t lmax, int32_t q0, R* __restrict__ delta,
+                                                 const int32_t* __restrict__ qlist = nullptr)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t q = q0 + (int32_t)blockIdx.y;
+    const int32_t row = q0 + (int32_t)blockIdx.y;
+    const int32_t q = qlist ? qlist[blockIdx.y] : row;
     if (p >= npix) return;
     const MapInfo mi = maps[q];
     double s = 0.0;
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __
     double v = __dmul_rn(s, inv);
     int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
     double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], win_b[col]);
-    delta[(int64_t)q * npix + p] = (R)__dsub_rn(v, W);
+    delta[(int64_t)row * npix + p] = (R)__dsub_rn(v, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,7 +533,8 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_r
 // row boundaries common to the whole wave, where every lane writes its pixel's delta
 // and restarts its running sums.  Per pixel the sum runs over its galaxies in order.
 // R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
-// w and the records rounded to FP32, FP32 Horner and pixel sums, FP32 delta.
+// w and the records rounded to FP32, FP32 Horner and w g products, FP64 pixel sums
+// (an FP32 running sum cost up to 4e-7 of an (n,l) block at nside 128), FP32 delta.
 template <bool DENSE, int NM, bool UNIT, typename R>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
@@ -599,7 +603,8 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
 #pragma unroll
             for (int k = 0; k < NM; ++k) st.w[k] = win_a[st.pc.x + npix * col[k]];
     };
-    R acc[NM];
+    // per-pixel sums in FP64 on both paths (FP32 spline values are widened exactly)
+    double acc[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) acc[k] = 0;
 
@@ -622,7 +627,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                     const double W = DENSE ? cs.w[k] : __dmul_rn(cs.w[k], wcol[k]);
                     // slot order: the 64 lanes of a slice store one contiguous row
                     const int64_t at = slot_order ? (int64_t)s * 64 + lane : (int64_t)cs.pc.x;
-                    delta[(int64_t)B.q[k] * npix + at] = (R)__dsub_rn(__dmul_rn((double)acc[k], inv), W);
+                    delta[(int64_t)B.q[k] * npix + at] = (R)__dsub_rn(__dmul_rn(acc[k], inv), W);
                 }
             }
 #pragma unroll
@@ -661,7 +666,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                 for (int k = 0; k < NM; ++k) v[k] = live ? v[k] : (R)0;
             }
 #pragma unroll
-            for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + v[k] : acc[k] + cur.w[u] * v[k];
+            for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + (double)v[k] : acc[k] + (double)(cur.w[u] * v[k]);
             close_slices(r + u + 1);
         }
         cur = nxt;
@@ -1113,70 +1118,6 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
         for (int ss = 0; ss < 4; ++ss) o[4 * j1 + ss] = scale * acc[1][ss];
 }
 
-// Polar ring pairs (npr = 4i < 4 nside): direct DFT of the min(L, npr/2)+1
-// distinct coefficients, two maps and both rings per work item.
-__global__ __launch_bounds__(256) void k_phi_polar(const double* __restrict__ delta, int64_t npix, int32_t nq,
-                                                   int64_t M2, int32_t K, int32_t L,
-                                                   const RingDev* __restrict__ rings, const double2* __restrict__ tw,
-                                                   double* __restrict__ X, int32_t TQ,
-                                                   const uint32_t* __restrict__ perm)
-{
-    extern __shared__ double2 ps[];
-    const int k = blockIdx.x;
-    const int q0 = blockIdx.y * TQ;
-    const RingDev R = rings[k];
-    const int npr = R.npr, n2 = 2 * npr, ld = npr + 1;
-    double2* twl = ps;                          // 2 npr twiddles
-    double* dN = (double*)(ps + n2);            // [TQ][ld]
-    double* dS = dN + TQ * ld;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) twl[e] = tw[R.tw_off + e];
-    for (int e = threadIdx.x; e < TQ * npr; e += blockDim.x) {
-        const int ql = e / npr, j = e - ql * npr;
-        const int q = q0 + ql;
-        double vn = 0.0, vs = 0.0;
-        if (q < nq) {
-            const int64_t pn = R.start_n + j, ps = R.start_s + j;  // pixels (slots via perm)
-            vn = delta[(int64_t)q * npix + (perm ? (int64_t)perm[pn] : pn)];
-            vs = delta[(int64_t)q * npix + (perm ? (int64_t)perm[ps] : ps)];
-        }
-        dN[ql * ld + j] = vn;
-        dS[ql * ld + j] = vs;
-    }
-    __syncthreads();
-    const int nu = min(L, npr / 2) + 1;
-    const int G = TQ / 2;
-    for (int e = threadIdx.x; e < nu * G; e += blockDim.x) {
-        const int kf = e / G, g = e - kf * G;  // map pairs fastest: contiguous X stores
-        const int qa = 2 * g;
-        double a0r = 0, a0i = 0, b0r = 0, b0i = 0, a1r = 0, a1i = 0, b1r = 0, b1i = 0;
-        const double* n0 = dN + qa * ld;
-        const double* n1 = n0 + ld;
-        const double* s0 = dS + qa * ld;
-        const double* s1 = s0 + ld;
-        int idx = 0;
-        const int didx = (2 * kf) % n2;
-        for (int j = 0; j < npr; ++j) {
-            const double2 w = twl[idx];
-            const double c = w.x, sn = w.y;
-            const double x0 = n0[j], x1 = n1[j], y0 = s0[j], y1 = s1[j];
-            a0r = fma(x0, c, a0r);
-            a0i = fma(-x0, sn, a0i);
-            a1r = fma(x1, c, a1r);
-            a1i = fma(-x1, sn, a1i);
-            b0r = fma(y0, c, b0r);
-            b0i = fma(-y0, sn, b0i);
-            b1r = fma(y1, c, b1r);
-            b1i = fma(-y1, sn, b1i);
-            idx += didx;
-            if (idx >= n2) idx -= n2;
-        }
-        if (q0 + qa < nq)
-            store_folded<double>(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa, a0r, a0i, b0r, b0i);
-        if (q0 + qa + 1 < nq)
-            store_folded<double>(X, M2, K, k, false, npr, R.shift, twl, L, kf, q0 + qa + 1, a1r, a1i, b1r, b1i);
-    }
-}
-
 // Polar ring pairs (npr = 4i, i < nside): one radix-4 decimation-in-time step,
 // X_kf = sum_{s<4} W^{kf s} Y_s[kf mod i], W = e^{-2 pi i/npr}, with Y_s the length-i
 // DFT of the decimated samples delta[4j + s], computed directly for u <= i/2
@@ -1525,12 +1466,17 @@ __global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restric
 // cf32: C stored as float (FP64 MFMA); spl: split K with the second A FP64 (1) or
 // FP32 (2) (FP64 MFMA).  With f32, af32 and cf32 all set (the mixed path) B and B2 are
 // FP32 tables too: the MFMA rounds them to FP32 either way, so they take half the bytes.
+// x32: A and C stored as float, B as double, FP64 MFMA and accumulation (the mixed path's
+// first analysis: its sum over the ring pairs cancels to ~1/4 of its largest terms, which
+// an FP32 accumulation turned into errors up to 2.7e-6 of an (n,l) block at nside 128).
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
                                double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false,
-                               int spl = 0)
+                               int spl = 0, bool x32 = false)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
+    if (x32)
+        hipLaunchKernelGGL((k_gemm_tall<false, true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
         hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32)
         hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -1675,6 +1621,20 @@ __global__ void k_pack(const TA* __restrict__ a, const int64_t* __restrict__ aof
     if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
 }
 
+// The mixed path's FP64 monopoles: a_{n00} of map qbase[n] from row n of the FP64 transform
+// of the l = 0 maps (l = m = 0: block (0, even), index 0).
+__global__ void k_pack_l0(const double* __restrict__ a, int32_t nmax, const int32_t* __restrict__ qbase,
+                          const int64_t* __restrict__ out_base, double* __restrict__ out, int32_t* __restrict__ flag)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nmax) return;
+    const double re = a[2 * n], im = a[2 * n + 1];
+    const int64_t o = out_base[qbase[n]];
+    out[2 * o] = re;
+    out[2 * o + 1] = im;
+    if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
+}
+
 // Healpix-layout alm of the first nq maps: alm[q][m(2L+1-m)/2 + l].
 __global__ void k_pack_alm(const double* __restrict__ a, const int64_t* __restrict__ aoff, int32_t nq, int32_t L,
                            int64_t M2, double* __restrict__ out)
@@ -1743,10 +1703,10 @@ __global__ void k_unpack_cot(const double* __restrict__ cot, const int64_t* __re
 __global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                    int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                    double scale, double* __restrict__ out, int32_t TQ,
-                                                   const int32_t* __restrict__ mall)
+                                                   const int32_t* __restrict__ mall, int32_t k0)
 {
     extern __shared__ double2 gs[];  // [TQ][side][L+1]
-    const int k = blockIdx.x;
+    const int k = k0 + blockIdx.x;
     const int q0 = blockIdx.y * TQ;
     const RingDev R = rings[k];
     const int npr = R.npr;
@@ -2317,7 +2277,7 @@ struct sfb_plan {
     int32_t K = 0;       // ring pairs (northern rings incl. equator)
     int64_t M2 = 0;      // 2 * nmaps
     int32_t niter = 3, precision = 0, TQ = 8;
-    int32_t TQF = 1, TQF32 = 2, TQP = 2, logM = 1;  // phi-transform tiles (FFT FP64 / FP32, polar)
+    int32_t TQF = 1, TQF32 = 2, logM = 1;  // phi-transform FFT tiles (FP64 / FP32)
     int32_t fft_nt = 512, fft_nt32 = 256;           // FFT threads per block (FP64 / FP32)
     int32_t p4_tq = 64, p4_tq32 = 64;               // polar maps per block, at most (FP64 / FP32)
     double tune_ms = 0.0;                           // time the plan-time tile search took
@@ -2334,6 +2294,7 @@ struct sfb_plan {
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
     // work
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0;
+    DevBuf d_delta0;               // mixed path: FP64 delta of the l = 0 maps (monopoles)
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
     DevBuf d_desc_syn32;           // aliasing-ring synthesis with G as floats (mixed path)
@@ -2346,7 +2307,7 @@ struct sfb_plan {
     std::vector<int64_t> ic0_off;  // per (m, parity): offset of I - C0 (nb x nb)
     std::vector<int64_t> lamw_off;  // per (m, parity): offset of dOmega Lam[k < K_al] (K_al x nb)
     DevBuf d_lamw, d_desc_cal, d_desc_cal32;  // the merged C0 + residual-analysis Jacobi GEMM
-    DevBuf d_lam32, d_lamT32, d_ic032, d_lamw32;  // FP32 copies of the GEMM tables (mixed path)
+    DevBuf d_lamT32, d_ic032, d_lamw32;  // FP32 copies of the Jacobi GEMM tables (mixed path)
     double* a_final = nullptr;     // the a buffer holding the last result
     bool a_f32 = false;            // ... stored as floats (mixed path)
     // catalogue
@@ -2800,7 +2761,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         HIP_TRY(hipMemcpy(buf.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
         return SFB_OK;
     };
-    if ((rc = upload32(P->d_lam32, lam)) || (rc = upload32(P->d_lamT32, lamT)) || (rc = upload32(P->d_ic032, ic0)))
+    if ((rc = upload32(P->d_lamT32, lamT)) || (rc = upload32(P->d_ic032, ic0)))
         return rc;
     // dOmega Lam on the aliasing ring pairs: the residual-analysis half of the merged
     // Jacobi GEMM (its rows k < K_al of every (m, parity) block)
@@ -2860,9 +2821,9 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         d.ldb = n;
         d.ldc = P->M2;
         ana.push_back(d);
-        // mixed path: X0 holds floats, block b starts b K M2 floats in; a written as floats
+        // mixed path: X0 holds floats, block b starts b K M2 floats in; a written as floats;
+        // Lam stays FP64 (FP64 MFMA)
         d.A = (const double*)(P->d_X0.as<float>() + (b * (int64_t)P->K + kc) * P->M2);
-        d.B = (const double*)(P->d_lam32.as<float>() + P->lam_off[(size_t)b] + kc * n);
         d.C = fa(P->d_a, (size_t)b);
         ana32.push_back(d);
     }
@@ -2967,7 +2928,9 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
     if (!out) return fail(SFB_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (!lmax_n || !knot_off || !knot_x || !knot_y || !knot_ypp) return fail(SFB_ERR_ARG, "null table pointer");
-    if (nside < 1 || nside > 1024 || (nside & (nside - 1))) return fail(SFB_ERR_ARG, "nside must be a power of two <= 1024");
+    // nside <= 512: beyond, the largest polar rings of the phi-transform need blocks wider
+    // than the 256 work items and LDS its kernels are built for
+    if (nside < 1 || nside > 512 || (nside & (nside - 1))) return fail(SFB_ERR_ARG, "nside must be a power of two <= 512");
     if (lmax < 0 || nmax < 1 || niter < 0) return fail(SFB_ERR_ARG, "bad lmax/nmax/niter");
     if (lmax > 4 * nside)
         return fail(SFB_ERR_LMAX, "alm.lmax > 4*nside is a poor choice (lmax=%lld, nside=%lld)", (long long)lmax,
@@ -3017,9 +2980,6 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         while (P->TQF < 4 && 2 * P->TQF <= fit &&
                16 * (2 * 2 * P->TQF * fft_plen(M, 8) + 2 * M + 1) <= 49152)
             P->TQF *= 2;
-        // polar: 2 npr twiddles + 2 rings x TQP maps x (npr+1) doubles, npr <= 4 nside - 4
-        const int64_t per = 2 * (N4 + 1) * 8;
-        P->TQP = (int32_t)std::max<int64_t>(2, std::min<int64_t>(16, (budget - 2 * N4 * 16) / per) & ~1LL);
     }
     hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -3052,9 +3012,9 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_sr, &P->d_sw, &P->d_groups, &P->d_gx, &P->d_grh, &P->d_rec4, &P->d_qorder,
                       &P->d_it_i, &P->d_it_t, &P->d_sblk, &P->d_sblk32, &P->d_rec4f, &P->d_swf, &P->d_pinfo,
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
-                      &P->d_desc_cal32, &P->d_mall, &P->d_lam32, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
+                      &P->d_desc_cal32, &P->d_mall, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
-                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo};
+                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_delta0};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -3325,29 +3285,79 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
     return SFB_OK;
 }
 
-// delta (first nq maps) -> X0: FFT for the equatorial ring pairs, direct DFT for the polar ones.
-// The polar kernel's work items (2 TQ x ceil((i/2 + 1)/2) <= 256) and LDS limit it
-// to nside <= 1024; beyond, the direct-DFT kernel (FP64 only) runs.
-// k_phi_polar4 launch shape: maps per block up to TQMAX (each ring takes
-// min(TQMAX, 256 / (2 nup)) maps), the LDS of the largest block over the rings, and
-// the grid height for nq maps (set by the ring with the fewest maps per block)
-static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t nq = 1, int32_t* gy = nullptr)
+// k_phi_polar4 launch shape for the polar ring pairs of an nside (i = 1 .. nside - 1):
+// each ring takes min(tqmax, 256 / (2 nup)) maps per block (nup = ceil((i/2 + 1)/2) work
+// items per map and ring), so its largest ring (nup <= 128) holds one map; the LDS is that
+// of the largest block over the rings, tqmin the fewest maps per block.  Host arithmetic
+// only; false when a block does not fit.
+static bool polar4_shape(int64_t nside, int32_t tqmax, int64_t lds_max, size_t vbytes, size_t* shp, int64_t* tqmin_out)
 {
-    const int32_t tqmax = vbytes == 8 ? P->p4_tq : P->p4_tq32;  // maps per block, at most
-    const int64_t npr_max = 4 * (P->nside - 1);
+    const int64_t npr_max = 4 * (nside - 1);
     const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
+    *shp = 0;
+    *tqmin_out = tqmax;
     if (2 * nup_max > 256) return false;
     size_t sh = 0;
     int64_t tqmin = tqmax;
-    for (int64_t i = 1; i < P->nside; ++i) {
+    for (int64_t i = 1; i < nside; ++i) {
         const int64_t nup = ((i / 2 + 1) + 1) / 2;
         const int64_t tq = std::min<int64_t>(tqmax, 256 / (2 * nup));
         sh = std::max(sh, 2 * vbytes * (size_t)(i + 2) + vbytes * 2 * (size_t)tq * (size_t)(4 * i + 8));
         tqmin = std::min(tqmin, tq);
     }
     *shp = sh;
-    if (gy) *gy = (int32_t)((nq + tqmin - 1) / tqmin);
-    return (int64_t)sh <= P->lds_max;
+    *tqmin_out = tqmin;
+    return tqmin >= 1 && (int64_t)sh <= lds_max;
+}
+
+// The plan's k_phi_polar4 shape: maps per block up to its tile (FP64 / FP32) and the grid
+// height for nq maps (set by the ring with the fewest maps per block).  Every nside the
+// plan accepts (<= 512) fits.
+static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t nq = 1, int32_t* gy = nullptr)
+{
+    const int32_t tqmax = vbytes == 8 ? P->p4_tq : P->p4_tq32;
+    int64_t tqmin = 1;
+    const bool ok = polar4_shape(P->nside, tqmax, P->lds_max, vbytes, shp, &tqmin);
+    if (gy) *gy = (int32_t)((nq + tqmin - 1) / std::max<int64_t>(tqmin, 1));
+    return ok;
+}
+
+// Inverse (synthesis) shapes.  Equatorial ring pairs: k_phi_ifft with tqi maps per block
+// (0: its LDS does not fit and the direct sum takes them).  Polar ring pairs k < nip4 on
+// the radix-4 inverse k_phi_ipolar4, whose work item is a map, a ring and two outputs: a
+// ring with i > 256 (more than 128 output pairs) does not fit one 256-thread block, so
+// those ring pairs (and every polar one when the LDS does not fit) take k_phi_synth.
+struct InvShapes {
+    int32_t tqi;      // equatorial maps per k_phi_ifft block (0: none)
+    int32_t k_eq0;    // first ring pair on k_phi_ifft
+    int32_t nip4;     // polar ring pairs on k_phi_ipolar4 (0 .. nip4-1)
+    int64_t tqmin;    // fewest maps per k_phi_ipolar4 block
+    size_t shq, shi;  // LDS bytes of k_phi_ipolar4 and k_phi_ifft blocks
+};
+constexpr int32_t IP_TQMAX = 32;
+
+static InvShapes inv_shapes(int64_t nside, int64_t L, int64_t lds_max)
+{
+    InvShapes S{};
+    const int64_t N = 4 * nside, M = N / 2;
+    auto ifft_lds = [&](int32_t tq) {
+        return sizeof(double2) * (size_t)(2 * tq * fft_plen(M, sizeof(double)) + N + 1 + 2 * tq * (L + 1));
+    };
+    S.tqi = 2;
+    while (S.tqi > 0 && (int64_t)ifft_lds(S.tqi) > lds_max) --S.tqi;
+    S.shi = S.tqi > 0 ? ifft_lds(S.tqi) : 0;
+    S.k_eq0 = S.tqi > 0 ? (int32_t)(nside - 1) : (int32_t)(2 * nside);
+    const int64_t ilast = std::min<int64_t>(nside - 1, 256);  // polar rings i = 1 .. ilast (njp <= 128)
+    S.shq = 0;
+    S.tqmin = IP_TQMAX;
+    for (int64_t i = 1; i <= ilast; ++i) {
+        const int64_t njp = (i + 1) / 2, nu = i / 2 + 1;
+        const int64_t tq = std::min<int64_t>(IP_TQMAX, 256 / (2 * njp));
+        S.shq = std::max(S.shq, sizeof(double2) * (size_t)(i + 2 * tq * 4 * i + 2 * tq * 4 * nu));
+        S.tqmin = std::min(S.tqmin, tq);
+    }
+    S.nip4 = (ilast > 0 && S.tqmin >= 1 && (int64_t)S.shq <= lds_max) ? (int32_t)ilast : 0;
+    return S;
 }
 
 // delta (first nq maps) -> X0: FFT for the equatorial ring pairs, radix-4 split
@@ -3355,9 +3365,11 @@ static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t n
 // cut: X of ring pair k written for m <= mall[k] only (what the analysis reads);
 // otherwise for every m <= lmax.  Tile shapes (maps and threads per FFT block, maps per
 // polar block) come from the plan's tile search (autotune_phi); they change no arithmetic.
-static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool cut = true)
+static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool cut = true,
+                         const void* src = nullptr)
 {
     const int32_t* mall = cut ? P->d_mall.as<int32_t>() : nullptr;
+    const void* dsrc = src ? src : P->d_delta.p;  // the maps (P->d_delta unless given)
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int64_t nside = P->nside;
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
@@ -3373,39 +3385,30 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<float>, grid_f, dim3(P->fft_nt32), shf, st,
-                           P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           (const float*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL(k_phi_fft<double>, grid_f, dim3(P->fft_nt), shf, st,
-                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           (const double*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     }
     LAUNCH_CHECK();
     if (k_eq0 == 0) return SFB_OK;
     int32_t gy4;
     size_t shp;
-    if (!polar4_fits(P, vb, &shp, nq, &gy4)) {
-        if (f32) return fail(SFB_ERR_ARG, "internal: FP32 phi-transform beyond nside 1024");
-        const int64_t npr_max = 4 * (nside - 1);
-        size_t sho = sizeof(double2) * 2 * (size_t)npr_max + sizeof(double) * 2 * (size_t)P->TQP * (size_t)(npr_max + 1);
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sho));
-        hipLaunchKernelGGL(k_phi_polar, dim3((unsigned)k_eq0, nblk(nq, P->TQP)), dim3(256), sho, st,
-                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->TQP, P->delta_perm);
-        LAUNCH_CHECK();
-        return SFB_OK;
-    }
+    if (!polar4_fits(P, vb, &shp, nq, &gy4))
+        return fail(SFB_ERR_ARG, "internal: the polar phi-transform blocks do not fit in LDS");
     const dim3 grid_p((unsigned)(32 * (((int64_t)k_eq0 * gy4 + 31) / 32)));
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<float>, grid_p, dim3(256), shp, st,
-                           P->d_delta.as<float>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           (const float*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<float>(), P->p4_tq32, mall, P->delta_perm, k_eq0, gy4);
     } else {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
         hipLaunchKernelGGL(k_phi_polar4<double>, grid_p, dim3(256), shp, st,
-                           P->d_delta.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           (const double*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), P->p4_tq, mall, P->delta_perm, k_eq0, gy4);
     }
     LAUNCH_CHECK();
@@ -3587,76 +3590,82 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
 }
 
 // delta (first nq maps) -> a, with niter refinements.
-static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
+static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false, const void* src = nullptr)
 {
     const int32_t L = (int32_t)P->lmax;
     const double domega = 4.0 * kPi / (double)P->npix;
     {
         PhaseTimer timer(P, SFB_PHASE_PHI, st);
-        int rc = phi_transform(P, nq, st, f32);
+        int rc = phi_transform(P, nq, st, f32, true, src);
         if (rc) return rc;
     }
     {
         PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
         launch_gemm((f32 ? P->d_desc_ana32 : P->d_desc_ana).as<GemmDesc>(), (unsigned)(2 * (L + 1)), 2 * nq,
-                    P->nb[0], domega, 0.0, st, f32, f32, f32);
+                    P->nb[0], domega, 0.0, st, false, false, false, 0, f32);
         LAUNCH_CHECK();
     }
     return jacobi_stage(P, nq, st, f32);
 }
 
-// scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta: the radix-4 inverse
-// on the polar ring pairs, the inverse FFT on the equatorial ones (the direct sum
-// k_phi_synth wherever their blocks do not fit in LDS)
-// X0 (from the full synthesis, written for m <= mall[k] on ring pair k) -> maps
+// scale * (synthesis of nq maps' ring coefficients d_X0) -> d_delta, pixel order.  X0 comes
+// from the full synthesis, written for m <= mall[k] on ring pair k.  Polar ring pairs: the
+// radix-4 inverse for i <= 256, the direct sum k_phi_synth beyond (and wherever the radix-4
+// block does not fit in LDS); equatorial ones: the inverse FFT (the direct sum where its
+// block does not fit).
 static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
-    const int64_t N = 4 * P->nside, M = N / 2;
-    int32_t TQI = 2;
-    auto ifft_lds = [&](int32_t tq) {
-        return sizeof(double2) * (size_t)(2 * tq * fft_plen(M, sizeof(double)) + N + 1 + 2 * tq * (L + 1));
-    };
-    while (TQI > 0 && (int64_t)ifft_lds(TQI) > P->lds_max) --TQI;
-    const int32_t k_eq0 = TQI > 0 ? (int32_t)(P->nside - 1) : K;  // first ring pair on the FFT
-    // polar ring pairs by the radix-4 inverse when its largest block fits in LDS
-    constexpr int32_t IP_TQMAX = 32;
-    size_t shq = 0;
-    int64_t tqmin = IP_TQMAX;
-    for (int64_t i = 1; i < P->nside; ++i) {
-        const int64_t njp = (i + 1) / 2, nu = i / 2 + 1;
-        const int64_t tq = std::min<int64_t>(IP_TQMAX, 256 / (2 * njp));
-        shq = std::max(shq, sizeof(double2) * (size_t)(i + 2 * tq * 4 * i + 2 * tq * 4 * nu));
-        tqmin = std::min(tqmin, tq);
-    }
-    if (TQI > 0 && k_eq0 > 0 && (int64_t)shq <= P->lds_max) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shq));
-        const int32_t ntile = (int32_t)((nq + tqmin - 1) / tqmin);
-        const int64_t units = (int64_t)k_eq0 * ntile;
-        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), shq, st,
+    const InvShapes S = inv_shapes(P->nside, L, P->lds_max);
+    if (S.nip4 > 0) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.shq));
+        const int32_t ntile = (int32_t)((nq + S.tqmin - 1) / S.tqmin);
+        const int64_t units = (int64_t)S.nip4 * ntile;
+        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), S.shq, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX, P->d_mall.as<int32_t>(),
-                           k_eq0, ntile);
+                           S.nip4, ntile);
         LAUNCH_CHECK();
-    } else if (k_eq0 > 0) {
+    }
+    if (S.k_eq0 > S.nip4) {
         const int32_t TQS = 8;
         const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
-        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)k_eq0, nblk(nq, TQS)), dim3(256), shs, st,
+        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)(S.k_eq0 - S.nip4), nblk(nq, TQS)), dim3(256), shs, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), scale,
-                           P->d_delta.as<double>(), TQS, P->d_mall.as<int32_t>());
+                           P->d_delta.as<double>(), TQS, P->d_mall.as<int32_t>(), S.nip4);
         LAUNCH_CHECK();
     }
-    if (k_eq0 < K) {
-        const size_t shi = ifft_lds(TQI);
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shi));
-        const int64_t units = (int64_t)(K - k_eq0) * nblk(nq, TQI);
-        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(512), shi, st,
+    if (S.k_eq0 < K) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.shi));
+        const int64_t units = (int64_t)(K - S.k_eq0) * nblk(nq, S.tqi);
+        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(512), S.shi, st,
                            P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), TQI, k_eq0, P->logM,
+                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), S.tqi, S.k_eq0, P->logM,
                            P->d_mall.as<int32_t>());
         LAUNCH_CHECK();
     }
+    return SFB_OK;
+}
+
+// Launch shapes of the ring transforms for a geometry (host arithmetic, no device).
+extern "C" int sfb_ring_kernel_shapes(int64_t nside, int64_t lmax, int64_t lds_bytes, int64_t* out)
+{
+    sfb::clear_error();
+    if (!out || nside < 1 || lmax < 0 || lds_bytes < 1) return fail(SFB_ERR_ARG, "bad ring-shape arguments");
+    size_t sh4 = 0;
+    int64_t tq4 = 0;
+    const bool fwd = polar4_shape(nside, 64, lds_bytes, sizeof(double), &sh4, &tq4);
+    const InvShapes S = inv_shapes(nside, lmax, lds_bytes);
+    out[0] = fwd ? nside - 1 : 0;
+    out[1] = (int64_t)sh4;
+    out[2] = tq4;
+    out[3] = S.nip4;
+    out[4] = S.k_eq0 - S.nip4;
+    out[5] = (int64_t)S.shq;
+    out[6] = S.tqmin;
+    out[7] = S.tqi;
+    if (!fwd) return fail(SFB_ERR_ARG, "nside %lld: the polar phi-transform blocks do not fit", (long long)nside);
     return SFB_OK;
 }
 
@@ -3666,6 +3675,52 @@ static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, c
     if (!wa) return fail(SFB_ERR_ARG, "window array is NULL");
     if (win_kind == SFB_WIN_SEPARABLE && !wb) return fail(SFB_ERR_ARG, "separable window needs w_ln");
     (void)P;
+    return SFB_OK;
+}
+
+// Mixed path, l = 0 maps: their a_{n00} recomputed in FP64 and written over the mixed
+// values.  A monopole is a difference of nearly equal sums: the galaxy counts add up to N
+// exactly, so every per-pixel count fluctuation cancels in it and what remains is the
+// radial sampling noise, a fraction ~sigma_g/|mean g| of the per-pixel fluctuations
+// (and at times far smaller by chance).  FP32 rounding of the pixel sums, of the spline
+// records, of delta and of the ring sums is relative to those fluctuations, so in FP32 a
+// monopole carries errors up to ~1e-4 of itself (measured at nside 128, kmax 0.1), while
+// every l > 0 coefficient sees the fluctuations unreduced.  So for the nmax maps (n, 0):
+// the FP64 per-pixel scatter (the reference's arithmetic), then the FP64 phi-transform
+// and the FP64 Legendre analysis and Jacobi steps over those nmax maps, after the mixed
+// pipeline is done with the work buffers they reuse.
+static int monopole_fp64(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
+                         double* d_out, hipStream_t st)
+{
+    const int32_t nmx = (int32_t)P->nmax;
+    int rc;
+    if ((rc = P->d_delta0.ensure(sizeof(double) * (size_t)nmx * (size_t)P->npix))) return rc;
+    PhaseTimer timer(P, SFB_PHASE_MONOPOLE, st);
+    const bool prof = P->prof;
+    P->prof = false;  // the inner stages are timed as this one phase
+    const double domega = 4.0 * kPi / (double)P->npix;
+    const double inv = 1.0 / (nbar * domega);
+    const dim3 grid(nblk(P->npix, 256), (unsigned)nmx);
+    if (win_kind == SFB_WIN_DENSE)
+        hipLaunchKernelGGL((k_scatter<true, double>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
+                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta0.as<double>(),
+                           P->d_qbase.as<int32_t>());
+    else
+        hipLaunchKernelGGL((k_scatter<false, double>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
+                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta0.as<double>(),
+                           P->d_qbase.as<int32_t>());
+    rc = hipGetLastError() == hipSuccess ? SFB_OK : fail(SFB_ERR_DEVICE, "monopole scatter launch failed");
+    const uint32_t* perm = P->delta_perm;
+    P->delta_perm = nullptr;  // these maps are in pixel order
+    if (!rc) rc = sht_stage(P, nmx, st, false, P->d_delta0.p);
+    P->delta_perm = perm;
+    P->prof = prof;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_l0, dim3(nblk(nmx, 64)), dim3(64), 0, st, P->a_final, nmx, P->d_qbase.as<int32_t>(),
+                       P->d_outbase.as<int64_t>(), d_out, P->d_flag.as<int32_t>());
+    LAUNCH_CHECK();
     return SFB_OK;
 }
 
@@ -3699,6 +3754,7 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
                                P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
         LAUNCH_CHECK();
     }
+    if (f32 && (rc = monopole_fp64(P, nbar, win_kind, d_win_a, d_win_b, d_out, st))) return rc;
     int32_t flag[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -3823,7 +3879,11 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     hipLaunchKernelGGL(k_vjp_reduce, dim3(nblk(N, 256)), dim3(256), 0, st, N, ngroups, P->d_part.as<double2>(),
                        P->d_idx.as<uint32_t>(), P->d_ws.as<double>(), inv, d_dr, d_dw);
     LAUNCH_CHECK();
+    int32_t flag[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    // as the forward call: a non-finite angle has no pixel (floor(Int, NaN) throws)
+    if (flag[1]) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
     return SFB_OK;
 }
 

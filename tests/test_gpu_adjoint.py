@@ -129,3 +129,18 @@ def test_vjp_c2_radius_finite_differences(c2):
         # contract the forward difference with dL/da (L's own difference would cancel)
         fd = rdot(c, plan.cat2amln(cp, nbar, win) - plan.cat2amln(cm, nbar, win)) / (2 * h)
         assert dr[i] == pytest.approx(fd, rel=1e-4)
+
+
+def test_vjp_rejects_nonfinite_angle(small):
+    """As the forward call: a NaN angle has no pixel, so the adjoint raises too."""
+    am, win, cat, w, nbar, c = small
+    bad = cat.copy()
+    bad[1, 10] = np.nan
+    with pytest.raises(S.SFBError) as e:
+        S.cat2amln_vjp(bad, am, nbar, c, w)
+    assert "not finite" in str(e.value)
+    with pytest.raises(S.SFBError):
+        S.cat2amln(bad, am, nbar, win, w)
+    # the plan stays usable
+    dr, dw, _ = S.cat2amln_vjp(cat, am, nbar, c, w)
+    assert np.all(np.isfinite(dr)) and np.all(np.isfinite(dw))
