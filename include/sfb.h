@@ -220,7 +220,7 @@ int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const doub
 #define SFB_PHASE_RESIDUAL 5   /* ring-Fourier residual with aliasing (niter launches) */
 #define SFB_PHASE_PACK 6       /* a_nlm packing and the non-finite check */
 #define SFB_PHASE_SEGMENT 7    /* per knot group and galaxy: spline segment and t (shared by the group's maps) */
-#define SFB_PHASE_MONOPOLE 8   /* mixed precision: the l = 0 maps' scatter, phi-transform and Legendre steps in FP64 */
+#define SFB_PHASE_MONOPOLE 8   /* mixed precision: FP64 monopole correction of the l = 0 maps */
 #define SFB_NPHASE 9
 
 /* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and

@@ -224,20 +224,17 @@ __device__ __forceinline__ double spline_dev(const MapInfo& mi, const double* __
     return fma(t, fma(t, fma(t, s[2], s[3]), s[4]), s[5]);
 }
 
-// delta[q][p] for every map q (blockIdx.y) and pixel p.  With a map list, row y of
-// delta is map qlist[y] (the mixed path's FP64 monopole maps).
+// delta[q][p] for every map q (blockIdx.y) and pixel p.
 template <bool DENSE, typename R = double>
 __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __restrict__ pstart,
                                                  const double* __restrict__ r_s, const double* __restrict__ w_s,
                                                  const MapInfo* __restrict__ maps, const double* __restrict__ kx,
                                                  const double* __restrict__ seg, double inv,
                                                  const double* __restrict__ win_a, const double* __restrict__ win_b,
-                                                 int64_t lmax, int32_t q0, R* __restrict__ delta,
-                                                 const int32_t* __restrict__ qlist = nullptr)
+                                                 int64_t lmax, int32_t q0, R* __restrict__ delta)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t row = q0 + (int32_t)blockIdx.y;
-    const int32_t q = qlist ? qlist[blockIdx.y] : row;
+    int32_t q = q0 + (int32_t)blockIdx.y;
     if (p >= npix) return;
     const MapInfo mi = maps[q];
     double s = 0.0;
@@ -249,7 +246,98 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t npix, const int32_t* __
     double v = __dmul_rn(s, inv);
     int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
     double W = DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], win_b[col]);
-    delta[(int64_t)row * npix + p] = (R)__dsub_rn(v, W);
+    delta[(int64_t)q * npix + p] = (R)__dsub_rn(v, W);
+}
+
+// Mixed path, l = 0 maps: partial FP64 sums of w_i g_{n0}(r_i) over the galaxies, with the
+// reference's spline arithmetic (k_scatter's).  1-D grid of nchunk x nmx blocks, XCD-aware:
+// workgroup b runs on XCD b mod 8, and XCD x takes the galaxy chunks x, x + 8, ... of every
+// map, so a chunk's radii and weights are fetched into that XCD's L2 once for all maps.
+// The map's knots and (d, c, b, a) records go to LDS (40 bytes per knot); each thread sums a
+// strided share of the chunk, MONO_UL galaxies' loads in flight, then a fixed-order tree.
+constexpr int MONO_UL = 8;
+
+template <bool UNIT>
+__global__ __launch_bounds__(1024) void k_mono_sum(int64_t N, int32_t nchunk, const double* __restrict__ r_s,
+                                                   const double* __restrict__ w_s, const MapInfo* __restrict__ maps,
+                                                   const double* __restrict__ kx, const double4* __restrict__ rec4,
+                                                   const int32_t* __restrict__ qlist, double* __restrict__ part)
+{
+    extern __shared__ double mono_sm[];
+    const int b = blockIdx.x, loc = b >> 3, cpx = nchunk >> 3;  // nchunk: a multiple of 8
+    const int chunk = (b & 7) + 8 * (loc % cpx), mq = loc / cpx;
+    const int32_t q = qlist[mq];
+    const MapInfo mi = maps[q];
+    double* xs = mono_sm;                       // knots [nk]
+    double4* rs = (double4*)(mono_sm + mi.nk);  // records (d, c, b, a) [nk - 1]
+    for (int j = threadIdx.x; j < mi.nk; j += blockDim.x) xs[j] = kx[mi.koff + j];
+    for (int j = threadIdx.x; j + 1 < mi.nk; j += blockDim.x) rs[j] = rec4[mi.koff + q + j];
+    __syncthreads();
+    const int64_t per = (N + nchunk - 1) / nchunk;
+    const int64_t i0 = (int64_t)chunk * per, i1 = min(N, i0 + per);
+    double s = 0.0;
+    for (int64_t g0 = i0 + threadIdx.x; g0 < i1; g0 += MONO_UL * (int64_t)blockDim.x) {
+        double rr[MONO_UL], ww[MONO_UL];
+#pragma unroll
+        for (int u = 0; u < MONO_UL; ++u) {
+            const int64_t g = g0 + u * (int64_t)blockDim.x;
+            rr[u] = g < i1 ? r_s[g] : 0.0;
+            ww[u] = (UNIT || g >= i1) ? 1.0 : w_s[g];
+        }
+#pragma unroll
+        for (int u = 0; u < MONO_UL; ++u) {
+            if (g0 + u * (int64_t)blockDim.x >= i1) break;
+            const double r = rr[u];
+            double val;
+            if (r < mi.xfirst) val = mi.yfirst;
+            else if (r > mi.xlast) val = mi.ylast;
+            else if (!(r == r)) val = r;
+            else {
+                int i = (int)(__dmul_rn(__dsub_rn(r, mi.xfirst), mi.invdx));
+                i = min(max(i, 0), mi.nk - 2);
+                while (i > 0 && r < xs[i]) --i;
+                while (i < mi.nk - 2 && r >= xs[i + 1]) ++i;
+                const double4 c = rs[i];  // (d, c, b, a)
+                const double t = __ddiv_rn(__dsub_rn(r, xs[i]), __dsub_rn(xs[i + 1], xs[i]));
+                val = fma(t, fma(t, fma(t, c.w, c.z), c.y), c.x);
+            }
+            s = __dadd_rn(s, __dmul_rn(ww[u], val));
+        }
+    }
+    __syncthreads();  // the tables are no longer read: their LDS takes the tree
+    xs[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) xs[threadIdx.x] = __dadd_rn(xs[threadIdx.x], xs[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)mq * nchunk + chunk] = xs[0];
+}
+
+// Partial sums of W_{n0}(p) over pixel chunk blockIdx.x of l = 0 map qlist[blockIdx.y]: the
+// dense window's column, or the separable mask times w_ln[0, n] (fixed-order tree).
+template <bool DENSE>
+__global__ __launch_bounds__(256) void k_win_sum(int64_t npix, const MapInfo* __restrict__ maps,
+                                                 const int32_t* __restrict__ qlist, const double* __restrict__ win_a,
+                                                 const double* __restrict__ win_b, int64_t lmax,
+                                                 double* __restrict__ wpart)
+{
+    __shared__ double red[256];
+    const MapInfo mi = maps[qlist[blockIdx.y]];
+    const int64_t col = mi.l + (lmax + 1) * (int64_t)(mi.n - 1);
+    const double wb = DENSE ? 0.0 : win_b[col];
+    const int64_t per = (npix + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * per, p1 = min(npix, p0 + per);
+    double s = 0.0;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x)
+        s = __dadd_rn(s, DENSE ? win_a[p + npix * col] : __dmul_rn(win_a[p], wb));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = __dadd_rn(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) wpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1327,8 +1415,12 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = M::zero();
     constexpr int BLD = BNW * GT_BK / 256;  // B elements per thread per k-step
+    // A stored as FP32 under FP64 MFMA: the raw FP32 values stay in rf until the LDS store,
+    // so no conversion waits on the prefetch loads while the MFMAs run
+    constexpr bool AWIDEN = AF && !F32;
+    static_assert(!(AWIDEN && SPL), "FP32 A under FP64 MFMA has no split-K form");
     T ra[GT_ALD], rb[BLD];
-    float rf[GT_ALD];  // SPL 2: the second A's FP32 values, converted when stored to LDS
+    float rf[GT_ALD];  // SPL 2 (second A) or AWIDEN: FP32 values, converted when stored to LDS
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < GT_ALD; ++u) {
@@ -1343,7 +1435,9 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
                     rf[u] = (gk < d.K && gi < rows) ? ((gcfptr)d.A2)[o] : 0.0f;
                 else
                     ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A2)[o] : (T)0;
-            } else if (AF)
+            } else if (AWIDEN)
+                rf[u] = (gk < d.K && gi < rows) ? ((gcfptr)d.A)[(int64_t)gk * d.lda + gi] : 0.0f;
+            else if (AF)
                 ra[u] = (gk < d.K && gi < rows) ? (T)((gcfptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
             else
                 ra[u] = (gk < d.K && gi < rows) ? (T)((gcdptr)d.A)[(int64_t)gk * d.lda + gi] : (T)0;
@@ -1368,7 +1462,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
 #pragma unroll
         for (int u = 0; u < GT_ALD; ++u) {
             const int e = tid + 256 * u;
-            As[e >> 6][e & 63] = (SPL == 2 && !F32 && k0 + (e >> 6) >= d.K1) ? (T)rf[u] : ra[u];
+            As[e >> 6][e & 63] = (AWIDEN || (SPL == 2 && !F32 && k0 + (e >> 6) >= d.K1)) ? (T)rf[u] : ra[u];
         }
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
@@ -1621,15 +1715,28 @@ __global__ void k_pack(const TA* __restrict__ a, const int64_t* __restrict__ aof
     if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
 }
 
-// The mixed path's FP64 monopoles: a_{n00} of map qbase[n] from row n of the FP64 transform
-// of the l = 0 maps (l = m = 0: block (0, even), index 0).
-__global__ void k_pack_l0(const double* __restrict__ a, int32_t nmax, const int32_t* __restrict__ qbase,
-                          const int64_t* __restrict__ out_base, double* __restrict__ out, int32_t* __restrict__ flag)
+// Mixed path, l = 0 maps: a_{n00} = a32 + (b64 - b32), with b32 and a32 the FP32 pipeline's
+// first analysis and its Jacobi result for (l, m) = (0, 0) (block (0, even), index 0, row
+// 2q) and b64 = dOmega Y_00 sum_p delta(p) = dOmega Y_00 (inv sum_i w_i g(r_i) - sum_p W(p))
+// in FP64 (lambda_00 = Y_00 on every ring, so the first analysis of the monopole is the
+// full-sky sum).  The Jacobi steps map b to a linearly and leave the monopole itself
+// unchanged ((A S)_{00,00} = 1): the FP32 error of b_00 passes to a_00 one to one and is
+// replaced here, while the steps' own FP32 rounding stays of the order of the other modes'.
+__global__ void k_mono_fix(int32_t nmx, const int32_t* __restrict__ qbase, const int64_t* __restrict__ out_base,
+                           const double* __restrict__ part, int32_t nchunk, const double* __restrict__ wpart,
+                           int32_t nwchunk, double inv, double c00, const float* __restrict__ a32,
+                           const float* __restrict__ b32, double* __restrict__ out, int32_t* __restrict__ flag)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nmax) return;
-    const double re = a[2 * n], im = a[2 * n + 1];
-    const int64_t o = out_base[qbase[n]];
+    if (n >= nmx) return;
+    double S = 0.0, Wsum = 0.0;
+    for (int c = 0; c < nchunk; ++c) S += part[(int64_t)n * nchunk + c];
+    for (int c = 0; c < nwchunk; ++c) Wsum += wpart[(int64_t)n * nwchunk + c];
+    const double b64 = c00 * (inv * S - Wsum);
+    const int q = qbase[n];
+    const double re = (double)a32[2 * q] + (b64 - (double)b32[2 * q]);
+    const double im = (double)a32[2 * q + 1] - (double)b32[2 * q + 1];
+    const int64_t o = out_base[q];
     out[2 * o] = re;
     out[2 * o + 1] = im;
     if (!isfinite(re) || !isfinite(im)) atomicOr(flag, 1);
@@ -2294,7 +2401,7 @@ struct sfb_plan {
     DevBuf d_maps, d_kx, d_seg, d_rings, d_tw, d_lam, d_lamT, d_outbase, d_aoff, d_desc_ana, d_desc_syn, d_flag;
     // work
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0;
-    DevBuf d_delta0;               // mixed path: FP64 delta of the l = 0 maps (monopoles)
+    DevBuf d_mono;                 // mixed path: monopole partial sums and window sums
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
     DevBuf d_desc_syn32;           // aliasing-ring synthesis with G as floats (mixed path)
@@ -3014,7 +3121,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
                       &P->d_desc_cal32, &P->d_mall, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
-                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_delta0};
+                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -3678,48 +3785,49 @@ static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, c
     return SFB_OK;
 }
 
-// Mixed path, l = 0 maps: their a_{n00} recomputed in FP64 and written over the mixed
-// values.  A monopole is a difference of nearly equal sums: the galaxy counts add up to N
-// exactly, so every per-pixel count fluctuation cancels in it and what remains is the
-// radial sampling noise, a fraction ~sigma_g/|mean g| of the per-pixel fluctuations
-// (and at times far smaller by chance).  FP32 rounding of the pixel sums, of the spline
-// records, of delta and of the ring sums is relative to those fluctuations, so in FP32 a
-// monopole carries errors up to ~1e-4 of itself (measured at nside 128, kmax 0.1), while
-// every l > 0 coefficient sees the fluctuations unreduced.  So for the nmax maps (n, 0):
-// the FP64 per-pixel scatter (the reference's arithmetic), then the FP64 phi-transform
-// and the FP64 Legendre analysis and Jacobi steps over those nmax maps, after the mixed
-// pipeline is done with the work buffers they reuse.
-static int monopole_fp64(sfb_plan* P, double nbar, int32_t win_kind, const double* d_wa, const double* d_wb,
-                         double* d_out, hipStream_t st)
+// Mixed path, l = 0 maps: their a_{n00} corrected to FP64 quality (k_mono_fix).  A monopole
+// is a difference of nearly equal sums: the galaxy counts add up to N exactly, so every
+// per-pixel count fluctuation cancels in it and what remains is the radial sampling noise,
+// a fraction ~sigma_g/|mean g| of the per-pixel fluctuations (and at times far smaller by
+// chance).  FP32 rounding of the pixel sums, the spline records, delta and the ring sums is
+// relative to those fluctuations, so in FP32 a monopole carries errors up to ~1e-4 of
+// itself (measured at nside 128, kmax 0.1), while every l > 0 coefficient sees the
+// fluctuations unreduced.  The full-sky sum of delta is exact in FP64 from the galaxies
+// directly: inv sum_i w_i g(r_i) - sum_p W(p).
+static int monopole_fix(sfb_plan* P, int64_t N, double nbar, int32_t win_kind, const double* d_wa,
+                        const double* d_wb, double* d_out, hipStream_t st)
 {
     const int32_t nmx = (int32_t)P->nmax;
+    constexpr int32_t NCH = 32, NWC = 16;  // galaxy chunks (a multiple of 8) and pixel chunks per map
+    const size_t shm = std::max<size_t>(40 * (size_t)P->tbl_cap, 8 * 1024);
+    if ((int64_t)shm > P->lds_max) return fail(SFB_ERR_ARG, "internal: monopole spline tables exceed LDS");
     int rc;
-    if ((rc = P->d_delta0.ensure(sizeof(double) * (size_t)nmx * (size_t)P->npix))) return rc;
+    if ((rc = P->d_mono.ensure(sizeof(double) * (size_t)nmx * (NCH + NWC)))) return rc;
     PhaseTimer timer(P, SFB_PHASE_MONOPOLE, st);
-    const bool prof = P->prof;
-    P->prof = false;  // the inner stages are timed as this one phase
+    double* part = P->d_mono.as<double>();
+    double* wpart = part + (size_t)nmx * NCH;
+    auto go = [&](auto kern) -> int {
+        HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        hipLaunchKernelGGL(kern, dim3((unsigned)(NCH * nmx)), dim3(1024), shm, st, N, NCH, P->d_rs.as<double>(),
+                           P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
+                           P->d_rec4.as<double4>(), P->d_qbase.as<int32_t>(), part);
+        LAUNCH_CHECK();
+        return SFB_OK;
+    };
+    if ((rc = P->unit_w ? go(k_mono_sum<true>) : go(k_mono_sum<false>))) return rc;
+    if (win_kind == SFB_WIN_DENSE)
+        hipLaunchKernelGGL(k_win_sum<true>, dim3(NWC, (unsigned)nmx), dim3(256), 0, st, P->npix,
+                           P->d_maps.as<MapInfo>(), P->d_qbase.as<int32_t>(), d_wa, d_wb, P->lmax, wpart);
+    else
+        hipLaunchKernelGGL(k_win_sum<false>, dim3(NWC, (unsigned)nmx), dim3(256), 0, st, P->npix,
+                           P->d_maps.as<MapInfo>(), P->d_qbase.as<int32_t>(), d_wa, d_wb, P->lmax, wpart);
+    LAUNCH_CHECK();
     const double domega = 4.0 * kPi / (double)P->npix;
     const double inv = 1.0 / (nbar * domega);
-    const dim3 grid(nblk(P->npix, 256), (unsigned)nmx);
-    if (win_kind == SFB_WIN_DENSE)
-        hipLaunchKernelGGL((k_scatter<true, double>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
-                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
-                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta0.as<double>(),
-                           P->d_qbase.as<int32_t>());
-    else
-        hipLaunchKernelGGL((k_scatter<false, double>), grid, dim3(256), 0, st, P->npix, P->d_pstart.as<int32_t>(),
-                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_maps.as<MapInfo>(), P->d_kx.as<double>(),
-                           P->d_seg.as<double>(), inv, d_wa, d_wb, P->lmax, 0, P->d_delta0.as<double>(),
-                           P->d_qbase.as<int32_t>());
-    rc = hipGetLastError() == hipSuccess ? SFB_OK : fail(SFB_ERR_DEVICE, "monopole scatter launch failed");
-    const uint32_t* perm = P->delta_perm;
-    P->delta_perm = nullptr;  // these maps are in pixel order
-    if (!rc) rc = sht_stage(P, nmx, st, false, P->d_delta0.p);
-    P->delta_perm = perm;
-    P->prof = prof;
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pack_l0, dim3(nblk(nmx, 64)), dim3(64), 0, st, P->a_final, nmx, P->d_qbase.as<int32_t>(),
-                       P->d_outbase.as<int64_t>(), d_out, P->d_flag.as<int32_t>());
+    const double c00 = domega / std::sqrt(4.0 * kPi);  // dOmega Y_00
+    hipLaunchKernelGGL(k_mono_fix, dim3(nblk(nmx, 64)), dim3(64), 0, st, nmx, P->d_qbase.as<int32_t>(),
+                       P->d_outbase.as<int64_t>(), part, NCH, wpart, NWC, inv, c00, (const float*)P->a_final,
+                       P->d_a.as<float>(), d_out, P->d_flag.as<int32_t>());
     LAUNCH_CHECK();
     return SFB_OK;
 }
@@ -3754,7 +3862,7 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
                                P->d_outbase.as<int64_t>(), (int32_t)P->nmaps, P->M2, d_out, P->d_flag.as<int32_t>());
         LAUNCH_CHECK();
     }
-    if (f32 && (rc = monopole_fp64(P, nbar, win_kind, d_win_a, d_win_b, d_out, st))) return rc;
+    if (f32 && (rc = monopole_fix(P, N, nbar, win_kind, d_win_a, d_win_b, d_out, st))) return rc;
     int32_t flag[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

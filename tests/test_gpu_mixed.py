@@ -2,10 +2,10 @@
 
 The path: FP32 spline values with FP64 pixel sums, FP32 delta maps and phi-transform,
 the first Legendre analysis on FP64 MFMA over FP32 ring coefficients, FP32 MFMA for the
-Jacobi corrections, and the l = 0 maps (whose monopoles cancel to the radial sampling
-noise) entirely in FP64.  The bar is the one the path is built to: for every (n, l) block
-of a_nlm, max_m |a - a_ref| <= 1e-6 x the block's RMS |a_ref|; the FP64 monopoles agree
-with the FP64 path to rounding.
+Jacobi corrections, and the monopoles of the l = 0 maps (which cancel to the radial
+sampling noise) corrected with their FP64 full-sky sums.  The bar is the one the path is
+built to: for every (n, l) block of a_nlm, max_m |a - a_ref| <= 1e-6 x the block's RMS
+|a_ref|; the corrected monopoles are held to 1e-7 (uncorrected they miss by up to 1e-4).
 """
 import numpy as np
 import pytest
@@ -72,7 +72,8 @@ def test_mixed_c1_against_oracle():
         say("C1 mixed", "weights" if weights is not None else "unit", "worst blocks", worst(e))
         assert max(e.values()) <= BLOCK_BAR
         mono = [v for (n, l), v in e.items() if l == 0]
-        assert max(mono) <= 1e-10  # the FP64 monopoles
+        say("   l = 0 blocks (FP64-corrected monopoles): worst", max(mono))
+        assert max(mono) <= 1e-7
 
 
 @pytest.fixture(scope="module")
@@ -95,7 +96,9 @@ def test_mixed_c2_every_block_against_fp64_gpu(c2):
     say("C2 mixed vs FP64 GPU, worst blocks", worst(e, 5))
     assert len(e) == am.nmaps
     assert max(e.values()) <= BLOCK_BAR
-    assert max(v for (n, l), v in e.items() if l == 0) <= 1e-12
+    mono = max(v for (n, l), v in e.items() if l == 0)
+    say("   l = 0 blocks (FP64-corrected monopoles): worst", mono)
+    assert mono <= 1e-7
 
 
 def test_mixed_c2_sample_against_oracle(c2):
