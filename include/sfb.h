@@ -157,6 +157,18 @@ int sfb_apply_window(int32_t device, int64_t nside, int64_t nr, double rmin, dou
                      const double* win, int64_t N, const double* rtp, const double* u, double* rtp_out,
                      int64_t* n_out);
 
+/* apply_window (as sfb_apply_window) with every array in device memory and no allocation:
+ * no counterpart in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP-graph capture, tight serving loops).  Its scratch is one static device
+ * array of about 16.1 MiB per GPU, held until the process ends, so N is at most 4194304
+ * (2^22; SFB_ERR_ARG beyond).  The window maximum is taken on the device.  Enqueued on
+ * stream (NULL = the default stream) and not synchronised: *d_n_out (device memory) and
+ * d_rtp_out are valid once the stream has reached this point.  Calls that overlap on one
+ * GPU share the scratch and race. */
+int sfb_apply_window_noalloc(int32_t device, int64_t nside, int64_t nr, double rmin, double rmax, double dr,
+                             const double* d_win, int64_t N, const double* d_rtp, const double* d_u,
+                             double* d_rtp_out, int64_t* d_n_out, void* stream);
+
 /* amln2clnn(anlm1, anlm2, cmodes) (src/SphericalFourierBesselDecompositions.jl:130-150)
  * for the (l, n1, n2) triples lnn[3 x nlnn] of a ClnnModes over the plan's modes:
  * C = (Re(a1_l0 a2_l0) + 2 sum_{m>=1} Re(a1_lm conj(a2_lm))) / (2l + 1). */

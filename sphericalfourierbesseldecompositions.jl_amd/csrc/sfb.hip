@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2273,12 +2274,14 @@ __global__ void k_galweights(int64_t N, const double* __restrict__ rtp, int64_t 
 }
 
 // apply_window's acceptance: keep[i] = (rmin <= r <= rmax) && u[i] <= win[idx_r, pix] / max(win)
+// (1/max(win) given, or read from d_oo when that is not NULL)
 __global__ void k_window_keep(int64_t N, const double* __restrict__ rtp, const double* __restrict__ u, int64_t nside,
                               int64_t nr, double rmin, double rmax, double dr, const double* __restrict__ win,
-                              double ooWmax, int32_t* __restrict__ keep)
+                              double ooWmax, int32_t* __restrict__ keep, const double* __restrict__ d_oo = nullptr)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (d_oo) ooWmax = *d_oo;
     const double r = rtp[3 * i], th = rtp[3 * i + 1], ph = rtp[3 * i + 2];
     int32_t k = 0;
     if (rmin <= r && r <= rmax && isfinite(th) && isfinite(ph)) {
@@ -2291,6 +2294,32 @@ __global__ void k_window_keep(int64_t N, const double* __restrict__ rtp, const d
     }
     keep[i] = k;
 }
+
+// maximum(win) over n values: partial maxima per block (NaN entries skipped, as a running
+// std::max from -inf skips them), then 1/max in one thread
+__global__ __launch_bounds__(256) void k_max_part(int64_t n, const double* __restrict__ v, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    double m = -INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = v[i] > m ? v[i] : m;
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = red[threadIdx.x + o] > red[threadIdx.x] ? red[threadIdx.x + o] : red[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_max_inv(int32_t nb, const double* __restrict__ part, double* __restrict__ oo)
+{
+    double m = -INFINITY;
+    for (int b = 0; b < nb; ++b) m = part[b] > m ? part[b] : m;
+    *oo = __ddiv_rn(1.0, m);
+}
+
+__global__ void k_copy_i64(const int64_t* __restrict__ src, int64_t* __restrict__ dst) { *dst = *src; }
 
 // stable compaction of the kept galaxies: count per block of 1024, scan, scatter
 __global__ __launch_bounds__(1024) void k_keep_count(int64_t N, const int32_t* __restrict__ keep,
@@ -4391,6 +4420,84 @@ extern "C" int sfb_apply_window(int32_t device, int64_t nside, int64_t nr, doubl
     HIP_TRY(hipMemcpy(&kept, dof.as<int64_t>() + nb, 8, hipMemcpyDeviceToHost));
     if (kept > 0) HIP_TRY(hipMemcpy(rtp_out, dout.p, 24 * (size_t)kept, hipMemcpyDeviceToHost));
     *n_out = kept;
+    return SFB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// apply_window without allocation.  Its scratch (keep flags, block counts and offsets, the
+// window-maximum partials) is one file-scope device array of NOALLOC_BYTES, per GPU, held
+// until the process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (kept per device), never through the array's name.  Every call
+// writes each scratch word it reads before reading it (no counters or atomics accumulate
+// in it), so nothing is cleared between calls.  Calls on one GPU share the array:
+// overlapping calls (different streams or threads, same device) race.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_N = (int64_t)1 << 22;          // galaxies per call, at most
+constexpr int64_t NOALLOC_MAX_NB = NOALLOC_MAX_N / 1024;     // compaction blocks of 1024
+constexpr int32_t NOALLOC_WBLK = 1024;                       // window-maximum partial blocks
+constexpr size_t NOALLOC_KEEP = 0;                           // int32 keep[N]
+constexpr size_t NOALLOC_CNT = NOALLOC_KEEP + 4 * (size_t)NOALLOC_MAX_N;  // int64 cnt[nb]
+constexpr size_t NOALLOC_OFF = NOALLOC_CNT + 8 * (size_t)NOALLOC_MAX_NB;  // int64 off[nb + 1]
+constexpr size_t NOALLOC_WMX = NOALLOC_OFF + 8 * (size_t)(NOALLOC_MAX_NB + 8);  // double part[WBLK], 1/max
+constexpr size_t NOALLOC_BYTES = NOALLOC_WMX + 8 * (size_t)(NOALLOC_WBLK + 1);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[256] = {};
+
+// The device address of scratch on `device` (which must be current), looked up once per device.
+static int scratch_address(int device, unsigned char** out)
+{
+    if (device < 0 || device >= 256) return fail(SFB_ERR_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(SFB_ERR_DEVICE, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch_ptr[device] = p;
+    }
+    *out = (unsigned char*)g_scratch_ptr[device];
+    return SFB_OK;
+}
+
+extern "C" int sfb_apply_window_noalloc(int32_t device, int64_t nside, int64_t nr, double rmin, double rmax,
+                                        double dr, const double* d_win, int64_t N, const double* d_rtp,
+                                        const double* d_u, double* d_rtp_out, int64_t* d_n_out, void* stream)
+{
+    sfb::clear_error();
+    if (nside < 1 || nr < 1 || !d_win || N < 0 || !d_n_out || (N > 0 && (!d_rtp || !d_u || !d_rtp_out)))
+        return fail(SFB_ERR_ARG, "bad apply_window_noalloc arguments");
+    if (N > NOALLOC_MAX_N)
+        return fail(SFB_ERR_ARG, "apply_window_noalloc: N = %lld exceeds its scratch (at most %lld galaxies)",
+                    (long long)N, (long long)NOALLOC_MAX_N);
+    HIP_TRY(hipSetDevice(device));
+    unsigned char* scr = nullptr;
+    int rc;
+    if ((rc = scratch_address(device, &scr))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    int32_t* keep = (int32_t*)(scr + NOALLOC_KEEP);
+    int64_t* cnt = (int64_t*)(scr + NOALLOC_CNT);
+    int64_t* off = (int64_t*)(scr + NOALLOC_OFF);
+    double* wpart = (double*)(scr + NOALLOC_WMX);
+    double* oo = wpart + NOALLOC_WBLK;
+    const int64_t nwin = nr * 12 * nside * nside;
+    const int32_t wb = (int32_t)std::min<int64_t>(NOALLOC_WBLK, nblk(nwin, 256));
+    hipLaunchKernelGGL(k_max_part, dim3((unsigned)wb), dim3(256), 0, st, nwin, d_win, wpart);
+    hipLaunchKernelGGL(k_max_inv, dim3(1), dim3(1), 0, st, wb, wpart, oo);
+    LAUNCH_CHECK();
+    const int64_t nb = (N + 1023) / 1024;
+    if (N > 0) {
+        hipLaunchKernelGGL(k_window_keep, dim3(nblk(N, 256)), dim3(256), 0, st, N, d_rtp, d_u, nside, nr, rmin, rmax,
+                           dr, d_win, 0.0, keep, (const double*)oo);
+        hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nb), dim3(1024), 0, st, N, keep, cnt);
+        LAUNCH_CHECK();
+    }
+    // nb = 0 leaves off[0] = 0 (the scan of no blocks)
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, (int32_t)nb, cnt, off);
+    if (N > 0)
+        hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nb), dim3(1024), 0, st, N, keep, off, d_rtp, d_rtp_out);
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(1), 0, st, off + nb, d_n_out);
+    LAUNCH_CHECK();
     return SFB_OK;
 }
 
