@@ -20,6 +20,10 @@
  *    of the last error on the calling thread is returned by sfb_last_error().
  *  - A plan may be used by one host thread at a time; different plans may run
  *    concurrently.  Calls are synchronous at return unless named _device.
+ *  - Fault injection for error-path tests: the environment variable SFB_FAULT=alloc
+ *    makes every device allocation fail (SFB_ERR_DEVICE), SFB_FAULT=maxevals gives the
+ *    k_nl root finder no evaluations (SFB_ERR_SETUP, as the reference's doublesecant_method
+ *    returning NaN after maxevals).  Read at each use; unset, nothing changes.
  */
 #ifndef SFB_MI355X_H
 #define SFB_MI355X_H
@@ -39,9 +43,11 @@ extern "C" {
 #define SFB_ERR_NOMEM 6      /* host allocation failed */
 
 #define SFB_PREC_FP64 0      /* plain path: reference method, FP64 throughout */
-#define SFB_PREC_MIXED 1     /* FP32 spline values, delta maps, phi-transform and every Legendre
-                                pass (FP32 MFMA; FP32 iterate and aliasing buffers); FP64 setup,
-                                segment search, lambda / C0 tables, GEMM epilogues and aliasing sums */
+#define SFB_PREC_MIXED 1     /* FP32 spline values, delta maps and phi-transform; the first Legendre
+                                analysis on FP64 MFMA over FP32 ring coefficients, the Jacobi passes
+                                on FP32 MFMA (FP32 iterate and aliasing buffers); FP64 setup, segment
+                                search, pixel sums, lambda tables, GEMM epilogues, aliasing sums and
+                                the l = 0 maps' monopoles.  Every (n,l) block within 1e-6 of FP64. */
 
 #define SFB_WIN_DENSE 0      /* win_a = W[npix, lmax+1, nmax] */
 #define SFB_WIN_SEPARABLE 1  /* win_a = mask[npix], win_b = w_ln[lmax+1, nmax] */

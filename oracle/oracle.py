@@ -65,11 +65,24 @@ def lib():
         L.orc_alm2map.argtypes = [i64, i64, P, P]
         L.orc_delta_map.restype = ctypes.c_int
         L.orc_delta_map.argtypes = [i64, i64, i64, P, P, P, P, P, i64, P, P, dbl, i32, P, P, i64, P]
+        L.orc_markstein_check.restype = i64
+        L.orc_markstein_check.argtypes = [i64, P, P, i64, ctypes.c_uint64, i32, P, P]
         L.orc_cat2amln.restype = ctypes.c_int
         L.orc_cat2amln.argtypes = [i64, i64, i64, P, P, P, P, P, i64, P, P, dbl, i32, P, P, i32,
                                    i64, i64, i32, P]
         _lib = L
     return _lib
+
+
+def markstein_check(x0, x1, nr=64, seed=0, variant=0):
+    """Mismatches of the divide-free t against IEEE division over the segments (x0, x1)
+    (variant 1: without the correction step, the check's control)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    x1 = np.ascontiguousarray(x1, dtype=np.float64)
+    b0, br = ctypes.c_double(), ctypes.c_double()
+    n = lib().orc_markstein_check(x0.size, _p(x0), _p(x1), int(nr), int(seed), int(variant), ctypes.byref(b0),
+                                 ctypes.byref(br))
+    return int(n), (b0.value, br.value)
 
 
 def _p(a):

@@ -623,3 +623,57 @@ int orc_cat2amln(int64_t nside, int64_t lmax, int64_t nmax, const int64_t* lmax_
     catalog_free(&C);
     return bad ? -3 : 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Check of the product's divide-free spline parameter (not a restatement):    */
+/* the scatter forms t = (r - x0)/h as y = RN(1/h), q0 = RN(n y),              */
+/* t = RN(q0 + RN(n - h q0) y) with n = r - x0 (Markstein's final correction). */
+/* Compared with IEEE division n/h for segments (x0[i], x1[i]): per segment    */
+/* x0, its successor, the midpoint, the last radius below x1 and nr - 4 radii  */
+/* drawn uniformly (splitmix64).  fma() is the correctly rounded C99 fma.      */
+/* variant 1 drops the correction (t = q0): the check's own control, which    */
+/* must find mismatches.  Returns the number of mismatches (one of them in     */
+/* *bad_x0, *bad_r).                                                          */
+/* ------------------------------------------------------------------------- */
+static uint64_t orc_splitmix(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int64_t orc_markstein_check(int64_t nseg, const double* x0s, const double* x1s, int64_t nr, uint64_t seed,
+                            int32_t variant, double* bad_x0, double* bad_r)
+{
+    int64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int64_t i = 0; i < nseg; i++) {
+        const double x0 = x0s[i], x1 = x1s[i];
+        const double h = x1 - x0, y = 1.0 / h;
+        uint64_t s = seed ^ (uint64_t)i * 0xD1B54A32D192ED03ull;
+        for (int64_t k = 0; k < nr; k++) {
+            double r;
+            if (k == 0) r = x0;
+            else if (k == 1) r = nextafter(x0, INFINITY);
+            else if (k == 2) r = x0 + 0.5 * h;
+            else if (k == 3) r = nextafter(x1, -INFINITY);
+            else r = x0 + (double)(orc_splitmix(&s) >> 11) * 0x1.0p-53 * h;
+            if (!(r >= x0 && r < x1)) continue;
+            const double n = r - x0;
+            const double q0 = n * y;
+            const double t = variant == 1 ? q0 : fma(fma(-h, q0, n), y, q0);
+            if (t != n / h) {
+#pragma omp critical
+                {
+                    if (bad_x0 && bad_r) {
+                        *bad_x0 = x0;
+                        *bad_r = r;
+                    }
+                }
+                bad++;
+            }
+        }
+    }
+    return bad;
+}

@@ -29,6 +29,12 @@ namespace sfb {
 
 static thread_local std::string g_last_error;
 
+bool fault(const char* what)
+{
+    const char* v = std::getenv("SFB_FAULT");
+    return v && std::strcmp(v, what) == 0;
+}
+
 int fail(int code, const char* fmt, ...)
 {
     char buf[1024];
@@ -202,7 +208,7 @@ static double next_zero(const ZeroFn& fn, double x, double delta)
         fnew = fn(xnew);
         if (++guard > 10000000) return NAN;
     }
-    return doublesecant(fn, x, xnew, 1000, 1e-10);
+    return doublesecant(fn, x, xnew, sfb::fault("maxevals") ? 0 : 1000, 1e-10);
 }
 
 // ---------------------------------------------------------------------------

@@ -2389,6 +2389,7 @@ struct DevBuf {
             bytes = 0;
         }
         size_t want = need ? need : 16;
+        if (sfb::fault("alloc")) return fail(SFB_ERR_DEVICE, "hipMalloc(%zu) failed: forced by SFB_FAULT=alloc", want);
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) {
             p = nullptr;

@@ -37,6 +37,12 @@ int spline_build(int64_t n, const double* x, const double* y, double* ypp);
 double spline_eval(int64_t n, const double* x, const double* y, const double* ypp, double r);
 double spline_deriv(int64_t n, const double* x, const double* y, const double* ypp, double r);
 
+// Fault injection for the error-path tests, from the environment variable SFB_FAULT:
+// "alloc" makes every device allocation fail (SFB_ERR_DEVICE), "maxevals" gives the k_nl
+// root finder no evaluations (the reference's doublesecant_method returns NaN after
+// maxevals, src/SphericalBesselGNLs.jl:126-127).  Read at each use; unset, nothing changes.
+bool fault(const char* what);
+
 }  // namespace sfb
 
 struct sfb_modes {
