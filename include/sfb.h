@@ -189,7 +189,9 @@ int sfb_amln2clnn(sfb_plan* p, int64_t nlnn, const int64_t* lnn, const double* a
  * pixel assignment is piecewise constant).  The window cotangent is not returned:
  * a is affine in the window with dL/dW[p, l, n] = -dbar_{nl}(p), the synthesised
  * cotangent map this function forms internally, but no entry point hands it out.
- * A non-finite angle is an error (SFB_ERR_ARG), as in the forward call.
+ * A non-finite angle is an error (SFB_ERR_ARG), as in the forward call.  With a plan of
+ * SFB_PREC_MIXED the adjoint runs in FP32 (Jacobi recursion, synthesis, delta-bar maps,
+ * per-galaxy products) with FP64 sums over the maps; otherwise in FP64.
  * g'(r) is the spline derivative of src/Splines.jl:367-377
  * (derivative(gnl, n, l, r), src/GNLs.jl:110).  No counterpart in the reference,
  * which has no derivative of cat2amln; it backs the ChainRulesCore.rrule of the
@@ -239,7 +241,8 @@ int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const doub
 #define SFB_PHASE_PACK 6       /* a_nlm packing and the non-finite check */
 #define SFB_PHASE_SEGMENT 7    /* per knot group and galaxy: spline segment and t (shared by the group's maps) */
 #define SFB_PHASE_MONOPOLE 8   /* mixed precision: FP64 monopole correction of the l = 0 maps */
-#define SFB_NPHASE 9
+#define SFB_PHASE_ADJOINT 9    /* sfb_cat2amln_vjp(_device) after its catalogue stage */
+#define SFB_NPHASE 10
 
 /* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and
  * clears the totals.  No reference counterpart (the reference prints @time). */

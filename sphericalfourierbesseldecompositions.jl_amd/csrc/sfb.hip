@@ -991,12 +991,14 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
 // sequences (map, ring) per block.  1-D grid over units (ring pair, tile of TQ maps),
 // tiles fastest; workgroup b runs on XCD b mod 8, and each XCD takes runs of four
 // consecutive units, which read the same 128-byte lines of X (TQ maps x 16 B each).
-__global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
+template <typename R>
+__global__ __launch_bounds__(512) void k_phi_ifft(const R* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                   int32_t K, int32_t L, const RingDev* __restrict__ rings,
                                                   const double2* __restrict__ tw, double scale,
-                                                  double* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM,
+                                                  R* __restrict__ out, int32_t TQ, int32_t k0, int32_t logM,
                                                   const int32_t* __restrict__ mall)
 {
+    typedef typename Vec2T<R>::T R2;
     extern __shared__ double2 zi_raw[];
     const int nqt = (nq + TQ - 1) / TQ;
     const int b = blockIdx.x, loc = b >> 3;
@@ -1005,13 +1007,13 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, 
     const int k = k0 + unit / nqt;
     const int q0 = (unit - (unit / nqt) * nqt) * TQ;
     const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<double>::SH);
+    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
     const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1 or 2)
-    double2* zs = zi_raw;                 // nf x Mp (padded sequences)
-    double2* twr = zs + nf * Mp;          // N + 1 twiddles
-    double2* hs = twr + N + 1;            // nf x (L + 1): H_m per sequence
+    R2* zs = (R2*)zi_raw;                 // nf x Mp (padded sequences)
+    R2* twr = zs + nf * Mp;               // N + 1 twiddles
+    R2* hs = twr + N + 1;                 // nf x (L + 1): H_m per sequence
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
         double2 w[PH_UL];
@@ -1020,13 +1022,16 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, 
             if (e0 + u * (int)blockDim.x <= N) w[u] = twg[e0 + u * blockDim.x];
 #pragma unroll
         for (int u = 0; u < PH_UL; ++u)
-            if (e0 + u * (int)blockDim.x <= N) twr[e0 + u * blockDim.x] = w[u];
+            if (e0 + u * (int)blockDim.x <= N) {
+                twr[e0 + u * blockDim.x].x = (R)w[u].x;
+                twr[e0 + u * blockDim.x].y = (R)w[u].y;
+            }
     }
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
         const int m = e >> ltq, ql = e & (TQ - 1);  // maps fastest: contiguous X loads
         const int q = q0 + ql;
-        double er = 0, ei = 0, orr = 0, oi = 0;
+        R er = 0, ei = 0, orr = 0, oi = 0;
         if (q < nq && m <= Lk) {
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             er = X[be];
@@ -1037,46 +1042,61 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, 
                 oi = X[bo + 1];
             }
         }
-        const double c = m ? 2.0 : 1.0;
-        const double2 ph = twg[(int)(((int64_t)m * Rg.shift) % (2 * N))];  // e^{i m phi_0}
-        const double nr = c * (er + orr), ni = c * (ei + oi), sr = c * (er - orr), si = c * (ei - oi);
-        hs[(2 * ql + 0) * (L + 1) + m] = make_double2(nr * ph.x - ni * ph.y, nr * ph.y + ni * ph.x);
-        hs[(2 * ql + 1) * (L + 1) + m] = make_double2(sr * ph.x - si * ph.y, sr * ph.y + si * ph.x);
+        const R c = m ? (R)2 : (R)1;
+        const double2 ph64 = twg[(int)(((int64_t)m * Rg.shift) % (2 * N))];  // e^{i m phi_0}
+        const R phx = (R)ph64.x, phy = (R)ph64.y;
+        const R nr = c * (er + orr), ni = c * (ei + oi), sr = c * (er - orr), si = c * (ei - oi);
+        R2 hn, hsv;
+        hn.x = nr * phx - ni * phy;
+        hn.y = nr * phy + ni * phx;
+        hsv.x = sr * phx - si * phy;
+        hsv.y = sr * phy + si * phx;
+        hs[(2 * ql + 0) * (L + 1) + m] = hn;
+        hs[(2 * ql + 1) * (L + 1) + m] = hsv;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
         const int f = e >> logM, kk = e & (M - 1);  // M = 2^logM
-        const double2* h = hs + f * (L + 1);
-        auto spec = [&](int b) -> double2 {  // X_b, 0 <= b <= M
-            if (b == 0) return make_double2(2.0 * h[0].x, 0.0);
-            double2 v = make_double2(0.0, 0.0);
-            if (b <= L) v = h[b];
-            if (N - b <= L) {
-                v.x += h[N - b].x;
-                v.y -= h[N - b].y;
+        const R2* h = hs + f * (L + 1);
+        auto spec = [&](int bb) -> R2 {  // X_b, 0 <= b <= M
+            R2 v;
+            v.x = 0;
+            v.y = 0;
+            if (bb == 0) {
+                v.x = (R)2 * h[0].x;
+                return v;
+            }
+            if (bb <= L) v = h[bb];
+            if (N - bb <= L) {
+                v.x += h[N - bb].x;
+                v.y -= h[N - bb].y;
             }
             return v;
         };
-        const double2 xa = spec(kk), xb = spec(M - kk);  // X_k, X_{M-k}
-        const double er = 0.5 * (xa.x + xb.x), ei = 0.5 * (xa.y - xb.y);
-        const double dr = 0.5 * (xa.x - xb.x), di = 0.5 * (xa.y + xb.y);
-        const double2 w = twr[2 * kk];  // e^{2 pi i k/N}
-        const double orr = dr * w.x - di * w.y, oi = dr * w.y + di * w.x;
+        const R2 xa = spec(kk), xb = spec(M - kk);  // X_k, X_{M-k}
+        const R er = (R)0.5 * (xa.x + xb.x), ei = (R)0.5 * (xa.y - xb.y);
+        const R dr = (R)0.5 * (xa.x - xb.x), di = (R)0.5 * (xa.y + xb.y);
+        const R2 w = twr[2 * kk];  // e^{2 pi i k/N}
+        const R orr = dr * w.x - di * w.y, oi = dr * w.y + di * w.x;
         // Z = E + i O; conj(Z) goes in bit-reversed
         const int rk = (int)(__brev((unsigned)kk) >> (32 - logM));
-        zs[f * Mp + fpad<double>(rk)] = make_double2(er - oi, -(ei + orr));
+        R2 z;
+        z.x = er - oi;
+        z.y = -(ei + orr);
+        zs[f * Mp + fpad<R>(rk)] = z;
     }
     __syncthreads();
-    fft_passes<double>(zs, nf, M, Mp, N, logM, twr);
+    fft_passes<R>(zs, nf, M, Mp, N, logM, twr);
+    const R sc = (R)scale;
     for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
         const int f = e >> logM, j = e & (M - 1);
         const int ql = f >> 1, side = f & 1;
         const int q = q0 + ql;
         if (q >= nq || (side && eq)) continue;
-        const double2 z = zs[f * Mp + fpad<double>(j)];  // conj(z_j)
+        const R2 z = zs[f * Mp + fpad<R>(j)];  // conj(z_j)
         const int64_t base = (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * j;
-        out[base] = scale * z.x;
-        out[base + 1] = -scale * z.y;
+        out[base] = sc * z.x;
+        out[base + 1] = -sc * z.y;
     }
 }
 
@@ -1087,15 +1107,18 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const double* __restrict__ X, 
 // V_s Hermitian in u, so f_{4j+s} = sum_{u<=i/2} Re(V'_s[u] e^{2 pi i u j/i}) with
 // V' = V/2 at u = 0 and u = i/2 (i even), V elsewhere.  A work item is a sequence
 // (map, ring) and two outputs j; each ring takes as many maps per block as fill the
-// work items (2 TQ ceil(i/2) <= blockDim.x), up to TQMAX.  FP64.
-__global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ X, int64_t npix, int32_t nq,
+// work items (2 TQ ceil(i/2) <= blockDim.x), up to TQMAX.  R = double (FP64 throughout)
+// or float (the mixed adjoint: FP32 X, arithmetic and maps; FP64 twiddle tables rounded).
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_ipolar4(const R* __restrict__ X, int64_t npix, int32_t nq,
                                                      int64_t M2, int32_t K, int32_t L,
                                                      const RingDev* __restrict__ rings,
                                                      const double2* __restrict__ tw, double scale,
-                                                     double* __restrict__ out, int32_t TQMAX,
+                                                     R* __restrict__ out, int32_t TQMAX,
                                                      const int32_t* __restrict__ mall, int32_t nring,
                                                      int32_t ntile)
 {
+    typedef typename Vec2T<R>::T R2;
     extern __shared__ double2 ip_raw[];
     // 1-D grid over units (ring pair, map tile), tiles fastest; workgroup b runs on XCD
     // b mod 8 and each XCD takes runs of four consecutive units, so neighbouring tiles,
@@ -1111,10 +1134,13 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     if (q0 >= nq) return;
     const int nf = 2 * TQ;
     const double2* twl = tw + Rg.tw_off;  // (cos, sin)(pi kk/npr), kk < 2 npr
-    double2* twi = ip_raw;                // (cos, sin)(2 pi v/i), v < i
-    double2* Xb = twi + ni;               // [nf][npr] folded spectrum
-    double2* V = Xb + nf * npr;           // [nf][4][nu]
-    for (int e = threadIdx.x; e < ni; e += blockDim.x) twi[e] = twl[8 * e];
+    R2* twi = (R2*)ip_raw;                // (cos, sin)(2 pi v/i), v < i
+    R2* Xb = twi + ni;                    // [nf][npr] folded spectrum
+    R2* V = Xb + nf * npr;                // [nf][4][nu]
+    for (int e = threadIdx.x; e < ni; e += blockDim.x) {
+        twi[e].x = (R)twl[8 * e].x;
+        twi[e].y = (R)twl[8 * e].y;
+    }
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     // fold: bin b of map ql sums its aliasing class, both signs of m, for both rings
     // (maps fastest: a wave reads TQ consecutive maps of each X row).  The spectrum of a
@@ -1122,23 +1148,26 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     // mirrored, so each X value is read once rather than once per sign.
     const int nbh = npr / 2 + 1;
     for (int e = threadIdx.x; e < TQ * nbh; e += blockDim.x) {
-        const int ql = e % TQ, b = e / TQ;
+        const int ql = e % TQ, bb = e / TQ;
         const int q = q0 + ql;
-        double nr = 0.0, ni_ = 0.0, sr = 0.0, si = 0.0;
+        R nr = 0, ni_ = 0, sr = 0, si = 0;
         if (q < nq) {
             for (int sg = 0; sg < 2; ++sg) {  // m = +b (mod npr), then m = -b (mod npr) as conj
-                const int m0 = sg ? (npr - b) % npr : b;
+                const int m0 = sg ? (npr - bb) % npr : bb;
                 for (int m = m0; m <= Lk; m += npr) {
                     const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
-                    const double2 ev = *(const double2*)(X + be);
-                    double2 od = make_double2(0.0, 0.0);
-                    if (m + 1 <= L) od = *(const double2*)(X + ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q);
-                    const double c = m ? 2.0 : 1.0;
-                    const double2 ph = twl[(int)(((int64_t)m * Rg.shift) % n2)];  // e^{i m phi_0}
-                    const double gnr = c * (ev.x + od.x), gni = c * (ev.y + od.y);
-                    const double gsr = c * (ev.x - od.x), gsi = c * (ev.y - od.y);
-                    const double hnr = gnr * ph.x - gni * ph.y, hni = gnr * ph.y + gni * ph.x;
-                    const double hsr = gsr * ph.x - gsi * ph.y, hsi = gsr * ph.y + gsi * ph.x;
+                    const R2 ev = *(const R2*)(X + be);
+                    R2 od;
+                    od.x = 0;
+                    od.y = 0;
+                    if (m + 1 <= L) od = *(const R2*)(X + ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q);
+                    const R c = m ? (R)2 : (R)1;
+                    const double2 ph64 = twl[(int)(((int64_t)m * Rg.shift) % n2)];  // e^{i m phi_0}
+                    const R phx = (R)ph64.x, phy = (R)ph64.y;
+                    const R gnr = c * (ev.x + od.x), gni = c * (ev.y + od.y);
+                    const R gsr = c * (ev.x - od.x), gsi = c * (ev.y - od.y);
+                    const R hnr = gnr * phx - gni * phy, hni = gnr * phy + gni * phx;
+                    const R hsr = gsr * phx - gsi * phy, hsi = gsr * phy + gsi * phx;
                     nr += hnr;
                     ni_ += sg ? -hni : hni;
                     sr += hsr;
@@ -1146,11 +1175,18 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
                 }
             }
         }
-        Xb[(2 * ql) * npr + b] = make_double2(nr, ni_);
-        Xb[(2 * ql + 1) * npr + b] = make_double2(sr, si);
-        if (b > 0 && 2 * b < npr) {
-            Xb[(2 * ql) * npr + npr - b] = make_double2(nr, -ni_);
-            Xb[(2 * ql + 1) * npr + npr - b] = make_double2(sr, -si);
+        R2 vn, vs;
+        vn.x = nr;
+        vn.y = ni_;
+        vs.x = sr;
+        vs.y = si;
+        Xb[(2 * ql) * npr + bb] = vn;
+        Xb[(2 * ql + 1) * npr + bb] = vs;
+        if (bb > 0 && 2 * bb < npr) {
+            vn.y = -ni_;
+            vs.y = -si;
+            Xb[(2 * ql) * npr + npr - bb] = vn;
+            Xb[(2 * ql + 1) * npr + npr - bb] = vs;
         }
     }
     __syncthreads();
@@ -1158,18 +1194,22 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     for (int e = threadIdx.x; e < nf * 4 * nu; e += blockDim.x) {
         const int f = e / (4 * nu), r = e - f * 4 * nu;
         const int ss = r / nu, u = r - ss * nu;
-        const double2* xf = Xb + f * npr;
-        double vr = 0.0, vi = 0.0;
+        const R2* xf = Xb + f * npr;
+        R vr = 0, vi = 0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int bb = u + t * ni;
-            const double2 x = xf[bb];
-            const double2 w = twl[(int)(((int64_t)2 * bb * ss) % n2)];  // e^{2 pi i bb s/npr}
-            vr += x.x * w.x - x.y * w.y;
-            vi += x.x * w.y + x.y * w.x;
+            const R2 x = xf[bb];
+            const double2 w64 = twl[(int)(((int64_t)2 * bb * ss) % n2)];  // e^{2 pi i bb s/npr}
+            const R wx = (R)w64.x, wy = (R)w64.y;
+            vr += x.x * wx - x.y * wy;
+            vi += x.x * wy + x.y * wx;
         }
-        const double hv = (u == 0 || 2 * u == ni) ? 0.5 : 1.0;
-        V[e] = make_double2(hv * vr, hv * vi);
+        const R hv = (u == 0 || 2 * u == ni) ? (R)0.5 : (R)1;
+        R2 v;
+        v.x = hv * vr;
+        v.y = hv * vi;
+        V[e] = v;
     }
     __syncthreads();
     // length-i real inverse DFTs, two outputs j per work item
@@ -1179,18 +1219,18 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
     const int q = q0 + (f >> 1), side = f & 1;
     if (q >= nq) return;
     const int j0 = 2 * jp, j1 = j0 + 1;
-    const double2* vf = V + f * 4 * nu;
-    double acc[2][4];
+    const R2* vf = V + f * 4 * nu;
+    R acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int ss = 0; ss < 4; ++ss) acc[a][ss] = 0.0;
+        for (int ss = 0; ss < 4; ++ss) acc[a][ss] = 0;
     int i0 = 0, i1 = 0;  // u j0, u j1 (mod i)
     for (int u = 0; u < nu; ++u) {
-        const double2 w0 = twi[i0], w1 = twi[i1];
+        const R2 w0 = twi[i0], w1 = twi[i1];
 #pragma unroll
         for (int ss = 0; ss < 4; ++ss) {
-            const double2 v = vf[ss * nu + u];
+            const R2 v = vf[ss * nu + u];
             acc[0][ss] = fma(v.x, w0.x, fma(-v.y, w0.y, acc[0][ss]));
             acc[1][ss] = fma(v.x, w1.x, fma(-v.y, w1.y, acc[1][ss]));
         }
@@ -1199,12 +1239,13 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const double* __restrict__ 
         i1 += j1;
         if (i1 >= ni) i1 -= ni;
     }
-    double* o = out + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n);
+    const R sc = (R)scale;
+    R* o = out + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n);
 #pragma unroll
-    for (int ss = 0; ss < 4; ++ss) o[4 * j0 + ss] = scale * acc[0][ss];
+    for (int ss = 0; ss < 4; ++ss) o[4 * j0 + ss] = sc * acc[0][ss];
     if (j1 < ni)
 #pragma unroll
-        for (int ss = 0; ss < 4; ++ss) o[4 * j1 + ss] = scale * acc[1][ss];
+        for (int ss = 0; ss < 4; ++ss) o[4 * j1 + ss] = sc * acc[1][ss];
 }
 
 // Polar ring pairs (npr = 4i, i < nside): one radix-4 decimation-in-time step,
@@ -1788,9 +1829,10 @@ __global__ void k_fill_f64(double* p, int64_t n, double v)
 
 // b = D_half(cotangent) in the a layout: per map q the row l = l_q, m = 0..l, halved
 // for m > 0; Im of m = 0 is dropped (Im a_l0 vanishes identically).  a is zeroed first.
+template <typename TA>
 __global__ void k_unpack_cot(const double* __restrict__ cot, const int64_t* __restrict__ aoff,
                              const MapInfo* __restrict__ maps, const int64_t* __restrict__ out_base, int32_t nq,
-                             int64_t M2, double* __restrict__ a)
+                             int64_t M2, TA* __restrict__ a)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
@@ -1798,33 +1840,34 @@ __global__ void k_unpack_cot(const double* __restrict__ cot, const int64_t* __re
     const int l = maps[q].l;
     if (m > l) return;
     const int par = (l - m) & 1, j = (l - m) >> 1;
-    double* dst = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
+    TA* dst = a + aoff[m * 2 + par] + (int64_t)j * M2 + 2 * q;
     const double re = cot[2 * (out_base[q] + m)], im = cot[2 * (out_base[q] + m) + 1];
-    dst[0] = m ? 0.5 * re : re;
-    dst[1] = m ? 0.5 * im : 0.0;
+    dst[0] = (TA)(m ? 0.5 * re : re);
+    dst[1] = (TA)(m ? 0.5 * im : 0.0);
 }
 
 // Pixel values from ring-Fourier coefficients (X layout of the phi-transform, E/O
 // form): out[q][p] = scale * sum_{m<=L} c_m Re(G_m e^{i m phi_j}), c_0 = 1, c_m = 2,
 // G = E + O on the north ring and E - O on the south ring of pair k, by complex
 // Horner in z = e^{i phi_j}.
-__global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_synth(const R* __restrict__ X, int64_t npix, int32_t nq, int64_t M2,
                                                    int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                   double scale, double* __restrict__ out, int32_t TQ,
+                                                   double scale, R* __restrict__ out, int32_t TQ,
                                                    const int32_t* __restrict__ mall, int32_t k0)
 {
     extern __shared__ double2 gs[];  // [TQ][side][L+1]
     const int k = k0 + blockIdx.x;
     const int q0 = blockIdx.y * TQ;
-    const RingDev R = rings[k];
-    const int npr = R.npr;
+    const RingDev Rg = rings[k];
+    const int npr = Rg.npr;
     const bool eq = (k == K - 1);
     const int nside = eq ? 1 : 2;
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
     for (int e = threadIdx.x; e < TQ * (L + 1); e += blockDim.x) {
         const int m = e / TQ, ql = e - m * TQ;
         const int q = q0 + ql;
-        double er = 0, ei = 0, orr = 0, oi = 0;
+        double er = 0, ei = 0, orr = 0, oi = 0;  // X values widened to FP64
         if (q < nq && m <= Lk) {
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             er = X[be];
@@ -1846,7 +1889,7 @@ __global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X,
         const int q = q0 + ql;
         if (q >= nq) continue;
         double zs, zc;
-        sincospi((double)(2 * j + R.shift) / (double)npr, &zs, &zc);
+        sincospi((double)(2 * j + Rg.shift) / (double)npr, &zs, &zc);
         const double2* g = gs + (ql * 2 + side) * (L + 1);
         double ar = 0.0, ai = 0.0;
         for (int m = L; m >= 0; --m) {
@@ -1856,7 +1899,7 @@ __global__ __launch_bounds__(256) void k_phi_synth(const double* __restrict__ X,
             ar = nr;
             ai = ni;
         }
-        out[(int64_t)q * npix + (side ? R.start_s : R.start_n) + j] = scale * ar;
+        out[(int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + j] = (R)(scale * ar);
     }
 }
 
@@ -1953,18 +1996,23 @@ __global__ __launch_bounds__(1024) void k_vjp_gather(int64_t N, int64_t npix, co
 // correctly rounded t); then per map of the group its records (d, c, b, a) go to LDS
 // and g = d + t(c + t(b + t a)) (the forward's value, bit for bit) and
 // g' = (c + t(2b + 3at)) / h, as the product with the tabulated 1/h (0 on the boundary
-// pseudo-records, the constant extrapolation).  Galaxies in pixel order.
+// pseudo-records, the constant extrapolation).  Galaxies in pixel order.  R = float (the
+// mixed adjoint): FP32 records, delta-bar and products (t still from the FP64 search),
+// FP64 sums over the maps; the record staging moves half the bytes.
 constexpr int VJ_GPT = 6;  // galaxies per thread: 8 spilled once the early gathers held their values
+template <typename R>
 __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_t cap, const double* __restrict__ r_s,
                                                  const uint32_t* __restrict__ pix_s,
                                                  const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
                                                  const double* __restrict__ grh, const int32_t* __restrict__ vgs,
                                                  const int32_t* __restrict__ vgm, const int2* __restrict__ vrange,
-                                                 const MapInfo* __restrict__ maps, const double4* __restrict__ rec4,
-                                                 const double* __restrict__ dbar, double2* __restrict__ part)
+                                                 const MapInfo* __restrict__ maps,
+                                                 const typename Vec4T<R>::T* __restrict__ rec4,
+                                                 const R* __restrict__ dbar, double2* __restrict__ part)
 {
+    typedef typename Vec4T<R>::T R4;
     extern __shared__ double4 vj_raw[];
-    double4* tb = vj_raw;                   // records, cap + 1
+    R4* tb = (R4*)vj_raw;                   // records, cap + 1
     double* xs = (double*)(tb + cap + 1);   // knots, cap
     double* rh = xs + cap;                  // 1/h, cap
     const int64_t k0 = (int64_t)blockIdx.x * (1024 * VJ_GPT) + threadIdx.x;
@@ -2027,10 +2075,10 @@ __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_
         for (int j = vgs[g]; j < vgs[g + 1]; ++j) {
             const int q = vgm[j];
             const MapInfo m = maps[q];
-            const double* dq = dbar + (int64_t)q * npix;
-            double dd[VJ_GPT];
+            const R* dq = dbar + (int64_t)q * npix;
+            R dd[VJ_GPT];
 #pragma unroll
-            for (int u = 0; u < VJ_GPT; ++u) dd[u] = (k0 + 1024 * u < N) ? dq[pp[u]] : 0.0;
+            for (int u = 0; u < VJ_GPT; ++u) dd[u] = (k0 + 1024 * u < N) ? dq[pp[u]] : (R)0;
             __syncthreads();  // the previous map's records are no longer read
             // the scatter's split layout: random record reads spread over every bank group
             // (in-order 32-byte records ran 5.24 ms at nside 128, split 4.89 ms)
@@ -2039,12 +2087,19 @@ __global__ __launch_bounds__(1024) void k_vjp_it(int64_t N, int64_t npix, int32_
 #pragma unroll
             for (int u = 0; u < VJ_GPT; ++u) {
                 if (k0 + 1024 * u < N) {
-                    const double4 c = lds_rec_get(tb, cap + 1, ii[u]);  // (d, c, b, a)
-                    const double tt = t[u];
-                    const double v = fma(tt, fma(tt, fma(tt, c.w, c.z), c.y), c.x);
-                    const double dv = ii[u] < G.nk - 1 ? fma(tt, fma(tt, 3.0 * c.w, 2.0 * c.z), c.y) * rh[ii[u]] : 0.0;
-                    aw[u] = fma(v, dd[u], aw[u]);
-                    ar[u] = fma(dv, dd[u], ar[u]);
+                    const R4 c = lds_rec_get(tb, cap + 1, ii[u]);  // (d, c, b, a)
+                    const R tt = (R)t[u];
+                    const R v = fma(tt, fma(tt, fma(tt, c.w, c.z), c.y), c.x);
+                    const R dv = ii[u] < G.nk - 1
+                                     ? fma(tt, fma(tt, (R)3 * c.w, (R)2 * c.z), c.y) * (R)rh[ii[u]]
+                                     : (R)0;
+                    if (sizeof(R) == 8) {  // FP64: the products fused into the sums
+                        aw[u] = fma((double)v, (double)dd[u], aw[u]);
+                        ar[u] = fma((double)dv, (double)dd[u], ar[u]);
+                    } else {  // FP32 products, FP64 sums over the maps
+                        aw[u] += (double)(v * dd[u]);
+                        ar[u] += (double)(dv * dd[u]);
+                    }
                 }
             }
         }
@@ -2433,6 +2488,7 @@ struct sfb_plan {
     DevBuf d_delta, d_X0, d_G, d_R, d_a, d_a1, d_a2, d_ic0;
     DevBuf d_mono;                 // mixed path: monopole partial sums and window sums
     DevBuf d_desc_synf;            // full synthesis a -> X0 (all ring pairs), per a buffer
+    DevBuf d_desc_synf32;          // the same with a, LamT and X0 as floats (mixed adjoint)
     DevBuf d_desc_ana32;           // first analysis with X0 holding floats (mixed path)
     DevBuf d_desc_syn32;           // aliasing-ring synthesis with G as floats (mixed path)
     DevBuf d_qbase, d_lmaxn;       // per n: map index of (n, 0) and lmax_n (int32)
@@ -3037,6 +3093,17 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         }
     if ((rc = P->d_desc_synf.ensure(sizeof(GemmDesc) * synf.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_synf.p, synf.data(), sizeof(GemmDesc) * synf.size(), hipMemcpyHostToDevice));
+    // the mixed adjoint's full synthesis: a (floats) LamT (FP32 copy) -> X0 (floats)
+    for (int slot = 0; slot < 3; ++slot)
+        for (int64_t b = 0; b < nbat; ++b) {
+            GemmDesc& fd = synf[(size_t)(slot * nbat + b)];
+            const int64_t kc = P->kcut[(size_t)(b / 2)];
+            fd.A = fa(*a_in_buf[slot], (size_t)b);
+            fd.B = (const double*)(P->d_lamT32.as<float>() + P->lam_off[(size_t)b] + kc);
+            fd.C = (double*)(P->d_X0.as<float>() + (b * (int64_t)P->K + kc) * P->M2);
+        }
+    if ((rc = P->d_desc_synf32.ensure(sizeof(GemmDesc) * synf.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_desc_synf32.p, synf.data(), sizeof(GemmDesc) * synf.size(), hipMemcpyHostToDevice));
     if ((rc = P->d_desc_ana.ensure(sizeof(GemmDesc) * ana.size()))) return rc;
     if ((rc = P->d_desc_ana32.ensure(sizeof(GemmDesc) * ana32.size()))) return rc;
     HIP_TRY(hipMemcpy(P->d_desc_ana32.p, ana32.data(), sizeof(GemmDesc) * ana32.size(), hipMemcpyHostToDevice));
@@ -3151,7 +3218,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
                       &P->d_desc_cal32, &P->d_mall, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
-                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono};
+                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono, &P->d_desc_synf32};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -3750,39 +3817,46 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false, 
 // radix-4 inverse for i <= 256, the direct sum k_phi_synth beyond (and wherever the radix-4
 // block does not fit in LDS); equatorial ones: the inverse FFT (the direct sum where its
 // block does not fit).
-static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
+template <typename R>
+static int phi_synthesis_t(sfb_plan* P, int32_t nq, double scale, hipStream_t st)
 {
     const int32_t K = P->K, L = (int32_t)P->lmax;
-    const InvShapes S = inv_shapes(P->nside, L, P->lds_max);
+    const InvShapes S = inv_shapes(P->nside, L, P->lds_max);  // sized for FP64: FP32 blocks use less
+    const R* X = P->d_X0.as<R>();
+    R* out = P->d_delta.as<R>();
     if (S.nip4 > 0) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.shq));
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ipolar4<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)S.shq));
         const int32_t ntile = (int32_t)((nq + S.tqmin - 1) / S.tqmin);
         const int64_t units = (int64_t)S.nip4 * ntile;
-        hipLaunchKernelGGL(k_phi_ipolar4, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), S.shq, st,
-                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), IP_TQMAX, P->d_mall.as<int32_t>(),
-                           S.nip4, ntile);
+        hipLaunchKernelGGL(k_phi_ipolar4<R>, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(256), S.shq, st, X,
+                           P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), P->d_tw.as<double2>(), scale, out,
+                           IP_TQMAX, P->d_mall.as<int32_t>(), S.nip4, ntile);
         LAUNCH_CHECK();
     }
     if (S.k_eq0 > S.nip4) {
         const int32_t TQS = 8;
         const size_t shs = sizeof(double2) * 2 * TQS * (size_t)(L + 1);
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
-        hipLaunchKernelGGL(k_phi_synth, dim3((unsigned)(S.k_eq0 - S.nip4), nblk(nq, TQS)), dim3(256), shs, st,
-                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), scale,
-                           P->d_delta.as<double>(), TQS, P->d_mall.as<int32_t>(), S.nip4);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_synth<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+        hipLaunchKernelGGL(k_phi_synth<R>, dim3((unsigned)(S.k_eq0 - S.nip4), nblk(nq, TQS)), dim3(256), shs, st, X,
+                           P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), scale, out, TQS,
+                           P->d_mall.as<int32_t>(), S.nip4);
         LAUNCH_CHECK();
     }
     if (S.k_eq0 < K) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.shi));
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_ifft<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.shi));
         const int64_t units = (int64_t)(K - S.k_eq0) * nblk(nq, S.tqi);
-        hipLaunchKernelGGL(k_phi_ifft, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(512), S.shi, st,
-                           P->d_X0.as<double>(), P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), scale, P->d_delta.as<double>(), S.tqi, S.k_eq0, P->logM,
-                           P->d_mall.as<int32_t>());
+        hipLaunchKernelGGL(k_phi_ifft<R>, dim3((unsigned)(32 * ((units + 31) / 32))), dim3(512), S.shi, st, X,
+                           P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(), P->d_tw.as<double2>(), scale, out,
+                           S.tqi, S.k_eq0, P->logM, P->d_mall.as<int32_t>());
         LAUNCH_CHECK();
     }
     return SFB_OK;
+}
+
+static int phi_synthesis(sfb_plan* P, int32_t nq, double scale, hipStream_t st, bool f32 = false)
+{
+    return f32 ? phi_synthesis_t<float>(P, nq, scale, st) : phi_synthesis_t<double>(P, nq, scale, st);
 }
 
 // Launch shapes of the ring transforms for a geometry (host arithmetic, no device).
@@ -3969,34 +4043,64 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     int rc;
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     const int32_t nq = (int32_t)P->nmaps, L = (int32_t)P->lmax;
+    // mixed precision (the plan's SFB_PREC_MIXED): FP32 Jacobi recursion, synthesis GEMM,
+    // ring synthesis, delta-bar maps and per-galaxy products, FP64 sums; it needs the
+    // knot-group gather (the records in LDS), else the adjoint runs in FP64
+    const size_t shv64 = 32 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
+    const size_t shv32 = 16 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
+    const bool grouped = P->sell_ok && P->nvrange > 0;
+    const bool f32 = P->precision == SFB_PREC_MIXED && grouped && (int64_t)shv32 <= P->lds_max;
+    PhaseTimer tj(P, SFB_PHASE_ADJOINT, st);
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(P->a_size, 256)), dim3(256), 0, st, P->d_a.as<double>(), P->a_size, 0.0);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_unpack_cot, dim3(nblk(nq, 256), (unsigned)(L + 1)), dim3(256), 0, st, d_cot,
-                       P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(), P->d_outbase.as<int64_t>(), nq, P->M2,
-                       P->d_a.as<double>());
+    if (f32)
+        hipLaunchKernelGGL(k_unpack_cot<float>, dim3(nblk(nq, 256), (unsigned)(L + 1)), dim3(256), 0, st, d_cot,
+                           P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(), P->d_outbase.as<int64_t>(), nq, P->M2,
+                           P->d_a.as<float>());
+    else
+        hipLaunchKernelGGL(k_unpack_cot<double>, dim3(nblk(nq, 256), (unsigned)(L + 1)), dim3(256), 0, st, d_cot,
+                           P->d_aoff.as<int64_t>(), P->d_maps.as<MapInfo>(), P->d_outbase.as<int64_t>(), nq, P->M2,
+                           P->d_a.as<double>());
     LAUNCH_CHECK();
-    if ((rc = jacobi_stage(P, nq, st))) return rc;
+    const bool prof = P->prof;
+    P->prof = false;  // the adjoint's stages are timed as one phase
+    rc = jacobi_stage(P, nq, st, f32);
+    P->prof = prof;
+    if (rc) return rc;
     // dbar = dOmega S a_final on every pixel of every map
     const int slot = (P->a_final == P->d_a.as<double>()) ? 0 : (P->a_final == P->d_a1.as<double>() ? 1 : 2);
     const unsigned nbat = (unsigned)(2 * (L + 1));
-    launch_gemm(P->d_desc_synf.as<GemmDesc>() + slot * nbat, nbat, 2 * nq, P->K, 1.0, 0.0, st);
+    if (f32)
+        launch_gemm(P->d_desc_synf32.as<GemmDesc>() + slot * nbat, nbat, 2 * nq, P->K, 1.0, 0.0, st, true, true, true);
+    else
+        launch_gemm(P->d_desc_synf.as<GemmDesc>() + slot * nbat, nbat, 2 * nq, P->K, 1.0, 0.0, st);
     LAUNCH_CHECK();
     const double domega = 4.0 * kPi / (double)P->npix;
-    if ((rc = phi_synthesis(P, nq, domega, st))) return rc;
+    if ((rc = phi_synthesis(P, nq, domega, st, f32))) return rc;
     // per-galaxy sums over the maps, in ranges of maps, then the ordered reduction
     int32_t ngroups;
-    const size_t shv = 32 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
-    if (P->sell_ok && P->nvrange > 0 && (int64_t)shv <= P->lds_max) {
+    if (f32 || (grouped && (int64_t)shv64 <= P->lds_max)) {
         // knot groups: the segment search once per group and galaxy
         ngroups = P->nvrange;
         if ((rc = P->d_part.ensure(sizeof(double2) * (size_t)ngroups * (size_t)N))) return rc;
-        HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_it, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shv));
-        hipLaunchKernelGGL(k_vjp_it, dim3(nblk(N, 1024 * VJ_GPT), (unsigned)ngroups), dim3(1024), shv, st, N,
-                           P->npix, P->tbl_cap, P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(),
-                           P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
-                           P->d_vgs.as<int32_t>(), P->d_vgm.as<int32_t>(), P->d_vrange.as<int2>(),
-                           P->d_maps.as<MapInfo>(), P->d_rec4.as<double4>(), P->d_delta.as<double>(),
-                           P->d_part.as<double2>());
+        const dim3 gv(nblk(N, 1024 * VJ_GPT), (unsigned)ngroups);
+        if (f32) {
+            HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_it<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)shv32));
+            hipLaunchKernelGGL(k_vjp_it<float>, gv, dim3(1024), shv32, st, N, P->npix, P->tbl_cap,
+                               P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(), P->d_groups.as<GroupInfo>(),
+                               P->d_gx.as<double>(), P->d_grh.as<double>(), P->d_vgs.as<int32_t>(),
+                               P->d_vgm.as<int32_t>(), P->d_vrange.as<int2>(), P->d_maps.as<MapInfo>(),
+                               P->d_rec4f.as<float4>(), P->d_delta.as<float>(), P->d_part.as<double2>());
+        } else {
+            HIP_TRY(hipFuncSetAttribute((const void*)k_vjp_it<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)shv64));
+            hipLaunchKernelGGL(k_vjp_it<double>, gv, dim3(1024), shv64, st, N, P->npix, P->tbl_cap,
+                               P->d_rs.as<double>(), P->d_pixkey2.as<uint32_t>(), P->d_groups.as<GroupInfo>(),
+                               P->d_gx.as<double>(), P->d_grh.as<double>(), P->d_vgs.as<int32_t>(),
+                               P->d_vgm.as<int32_t>(), P->d_vrange.as<int2>(), P->d_maps.as<MapInfo>(),
+                               P->d_rec4.as<double4>(), P->d_delta.as<double>(), P->d_part.as<double2>());
+        }
         LAUNCH_CHECK();
     } else {
         const int32_t MG = (nq + 31) / 32;
@@ -4017,9 +4121,14 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     hipLaunchKernelGGL(k_vjp_reduce, dim3(nblk(N, 256)), dim3(256), 0, st, N, ngroups, P->d_part.as<double2>(),
                        P->d_idx.as<uint32_t>(), P->d_ws.as<double>(), inv, d_dr, d_dw);
     LAUNCH_CHECK();
+    tj.end();
     int32_t flag[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (P->prof) {
+        prof_collect(P);
+        P->prof_calls += 1;
+    }
     // as the forward call: a non-finite angle has no pixel (floor(Int, NaN) throws)
     if (flag[1]) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
     return SFB_OK;

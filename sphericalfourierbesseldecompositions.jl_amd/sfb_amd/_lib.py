@@ -71,7 +71,7 @@ SIGNATURES = {
     "sfb_plan_tiles": (ctypes.c_int, [_P, _P, _P]),
 }
 
-PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment", "monopole"]
+PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment", "monopole", "adjoint"]
 
 _lib = None
 
