@@ -624,7 +624,7 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_r
 // R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
 // w and the records rounded to FP32, FP32 Horner and w g products, FP64 pixel sums
 // (an FP32 running sum cost up to 4e-7 of an (n,l) block at nside 128), FP32 delta.
-template <bool DENSE, int NM, bool UNIT, typename R>
+template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
                                                      int32_t tstride, const int2* __restrict__ pinfo,
@@ -636,11 +636,13 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                                                      const typename Vec4T<R>::T* __restrict__ rec4, double inv,
                                                      const double* __restrict__ win_a,
                                                      const double* __restrict__ win_b, int64_t lmax,
-                                                     R* __restrict__ delta, int32_t slot_order)
+                                                     R* __restrict__ delta, int32_t slot_order,
+                                                     const float* __restrict__ reclo)
 {
     typedef typename Vec4T<R>::T R4;
     extern __shared__ double4 sm_raw[];
     R4* sm4 = (R4*)sm_raw;
+    float* lo_sm = (float*)(sm4 + NM * tstride);  // SPLIT: the records' d - RN32(d), per table
     const ScatterBlock B = blk[(int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3)];
     bool has[NM];
     int64_t col[NM];
@@ -655,7 +657,10 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         const MapInfo m = maps[q];
         if (k == 0) pad = m.pad;
         if (has[k])
-            for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) lds_rec_put(sm4 + k * tstride, tstride, j, rec4[m.koff + q + j]);
+            for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) {
+                lds_rec_put(sm4 + k * tstride, tstride, j, rec4[m.koff + q + j]);
+                if (SPLIT) lo_sm[k * tstride + j] = reclo[m.koff + q + j];
+            }
         tab[k] = sm4 + (has[k] ? k : 0) * tstride;
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
@@ -738,24 +743,45 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     for (; r <= rlast; r += 4) {
         load_rows<UNIT, R>(nxt, gi, gt, gw, r + 4, lane);
         R4 c[4][NM];
+        float clo[4][NM];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int k = 0; k < NM; ++k) c[u][k] = lds_rec_get(tab[k], tstride, cur.i[u]);  // (d, c, b, a)
+            for (int k = 0; k < NM; ++k) {
+                c[u][k] = lds_rec_get(tab[k], tstride, cur.i[u]);  // (d, c, b, a)
+                if (SPLIT) clo[u][k] = lo_sm[(has[k] ? k : 0) * tstride + cur.i[u]];
+            }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
             const R t = cur.t[u];
-            R v[NM];
+            if (SPLIT) {
+                // g = d_hi + (d_lo + t (c + t (b + t a))): the FP32 part rounds once, relative to
+                // t c; d_hi and that part are added in FP64
+                double v[NM];
 #pragma unroll
-            for (int k = 0; k < NM; ++k) v[k] = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
-            if (UNIT && r + u - first >= cs.minc) {  // wave-uniform: some lane is past its galaxies
-                const bool live = r + u - first < cs.pc.y;
+                for (int k = 0; k < NM; ++k)
+                    v[k] = (double)c[u][k].x +
+                           (double)fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), clo[u][k]);
+                if (UNIT && r + u - first >= cs.minc) {  // wave-uniform: some lane is past its galaxies
+                    const bool live = r + u - first < cs.pc.y;
 #pragma unroll
-                for (int k = 0; k < NM; ++k) v[k] = live ? v[k] : (R)0;
+                    for (int k = 0; k < NM; ++k) v[k] = live ? v[k] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + v[k] : acc[k] + (double)cur.w[u] * v[k];
+            } else {
+                R v[NM];
+#pragma unroll
+                for (int k = 0; k < NM; ++k) v[k] = fma(t, fma(t, fma(t, c[u][k].w, c[u][k].z), c[u][k].y), c[u][k].x);
+                if (UNIT && r + u - first >= cs.minc) {  // wave-uniform: some lane is past its galaxies
+                    const bool live = r + u - first < cs.pc.y;
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) v[k] = live ? v[k] : (R)0;
+                }
+#pragma unroll
+                for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + (double)v[k] : acc[k] + (double)(cur.w[u] * v[k]);
             }
-#pragma unroll
-            for (int k = 0; k < NM; ++k) acc[k] = UNIT ? acc[k] + (double)v[k] : acc[k] + (double)(cur.w[u] * v[k]);
             close_slices(r + u + 1);
         }
         cur = nxt;
@@ -874,17 +900,20 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
 // through a complex FFT of length M = npr/2 in LDS, 2*TQ sequences per block:
 // bit-reversed load, then the radix-2 stages fused in pairs (one radix-4 pass of
 // 4 loads / 3 twiddles / 4 butterflies per thread item between barriers).  R is
-// the value type (double: plain path; float: mixed path).
+// the arithmetic type, TS that of the maps and of X (double, double: the plain path;
+// double, float: the mixed path, whose FP32 butterflies erred by ~1e-7 of a
+// coefficient, too much for the (n,l) blocks whose signal is ten times below the rest).
 constexpr int PH_UL = 4;  // phi-transform sample loads in flight per thread
 
-template <typename R>
-__global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
+template <typename R, typename TS = R>
+__global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, int64_t npix, int32_t nq, int64_t M2,
                                                  int32_t K, int32_t L, const RingDev* __restrict__ rings,
-                                                 const double2* __restrict__ tw, R* __restrict__ X, int32_t TQ,
+                                                 const double2* __restrict__ tw, TS* __restrict__ X, int32_t TQ,
                                                  int32_t k0, int32_t logM, const int32_t* __restrict__ mall,
                                                  const uint32_t* __restrict__ perm, int32_t nring, int32_t ntile)
 {
     typedef typename Vec2T<R>::T R2;
+    typedef typename Vec2T<TS>::T S2;
     extern __shared__ double2 zs_raw[];
     R2* zs = (R2*)zs_raw;
     // 1-D grid over units (map tile, ring pair), ring pairs fastest; workgroup b runs on
@@ -931,8 +960,8 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
                 const int q = q0 + ql;
                 R re = 0, im = 0;
                 if (live && q < nq) {
-                    re = delta[(int64_t)q * npix + sl.x];
-                    im = delta[(int64_t)q * npix + sl.y];
+                    re = (R)delta[(int64_t)q * npix + sl.x];
+                    im = (R)delta[(int64_t)q * npix + sl.y];
                 }
                 zs[(2 * ql + side) * Mp + fpad<R>(rj)].x = re;
                 zs[(2 * ql + side) * Mp + fpad<R>(rj)].y = im;
@@ -945,9 +974,9 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
             const int q = q0 + ql;
             R re = 0, im = 0;
             if (q < nq && !(side && eq)) {  // sample pairs start at even pixels
-                const R2 v = *(const R2*)(delta + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
-                re = v.x;
-                im = v.y;
+                const S2 v = *(const S2*)(delta + (int64_t)q * npix + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
+                re = (R)v.x;
+                im = (R)v.y;
             }
             const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
             zs[f * Mp + fpad<R>(rj)].x = re;
@@ -976,7 +1005,7 @@ __global__ __launch_bounds__(512) void k_phi_fft(const R* __restrict__ delta, in
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
         // ring phases from the LDS table: (m' shift) mod 2N <= L <= N for every m' <= L
-        store_folded<R>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<TS>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, (TS)xr[0], (TS)xi[0], (TS)xr[1], (TS)xi[1]);
     }
 }
 
@@ -1347,21 +1376,53 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         const int u0 = 2 * up, u1 = u0 + 1;
         const R* d0 = D + f * ld;
         int i0 = 0, i1 = 0;
-        for (int j = 0; j < ni; ++j) {
-            const R2 w0 = twi[i0], w1 = twi[i1];
-            const R4 x = *(const R4*)(d0 + 4 * j);
-            const R xs[4] = {x.x, x.y, x.z, x.w};
+        // FP32 (mixed path): the running sums restart every PH_CHUNK samples and are flushed
+        // into FP64 sums, so a rounding is relative to at most PH_CHUNK terms rather than to
+        // up to 255 (a length-i sequential FP32 sum errs by ~sqrt(i/2) ulp: 1e-6 of the
+        // coefficients at nside 256, the mixed path's whole error budget)
+        constexpr int PH_CHUNK = sizeof(R) == 4 ? 16 : (1 << 30);
+        double A[2][4][2];
 #pragma unroll
-            for (int ss = 0; ss < 4; ++ss) {
-                a[0][ss][0] = fma(xs[ss], w0.x, a[0][ss][0]);
-                a[0][ss][1] = fma(-xs[ss], w0.y, a[0][ss][1]);
-                a[1][ss][0] = fma(xs[ss], w1.x, a[1][ss][0]);
-                a[1][ss][1] = fma(-xs[ss], w1.y, a[1][ss][1]);
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int ss = 0; ss < 4; ++ss) A[u][ss][0] = A[u][ss][1] = 0.0;
+        for (int jc = 0; jc < ni; jc += PH_CHUNK) {
+            const int je = min(ni, jc + PH_CHUNK);
+            for (int j = jc; j < je; ++j) {
+                const R2 w0 = twi[i0], w1 = twi[i1];
+                const R4 x = *(const R4*)(d0 + 4 * j);
+                const R xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int ss = 0; ss < 4; ++ss) {
+                    a[0][ss][0] = fma(xs[ss], w0.x, a[0][ss][0]);
+                    a[0][ss][1] = fma(-xs[ss], w0.y, a[0][ss][1]);
+                    a[1][ss][0] = fma(xs[ss], w1.x, a[1][ss][0]);
+                    a[1][ss][1] = fma(-xs[ss], w1.y, a[1][ss][1]);
+                }
+                i0 += u0;
+                if (i0 >= ni) i0 -= ni;
+                i1 += u1;
+                if (i1 >= ni) i1 -= ni;
             }
-            i0 += u0;
-            if (i0 >= ni) i0 -= ni;
-            i1 += u1;
-            if (i1 >= ni) i1 -= ni;
+            if (sizeof(R) == 4) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int ss = 0; ss < 4; ++ss)
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            A[u][ss][c] += (double)a[u][ss][c];
+                            a[u][ss][c] = 0;
+                        }
+            }
+        }
+        if (sizeof(R) == 4) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ss = 0; ss < 4; ++ss)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) a[u][ss][c] = (R)A[u][ss][c];
         }
     }
     __syncthreads();  // every sample read; Y overwrites them
@@ -2453,11 +2514,27 @@ struct DevBuf {
         bytes = want;
         return SFB_OK;
     }
+    bool pinned = false;  // pinned host memory (ensure_host) rather than device memory
+    int ensure_host(size_t need)
+    {
+        if (need <= bytes && p) return SFB_OK;
+        release();
+        if (sfb::fault("alloc")) return fail(SFB_ERR_DEVICE, "hipHostMalloc(%zu) failed: forced by SFB_FAULT=alloc", need);
+        hipError_t e = hipHostMalloc(&p, need ? need : 16, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(SFB_ERR_DEVICE, "hipHostMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+        }
+        pinned = true;
+        bytes = need ? need : 16;
+        return SFB_OK;
+    }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
         p = nullptr;
         bytes = 0;
+        pinned = false;
     }
     template <class T> T* as() const { return (T*)p; }
 };
@@ -2526,7 +2603,8 @@ struct sfb_plan {
     int32_t ngroup = 0, nk_max_group = 0;
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
     int32_t sc_nchunk_it = 0;
-    int32_t sc_nm[2] = {1, 1}, sc_per_xcd[2] = {0, 0};  // [0] FP64, [1] FP32 scatter
+    int32_t sc_nm[3] = {1, 1, 1}, sc_per_xcd[3] = {0, 0, 0};  // [0] FP64, [1] FP32, [2] FP32 split records
+    DevBuf d_reclo, d_sblk32s, d_mrange, h_mrange;  // split records' d - RN32(d); mask range (device, pinned host)
     bool unit_w = false;  // the last catalogue came without weights (w = 1)
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
@@ -2718,6 +2796,11 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         std::vector<float> rec4f(rec4.begin(), rec4.end());  // the mixed path's FP32 records
         if ((rc = P->d_rec4f.ensure(sizeof(float) * rec4f.size()))) return rc;
         HIP_TRY(hipMemcpy(P->d_rec4f.p, rec4f.data(), sizeof(float) * rec4f.size(), hipMemcpyHostToDevice));
+        // split records: the remainder of d (the knot value) below its FP32 rounding
+        std::vector<float> reclo(rec4.size() / 4);
+        for (size_t e = 0; e < reclo.size(); ++e) reclo[e] = (float)(rec4[4 * e] - (double)(float)rec4[4 * e]);
+        if ((rc = P->d_reclo.ensure(sizeof(float) * reclo.size()))) return rc;
+        HIP_TRY(hipMemcpy(P->d_reclo.p, reclo.data(), sizeof(float) * reclo.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P->d_qorder.p, qorder.data(), sizeof(int32_t) * qorder.size(), hipMemcpyHostToDevice));
         // scatter block list: per XCD, its pixel chunks, group by group, maps in
         // pairs of the same group when two record tables fit in LDS
@@ -2730,6 +2813,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             while (nm < (f ? 4 : 2) && (int64_t)(2 * nm) * (P->tbl_cap + 1) * rb <= P->lds_max) nm *= 2;
             P->sc_nm[f] = nm;
         }
+        // split FP32 records (20 B: the record and the remainder of d): up to three per block
+        P->sc_nm[2] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(3, P->lds_max / (20 * (int64_t)(P->tbl_cap + 1))));
         // pixel chunks per map: one per XCD.  Every block stages its maps' record tables
         // (up to 154 KB) once, so fewer, longer chunks move less from L2: at nside 128,
         // 8 chunks ran FP64 2.73 ms, 16: 2.83, 32: 3.13, 64: 3.53 (coalesced delta stores)
@@ -2762,7 +2847,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
             HIP_TRY(hipMemcpy(P->d_vgs.p, vgs.data(), sizeof(int32_t) * vgs.size(), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(P->d_vrange.p, vr.data(), sizeof(int2) * vr.size(), hipMemcpyHostToDevice));
         }
-        for (int f = 0; f < 2; ++f) {
+        for (int f = 0; f < 3; ++f) {
             const size_t step = (size_t)P->sc_nm[f];
             std::vector<ScatterBlock> per[8];
             for (int x = 0; x < 8; ++x)
@@ -2783,7 +2868,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
                     return fail(SFB_ERR_ARG, "internal: uneven scatter blocks");
                 all.insert(all.end(), per[x].begin(), per[x].end());
             }
-            DevBuf& d = f ? P->d_sblk32 : P->d_sblk;
+            DevBuf& d = f == 2 ? P->d_sblk32s : (f ? P->d_sblk32 : P->d_sblk);
             if ((rc = d.ensure(sizeof(ScatterBlock) * all.size()))) return rc;
             HIP_TRY(hipMemcpy(d.p, all.data(), sizeof(ScatterBlock) * all.size(), hipMemcpyHostToDevice));
         }
@@ -3173,17 +3258,16 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         P->logM = 0;
         while ((1LL << P->logM) < M) ++P->logM;
         const int64_t budget = std::min<int64_t>(P->lds_max, 98304);
-        // 2 TQF sequences of M complex values plus the N4 + 1 twiddles, TQF a power of
-        // two: FP32 takes 2 x min(2, fit) (about six blocks resident per CU at nside 128);
-        // FP64 takes up to 4 while a block stays within 48 KB (three blocks per CU),
-        // which measured faster than 2 at nside 128 (1.65 vs 1.74 ms for 2514 maps)
+        // 2 TQF sequences of M complex FP64 values plus the N4 + 1 twiddles, TQF a power of
+        // two: up to 4 while a block stays within 48 KB (three blocks per CU), which measured
+        // faster than 2 at nside 128 (1.65 vs 1.74 ms for 2514 maps).  The mixed path's FFT
+        // also computes in FP64, so its default is the same (the tile search then picks).
         const int64_t fit = (budget - 2 * N4 * 16) / (2 * M * 16);
-        const int32_t tq2 = (int32_t)std::max<int64_t>(1, std::min<int64_t>(2, fit));
-        P->TQF32 = 2 * tq2;
-        P->TQF = tq2;
+        P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(2, fit));
         while (P->TQF < 4 && 2 * P->TQF <= fit &&
                16 * (2 * 2 * P->TQF * fft_plen(M, 8) + 2 * M + 1) <= 49152)
             P->TQF *= 2;
+        P->TQF32 = P->TQF;
     }
     hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -3218,7 +3302,8 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_pwin, &P->d_vgm, &P->d_vgs, &P->d_vrange, &P->d_lamw, &P->d_desc_cal,
                       &P->d_desc_cal32, &P->d_mall, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
-                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono, &P->d_desc_synf32};
+                      &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono, &P->d_desc_synf32,
+                      &P->d_reclo, &P->d_sblk32s, &P->d_mrange, &P->h_mrange};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -3364,7 +3449,7 @@ template <typename R> struct ScatterNM;
 template <> struct ScatterNM<double> { static constexpr int MAX = 2; };  // two 32-byte tables fill LDS
 template <> struct ScatterNM<float> { static constexpr int MAX = 4; };
 
-template <typename R, int NM>
+template <typename R, int NM, bool SPLIT = false>
 static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size_t shm, hipStream_t st,
                              int32_t spc, int64_t T, int32_t tstride, const ScatterBlock* blk, int32_t per_xcd,
                              const R* sw, const typename Vec4T<R>::T* rec, double inv, const double* d_wa,
@@ -3376,12 +3461,62 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
                            P->d_pinfo.as<int2>(), P->d_smin.as<int32_t>(), P->d_pwin.as<double>(),
                            P->d_soff.as<int64_t>(), sw,
                            P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
-                           d_wb, P->lmax, P->d_delta.as<R>(), (int32_t)(P->delta_perm != nullptr));
+                           d_wb, P->lmax, P->d_delta.as<R>(), (int32_t)(P->delta_perm != nullptr),
+                           P->d_reclo.as<float>());
         LAUNCH_CHECK();
         return SFB_OK;
     };
-    if (dense) return unit ? go(k_scatter_it<true, NM, true, R>) : go(k_scatter_it<true, NM, false, R>);
-    return unit ? go(k_scatter_it<false, NM, true, R>) : go(k_scatter_it<false, NM, false, R>);
+    if (dense)
+        return unit ? go(k_scatter_it<true, NM, true, R, SPLIT>) : go(k_scatter_it<true, NM, false, R, SPLIT>);
+    return unit ? go(k_scatter_it<false, NM, true, R, SPLIT>) : go(k_scatter_it<false, NM, false, R, SPLIT>);
+}
+
+// the mixed path's split-record scatter (FP32 records plus the remainder of d)
+static int launch_scatter_split(sfb_plan* P, bool dense, hipStream_t st, int64_t T, const float* sw, double inv,
+                                const double* d_wa, const double* d_wb)
+{
+    const int nm = P->sc_nm[2];
+    const int32_t tstride = P->tbl_cap + 1;
+    const size_t shm = (size_t)tstride * 20 * (size_t)nm;
+    const int32_t spc = (int32_t)((P->sell_nslice + P->sc_nchunk_it - 1) / P->sc_nchunk_it);
+    const dim3 grid((unsigned)(8 * P->sc_per_xcd[2]));
+    const ScatterBlock* blk = P->d_sblk32s.as<ScatterBlock>();
+    const int32_t per = P->sc_per_xcd[2];
+    const bool unit = P->unit_w;
+    const float4* rec = P->d_rec4f.as<float4>();
+    if (nm == 3)
+        return launch_scatter_nm<float, 3, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec,
+                                                 inv, d_wa, d_wb);
+    if (nm == 2)
+        return launch_scatter_nm<float, 2, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec,
+                                                 inv, d_wa, d_wb);
+    return launch_scatter_nm<float, 1, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv,
+                                             d_wa, d_wb);
+}
+
+// smallest and largest value of n doubles (one block): the separable mask's range
+__global__ __launch_bounds__(1024) void k_range(int64_t n, const double* __restrict__ v, double* __restrict__ out)
+{
+    __shared__ double mn[1024], mx[1024];
+    double a = INFINITY, b = -INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        a = fmin(a, v[i]);
+        b = fmax(b, v[i]);
+    }
+    mn[threadIdx.x] = a;
+    mx[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + o]);
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = mn[0];
+        out[1] = mx[0];
+    }
 }
 
 template <typename R>
@@ -3461,7 +3596,25 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                P->d_pinfo.as<int2>(), d_wa, P->d_pwin.as<double>());
             LAUNCH_CHECK();
         }
-        if (f32)
+        // Mixed path: the FP32 records' rounding is the same for every galaxy in a segment,
+        // so its error sums coherently over the catalogue.  With a sky-uniform window that
+        // coherent part lands in the monopole only (corrected in FP64 afterwards); a mask
+        // or a dense window carries it into the low-l modes its pattern holds, where the
+        // window term cancels the galaxies' mean (measured: 7e-6 of the l = 1 blocks under
+        // a hemisphere mask at nside 256), so there the records keep the remainder of d.
+        bool split = dense;
+        if (f32 && !dense) {
+            if ((rc = P->d_mrange.ensure(16)) || (rc = P->h_mrange.ensure_host(16))) return rc;
+            hipLaunchKernelGGL(k_range, dim3(1), dim3(1024), 0, st, P->npix, d_wa, P->d_mrange.as<double>());
+            LAUNCH_CHECK();
+            HIP_TRY(hipMemcpyAsync(P->h_mrange.p, P->d_mrange.p, 16, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            const double* mr = P->h_mrange.as<double>();
+            split = !(mr[0] == mr[1]);
+        }
+        if (f32 && split)
+            rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb);
+        else if (f32)
             rc = launch_scatter_it<float>(P, dense, st, T, P->d_swf.as<float>(), P->d_rec4f.as<float4>(), inv, d_wa,
                                           d_wb);
         else
@@ -3579,16 +3732,16 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     const int32_t k_eq0 = (int32_t)(nside - 1);  // first ring pair with 4 nside pixels
     const int64_t M = 2 * nside;
     const size_t vb = f32 ? sizeof(float) : sizeof(double);
-    // FP32 sequences take half the LDS: twice as many per block at the same occupancy
+    // the FFT computes in FP64 on both paths (FP32 maps and X on the mixed one)
     const int32_t tqf = f32 ? P->TQF32 : P->TQF;
-    const size_t shf = 2 * vb * (2 * (size_t)tqf * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
+    const size_t shf = 2 * 8 * (2 * (size_t)tqf * (size_t)fft_plen(M, 8) + 2 * (size_t)M + 1);
     // 1-D grid over (map tile, ring pair) units, rounded up to whole runs of 8 units on 8 XCDs
     // (runs of 2, 4 and 8 were timed: FP32 0.92, 0.87, 0.82 ms; the polar rings keep runs of 4)
     const int32_t nring_f = K - k_eq0, ntile_f = (int32_t)nblk(nq, tqf);
     const dim3 grid_f((unsigned)(64 * (((int64_t)nring_f * ntile_f + 63) / 64)));
     if (f32) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
-        hipLaunchKernelGGL(k_phi_fft<float>, grid_f, dim3(P->fft_nt32), shf, st,
+        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
+        hipLaunchKernelGGL((k_phi_fft<double, float>), grid_f, dim3(P->fft_nt32), shf, st,
                            (const float*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
                            P->d_tw.as<double2>(), P->d_X0.as<float>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     } else {
@@ -3699,7 +3852,7 @@ static int autotune_phi(sfb_plan* P)
         int32_t btq = tq, bnt = nt;
         float bms = 1e30f;
         for (int32_t cq = 1; cq <= 8 && !rc; cq *= 2) {
-            const size_t sh = 2 * vb * (2 * (size_t)cq * (size_t)fft_plen(M, vb) + 2 * (size_t)M + 1);
+            const size_t sh = 2 * 8 * (2 * (size_t)cq * (size_t)fft_plen(M, 8) + 2 * (size_t)M + 1);
             if ((int64_t)sh > P->lds_max) break;
             for (int32_t cn = 256; cn <= 512 && !rc; cn *= 2) {
                 tq = cq;

@@ -140,3 +140,24 @@ def test_mixed_switch_back_to_fp64(c2):
     q.cat2amln(cat[:, :1000], nbar, win)
     q.set_precision("fp64")
     assert np.array_equal(f1, q.cat2amln(cat[:, :1000], nbar, win))
+
+
+def test_mixed_c1_masked_dense_window():
+    """A dense window (hemisphere mask times a radial profile, as win_rhat_ln's
+    (npix, lmax+1, nmax) array) on a northern-hemisphere catalogue: the mixed path keeps the
+    remainder of each spline record's knot value (the window's pattern would otherwise
+    carry the records' FP32 rounding into the low-l modes), against the oracle."""
+    am = S.AnlmModes(0.05, 500.0, 1000.0, nside=32)
+    wm = S.ConfigurationSpaceModes(500.0, 1000.0, 200, 32)
+    th, _ = O.pix2ang_ring(32, np.arange(12 * 32 ** 2))
+    mask = (th < np.pi / 2).astype(np.float64)
+    phi = np.exp(-(wm.r - 750.0) ** 2 / (2 * 150.0 ** 2))
+    dense = S.win_rhat_ln(phi[:, None] * mask[None, :], wm, am, device=0)
+    dense = np.nan_to_num(dense, nan=0.0)
+    cat = uniform_shell_catalog(100_000, 500.0, 1000.0, seed=21, theta_max=np.pi / 2)
+    nbar = cat.shape[1] / shell_volume(500.0, 1000.0) * 2
+    ref = O.cat2amln(oracle_view(am), cat, nbar, 0, dense.reshape(-1, order="F"))
+    a = S.Plan(am, precision="mixed").cat2amln(cat, nbar, dense)
+    e = block_errors(a, ref, am)
+    say("C1 mixed, masked dense window, worst blocks", worst(e))
+    assert max(e.values()) <= BLOCK_BAR
