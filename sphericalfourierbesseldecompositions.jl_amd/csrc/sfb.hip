@@ -428,11 +428,12 @@ __global__ __launch_bounds__(1024) void k_scan_i64(int32_t n, const int64_t* __r
     if (t == 1023) out[n] = part[1023];
 }
 
-// galaxy k (pixel order) -> its SELL position
+// galaxy k (pixel order) -> its SELL position (slots past the arrays' capacity are not
+// written: the call sees the overflow flag and is redone with exact sizes)
 __global__ void k_sell_fill(int64_t N, const uint32_t* __restrict__ pix_s, const int32_t* __restrict__ pstart,
                             const uint32_t* __restrict__ invperm, const int64_t* __restrict__ soff,
                             const double* __restrict__ r_s, const double* __restrict__ w_s, double* __restrict__ sr,
-                            double* __restrict__ sw)
+                            double* __restrict__ sw, int64_t tcap)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
@@ -440,8 +441,22 @@ __global__ void k_sell_fill(int64_t N, const uint32_t* __restrict__ pix_s, const
     const int64_t j = k - pstart[p];
     const uint32_t e = invperm[p];
     const int64_t dst = soff[e >> 6] + 64 * j + (e & 63);
+    if (dst >= tcap) return;
     sr[dst] = r_s[k];
     sw[dst] = w_s[k];
+}
+
+// padding of the T = *Tdev slots (r = -inf, w = 0: they add an exact +-0 to every pixel
+// sum); flag[2] = 1 when T exceeds the capacity tcap
+__global__ void k_sell_pad(int64_t tcap, const int64_t* __restrict__ Tdev, double* __restrict__ sr,
+                           double* __restrict__ sw, int32_t* __restrict__ flag)
+{
+    const int64_t T = *Tdev;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && T > tcap) flag[2] = 1;
+    if (i >= min(T, tcap)) return;
+    sr[i] = -INFINITY;
+    sw[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -470,13 +485,18 @@ struct GroupInfo {
 // of 16 waves per CU instead of four of 4 (0.54 -> 0.40 ms FP64 for 195 groups).
 // i is 16-bit: the sliced layout is only used when a map's records fit in LDS
 // (nk + 1 <= 5120 FP64 records).
+// The slot total T is read from device memory (the catalogue stage's prefix sum, no host
+// round trip); the group arrays have stride Tcap >= T, and gridDim.x blocks split the slots.
 template <typename TT>
-__global__ __launch_bounds__(1024) void k_seg_t(int64_t T, int64_t per_blk, const double* __restrict__ sr,
+__global__ __launch_bounds__(1024) void k_seg_t(int64_t Tcap, const int64_t* __restrict__ Tdev,
+                                               const double* __restrict__ sr,
                                                const GroupInfo* __restrict__ groups, const double* __restrict__ gx,
                                                const double* __restrict__ grh, uint16_t* __restrict__ it_i,
                                                TT* __restrict__ it_t)
 {
     extern __shared__ double sm_seg[];
+    const int64_t T = min(*Tdev, Tcap);
+    const int64_t per_blk = ((T + gridDim.x - 1) / gridDim.x + 1023) / 1024 * 1024;
     const int g = blockIdx.y;
     const GroupInfo G = groups[g];
     double* xs = sm_seg;
@@ -487,8 +507,8 @@ __global__ __launch_bounds__(1024) void k_seg_t(int64_t T, int64_t per_blk, cons
     }
     __syncthreads();
     const int64_t e0 = (int64_t)blockIdx.x * per_blk, e1 = min(T, e0 + per_blk);
-    uint16_t* oi = it_i + (int64_t)g * T;
-    TT* ot = it_t + (int64_t)g * T;
+    uint16_t* oi = it_i + (int64_t)g * Tcap;
+    TT* ot = it_t + (int64_t)g * Tcap;
     for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += 4 * (int64_t)blockDim.x) {
         double rr[4];
 #pragma unroll
@@ -625,7 +645,8 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_r
 // w and the records rounded to FP32, FP32 Horner and w g products, FP64 pixel sums
 // (an FP32 running sum cost up to 4e-7 of an (n,l) block at nside 128), FP32 delta.
 template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false>
-__global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t T,
+__global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t Tcap,
+                                                     const int64_t* __restrict__ Tdev, const double* __restrict__ msel,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
                                                      int32_t tstride, const int2* __restrict__ pinfo,
                                                      const int32_t* __restrict__ smin,
@@ -640,9 +661,13 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                                                      const float* __restrict__ reclo)
 {
     typedef typename Vec4T<R>::T R4;
+    // msel (mixed path): the window's mask range (min, max); the split-record kernel runs when
+    // the mask varies, the plain FP32 one when it does not (both are launched, one leaves)
+    if (msel && SPLIT != (msel[0] != msel[1])) return;
     extern __shared__ double4 sm_raw[];
     R4* sm4 = (R4*)sm_raw;
     float* lo_sm = (float*)(sm4 + NM * tstride);  // SPLIT: the records' d - RN32(d), per table
+    const int64_t T = __builtin_amdgcn_readfirstlane((int)min(*Tdev, Tcap));  // slots (< 2^29)
     const ScatterBlock B = blk[(int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3)];
     bool has[NM];
     int64_t col[NM];
@@ -665,8 +690,8 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
     }
-    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * T, (uint32_t)(2 * T));
-    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * T, (uint32_t)(sizeof(R) * T));
+    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * Tcap, (uint32_t)(2 * T));
+    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * Tcap, (uint32_t)(sizeof(R) * T));
     const __amdgpu_buffer_rsrc_t gw = slot_rsrc(sw, (uint32_t)(sizeof(R) * T));
     // wave-uniform slice bookkeeping (scalar registers and loads)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -2514,27 +2539,11 @@ struct DevBuf {
         bytes = want;
         return SFB_OK;
     }
-    bool pinned = false;  // pinned host memory (ensure_host) rather than device memory
-    int ensure_host(size_t need)
-    {
-        if (need <= bytes && p) return SFB_OK;
-        release();
-        if (sfb::fault("alloc")) return fail(SFB_ERR_DEVICE, "hipHostMalloc(%zu) failed: forced by SFB_FAULT=alloc", need);
-        hipError_t e = hipHostMalloc(&p, need ? need : 16, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(SFB_ERR_DEVICE, "hipHostMalloc(%zu) failed: %s", need, hipGetErrorString(e));
-        }
-        pinned = true;
-        bytes = need ? need : 16;
-        return SFB_OK;
-    }
     void release()
     {
-        if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
-        pinned = false;
     }
     template <class T> T* as() const { return (T*)p; }
 };
@@ -2593,7 +2602,8 @@ struct sfb_plan {
     // pixel p of every map at slot perm[p] (the catalogue stage's inverse permutation)
     const uint32_t* delta_perm = nullptr;
     int32_t sell_nslice = 0;
-    int64_t sell_total = 0;
+    int64_t sell_cap = 0;   // capacity of the sliced layout's slot arrays (the slot total's bound)
+    int64_t sell_hint = 0;  // the exact slot total of a call that overflowed the bound
     DevBuf d_skey, d_skey2, d_sval, d_perm, d_invperm, d_swidth, d_soff, d_sr, d_sw;
     DevBuf d_smin;  // per slice: the smallest galaxy count of its lanes
     DevBuf d_pinfo, d_pwin;  // per slice lane: (pixel, count), separable window factor
@@ -2604,7 +2614,8 @@ struct sfb_plan {
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
     int32_t sc_nchunk_it = 0;
     int32_t sc_nm[3] = {1, 1, 1}, sc_per_xcd[3] = {0, 0, 0};  // [0] FP64, [1] FP32, [2] FP32 split records
-    DevBuf d_reclo, d_sblk32s, d_mrange, h_mrange;  // split records' d - RN32(d); mask range (device, pinned host)
+    DevBuf d_reclo, d_sblk32s, d_mrange;  // split records' d - RN32(d); the separable mask's range
+    DevBuf d_bflag;                       // batched mocks: each mock's flags
     bool unit_w = false;  // the last catalogue came without weights (w = 1)
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
@@ -3068,7 +3079,7 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
     if ((rc = P->d_a.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
     if ((rc = P->d_a1.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
     if ((rc = P->d_a2.ensure(sizeof(double) * (size_t)P->a_size))) return rc;
-    if ((rc = P->d_flag.ensure(2 * sizeof(int32_t)))) return rc;
+    if ((rc = P->d_flag.ensure(4 * sizeof(int32_t)))) return rc;  // non-finite, angle, slot overflow
 
     // GEMM descriptors.  First analysis: A = X0 (k = ring pair), B = Lam, C = b (d_a).
     // Iteration slot s reads a_in and writes a_out:
@@ -3303,7 +3314,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_desc_cal32, &P->d_mall, &P->d_lamT32, &P->d_ic032, &P->d_lamw32,
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono, &P->d_desc_synf32,
-                      &P->d_reclo, &P->d_sblk32s, &P->d_mrange, &P->h_mrange};
+                      &P->d_reclo, &P->d_sblk32s, &P->d_mrange, &P->d_bflag};
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -3337,7 +3348,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     P->ncat = N;
     P->unit_w = d_w == nullptr;
     if ((rc = P->d_pstart.ensure(sizeof(int32_t) * (size_t)(P->npix + 1)))) return rc;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 2, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(), 4, 0);
     LAUNCH_CHECK();
     if (N == 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->d_pstart.as<int32_t>(),
@@ -3421,24 +3432,26 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
         hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, P->sell_nslice, P->d_swidth.as<int64_t>(),
                            P->d_soff.as<int64_t>());
         LAUNCH_CHECK();
-        int64_t total = 0;
-        HIP_TRY(hipMemcpyAsync(&total, P->d_soff.as<int64_t>() + P->sell_nslice, sizeof(int64_t),
-                               hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        P->sell_total = total;
-        P->sell_fits = total < ((int64_t)1 << 29);
-        if ((rc = P->d_sr.ensure(8 * (size_t)std::max<int64_t>(total, 1))) ||
-            (rc = P->d_sw.ensure(8 * (size_t)std::max<int64_t>(total, 1))))
-            return rc;
-        // padding slots: r = -inf (the y_first pseudo-record with t = -0, a finite
-        // value) and w = 0, so they add an exact +-0 to every running pixel sum
-        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(total, 256)), dim3(256), 0, st, P->d_sr.as<double>(), total,
-                           -INFINITY);
-        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(total, 256)), dim3(256), 0, st, P->d_sw.as<double>(), total, 0.0);
+        // The slot total T stays on the device (no host round trip): the slot arrays get a
+        // capacity from a bound.  Pixels are sorted by count inside each window of 1024, so a
+        // slice is never wider than the previous slice's mean count, and T <= N + 64 x (the
+        // largest count of every window); the bound takes the Poisson tail of a uniform
+        // catalogue's largest count (or the exact T of an earlier call that overflowed).
+        const int64_t nwin = (P->npix + SELL_SIGMA - 1) / SELL_SIGMA;
+        const double cbar = (double)N / (double)P->npix;
+        const int64_t cmax = (int64_t)(cbar + 8.0 * std::sqrt(cbar) + 16.0);
+        const int64_t tcap = std::max<int64_t>(N + 64 * nwin * cmax, P->sell_hint);
+        P->sell_cap = tcap;
+        P->sell_fits = tcap < ((int64_t)1 << 29);
+        if ((rc = P->d_sr.ensure(8 * (size_t)tcap)) || (rc = P->d_sw.ensure(8 * (size_t)tcap))) return rc;
+        const int64_t* Tdev = P->d_soff.as<int64_t>() + P->sell_nslice;
+        hipLaunchKernelGGL(k_sell_pad, dim3(nblk(tcap, 256)), dim3(256), 0, st, tcap, Tdev, P->d_sr.as<double>(),
+                           P->d_sw.as<double>(), P->d_flag.as<int32_t>());
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_sell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_pixkey2.as<uint32_t>(),
                            P->d_pstart.as<int32_t>(), P->d_invperm.as<uint32_t>(), P->d_soff.as<int64_t>(),
-                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_sr.as<double>(), P->d_sw.as<double>());
+                           P->d_rs.as<double>(), P->d_ws.as<double>(), P->d_sr.as<double>(), P->d_sw.as<double>(),
+                           tcap);
         LAUNCH_CHECK();
     }
     return SFB_OK;
@@ -3453,11 +3466,12 @@ template <typename R, int NM, bool SPLIT = false>
 static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size_t shm, hipStream_t st,
                              int32_t spc, int64_t T, int32_t tstride, const ScatterBlock* blk, int32_t per_xcd,
                              const R* sw, const typename Vec4T<R>::T* rec, double inv, const double* d_wa,
-                             const double* d_wb)
+                             const double* d_wb, const double* msel)
 {
     auto go = [&](auto kern) -> int {
         HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T, blk, per_xcd, tstride,
+        hipLaunchKernelGGL(kern, grid, dim3(1024), shm, st, P->npix, P->sell_nslice, spc, T,
+                           P->d_soff.as<int64_t>() + P->sell_nslice, msel, blk, per_xcd, tstride,
                            P->d_pinfo.as<int2>(), P->d_smin.as<int32_t>(), P->d_pwin.as<double>(),
                            P->d_soff.as<int64_t>(), sw,
                            P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
@@ -3473,7 +3487,7 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
 
 // the mixed path's split-record scatter (FP32 records plus the remainder of d)
 static int launch_scatter_split(sfb_plan* P, bool dense, hipStream_t st, int64_t T, const float* sw, double inv,
-                                const double* d_wa, const double* d_wb)
+                                const double* d_wa, const double* d_wb, const double* msel)
 {
     const int nm = P->sc_nm[2];
     const int32_t tstride = P->tbl_cap + 1;
@@ -3486,12 +3500,12 @@ static int launch_scatter_split(sfb_plan* P, bool dense, hipStream_t st, int64_t
     const float4* rec = P->d_rec4f.as<float4>();
     if (nm == 3)
         return launch_scatter_nm<float, 3, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec,
-                                                 inv, d_wa, d_wb);
+                                                 inv, d_wa, d_wb, msel);
     if (nm == 2)
         return launch_scatter_nm<float, 2, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec,
-                                                 inv, d_wa, d_wb);
+                                                 inv, d_wa, d_wb, msel);
     return launch_scatter_nm<float, 1, true>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv,
-                                             d_wa, d_wb);
+                                             d_wa, d_wb, msel);
 }
 
 // smallest and largest value of n doubles (one block): the separable mask's range
@@ -3521,7 +3535,8 @@ __global__ __launch_bounds__(1024) void k_range(int64_t n, const double* __restr
 
 template <typename R>
 static int launch_scatter_it(sfb_plan* P, bool dense, hipStream_t st, int64_t T, const R* sw,
-                             const typename Vec4T<R>::T* rec, double inv, const double* d_wa, const double* d_wb)
+                             const typename Vec4T<R>::T* rec, double inv, const double* d_wa, const double* d_wb,
+                             const double* msel = nullptr)
 {
     const int f = sizeof(R) == 4;
     const int nm = std::min(P->sc_nm[f], ScatterNM<R>::MAX);
@@ -3535,12 +3550,12 @@ static int launch_scatter_it(sfb_plan* P, bool dense, hipStream_t st, int64_t T,
     const bool unit = P->unit_w;
     if (nm == 4)
         return launch_scatter_nm<R, ScatterNM<R>::MAX>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw,
-                                                        rec, inv, d_wa, d_wb);
+                                                        rec, inv, d_wa, d_wb, msel);
     if (nm == 2)
         return launch_scatter_nm<R, 2>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv, d_wa,
-                                       d_wb);
+                                       d_wb, msel);
     return launch_scatter_nm<R, 1>(P, dense, unit, grid, shm, st, spc, T, tstride, blk, per, sw, rec, inv, d_wa,
-                                   d_wb);
+                                   d_wb, msel);
 }
 
 
@@ -3555,16 +3570,17 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         // each slice end stores one contiguous row per map (FP64 4.2 -> 3.5 ms, FP32 2.35 ->
         // 2.10 ms at C2) and the phi-transform gathers through the permutation (+0.14 ms).
         if (64 * (int64_t)P->sell_nslice == P->npix) P->delta_perm = P->d_invperm.as<uint32_t>();
-        // shared segment search per knot group, then three fma per map and galaxy
-        const int64_t T = std::max<int64_t>(P->sell_total, 1);
+        // shared segment search per knot group, then three fma per map and galaxy.  The slot
+        // total stays on the device; arrays are sized by the catalogue stage's bound Tcap.
+        const int64_t T = std::max<int64_t>(P->sell_cap, 1);
+        const int64_t* Tdev = P->d_soff.as<int64_t>() + P->sell_nslice;
         int rc;
         PhaseTimer tseg(P, SFB_PHASE_SEGMENT, st);
         if ((rc = P->d_it_i.ensure(sizeof(uint16_t) * (size_t)(T * P->ngroup))) ||
             (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
             return rc;
         // 64 slot ranges per group; LDS holds the group's knots and 1/h
-        const int64_t per_blk = ((T + 63) / 64 + 1023) / 1024 * 1024;
-        const dim3 gseg((unsigned)((T + per_blk - 1) / per_blk), (unsigned)P->ngroup);
+        const dim3 gseg(64, (unsigned)P->ngroup);
         const size_t shm_seg = 2 * sizeof(double) * (size_t)P->tbl_cap;
         HIP_TRY(hipFuncSetAttribute((const void*)k_seg_t<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)shm_seg));
@@ -3577,11 +3593,11 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                    P->d_swf.as<float>());
                 LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(1024), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(1024), shm_seg, st, T, Tdev, P->d_sr.as<double>(),
                                P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
                                P->d_it_i.as<uint16_t>(), P->d_it_t.as<float>());
         } else {
-            hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(1024), shm_seg, st, T, per_blk, P->d_sr.as<double>(),
+            hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(1024), shm_seg, st, T, Tdev, P->d_sr.as<double>(),
                                P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
                                P->d_it_i.as<uint16_t>(), P->d_it_t.as<double>());
         }
@@ -3602,24 +3618,23 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         // or a dense window carries it into the low-l modes its pattern holds, where the
         // window term cancels the galaxies' mean (measured: 7e-6 of the l = 1 blocks under
         // a hemisphere mask at nside 256), so there the records keep the remainder of d.
-        bool split = dense;
-        if (f32 && !dense) {
-            if ((rc = P->d_mrange.ensure(16)) || (rc = P->h_mrange.ensure_host(16))) return rc;
+        // A separable window's mask range is found on the device and both kernels are
+        // launched: the one that does not apply leaves at once (no host round trip).
+        if (f32 && dense) {
+            rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb, nullptr);
+        } else if (f32) {
+            if ((rc = P->d_mrange.ensure(16))) return rc;
             hipLaunchKernelGGL(k_range, dim3(1), dim3(1024), 0, st, P->npix, d_wa, P->d_mrange.as<double>());
             LAUNCH_CHECK();
-            HIP_TRY(hipMemcpyAsync(P->h_mrange.p, P->d_mrange.p, 16, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            const double* mr = P->h_mrange.as<double>();
-            split = !(mr[0] == mr[1]);
-        }
-        if (f32 && split)
-            rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb);
-        else if (f32)
-            rc = launch_scatter_it<float>(P, dense, st, T, P->d_swf.as<float>(), P->d_rec4f.as<float4>(), inv, d_wa,
-                                          d_wb);
-        else
+            const double* msel = P->d_mrange.as<double>();
+            rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb, msel);
+            if (!rc)
+                rc = launch_scatter_it<float>(P, dense, st, T, P->d_swf.as<float>(), P->d_rec4f.as<float4>(), inv,
+                                              d_wa, d_wb, msel);
+        } else {
             rc = launch_scatter_it<double>(P, dense, st, T, P->d_sw.as<double>(), P->d_rec4.as<double4>(), inv,
                                            d_wa, d_wb);
+        }
         if (rc) return rc;
         LAUNCH_CHECK();
         return SFB_OK;
@@ -4042,6 +4057,17 @@ static int check_window(const sfb_plan* P, int32_t win_kind, const double* wa, c
     return SFB_OK;
 }
 
+// After a call whose sliced layout overflowed its capacity (flag[2]): the exact slot
+// total becomes the plan's capacity hint (it only grows), for the call to be redone.
+static int take_slot_total(sfb_plan* P)
+{
+    int64_t T = 0;
+    HIP_TRY(hipMemcpy(&T, P->d_soff.as<int64_t>() + P->sell_nslice, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (T <= P->sell_cap) return fail(SFB_ERR_ARG, "internal: slot overflow flagged below the capacity");
+    P->sell_hint = T;
+    return SFB_OK;
+}
+
 // Mixed path, l = 0 maps: their a_{n00} corrected to FP64 quality (k_mono_fix).  A monopole
 // is a difference of nearly equal sums: the galaxy counts add up to N exactly, so every
 // per-pixel count fluctuation cancels in it and what remains is the radial sampling noise,
@@ -4089,20 +4115,15 @@ static int monopole_fix(sfb_plan* P, int64_t N, double nbar, int32_t win_kind, c
     return SFB_OK;
 }
 
-extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, double nbar,
-                                   int32_t win_kind, const double* d_win_a, const double* d_win_b, double* d_out,
-                                   void* stream)
+// One cat2amln enqueued on stream st with no host synchronisation: its flags (non-finite
+// a_nlm, non-finite angle, slot overflow) are left in d_flag[0..2].
+static int cat2amln_enqueue(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                            int32_t win_kind, const double* d_win_a, const double* d_win_b, double* d_out,
+                            hipStream_t st)
 {
-    sfb::clear_error();
-    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
-    if (N < 0 || (N > 0 && !d_rtp) || !d_out) return fail(SFB_ERR_ARG, "bad catalogue or output pointer");
-    int rc = check_window(P, win_kind, d_win_a, d_win_b);
-    if (rc) return rc;
-    HIP_TRY(hipSetDevice(P->device));
-    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    int rc;
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
-    // mixed path: FP32 spline values, delta maps and phi-transform when the FP32
-    // phi kernels fit (nside <= 1024), FP32 MFMA for every Legendre pass
+    // mixed path: FP32 spline values and delta maps, FP32 MFMA for the Jacobi passes
     size_t shp4;
     const bool f32 = P->precision == SFB_PREC_MIXED && polar4_fits(P, sizeof(float), &shp4);
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st, f32))) return rc;
@@ -4120,14 +4141,35 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
         LAUNCH_CHECK();
     }
     if (f32 && (rc = monopole_fix(P, N, nbar, win_kind, d_win_a, d_win_b, d_out, st))) return rc;
-    int32_t flag[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    return SFB_OK;
+}
+
+extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, double nbar,
+                                   int32_t win_kind, const double* d_win_a, const double* d_win_b, double* d_out,
+                                   void* stream)
+{
+    sfb::clear_error();
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (N < 0 || (N > 0 && !d_rtp) || !d_out) return fail(SFB_ERR_ARG, "bad catalogue or output pointer");
+    int rc = check_window(P, win_kind, d_win_a, d_win_b);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    if ((rc = cat2amln_enqueue(P, N, d_rtp, d_w, nbar, win_kind, d_win_a, d_win_b, d_out, st))) return rc;
+    int32_t flag[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (P->prof) {
         prof_collect(P);
         P->prof_calls += 1;
     }
     if (flag[1]) return fail(SFB_ERR_ARG, "InexactError: a galaxy angle (theta or phi) is not finite");
+    if (flag[2]) {
+        // the sliced layout outgrew its bound (a crowded catalogue): redo the call with the
+        // exact slot total, which the plan keeps as the capacity for later calls
+        if ((rc = take_slot_total(P))) return rc;
+        return sfb_cat2amln_device(P, N, d_rtp, d_w, nbar, win_kind, d_win_a, d_win_b, d_out, stream);
+    }
     if (flag[0]) return fail(SFB_ERR_NONFINITE, "AssertionError: all(isfinite.(anlm)) failed");
     return SFB_OK;
 }
@@ -4275,7 +4317,7 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
                        P->d_idx.as<uint32_t>(), P->d_ws.as<double>(), inv, d_dr, d_dw);
     LAUNCH_CHECK();
     tj.end();
-    int32_t flag[2] = {0, 0};
+    int32_t flag[2] = {0, 0};  // (the adjoint does not read the sliced layout: its overflow flag is moot)
     HIP_TRY(hipMemcpyAsync(flag, P->d_flag.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (P->prof) {
@@ -4336,6 +4378,12 @@ extern "C" int sfb_cat2amln_vjp(sfb_plan* P, int64_t N, const double* rtp, const
 // Batched mocks: B catalogues, concatenated, through one plan (its tables and
 // work buffers are reused; each catalogue is a full cat2amln).
 // ---------------------------------------------------------------------------
+// d_bflag[4 b .. 4 b + 2] = this mock's flags (a device-side copy: no host round trip)
+__global__ void k_save_flags(const int32_t* __restrict__ flag, int32_t* __restrict__ dst)
+{
+    if (threadIdx.x < 3) dst[threadIdx.x] = flag[threadIdx.x];
+}
+
 extern "C" int sfb_cat2amln_batch_device(sfb_plan* P, int32_t B, const int64_t* N_b, const double* d_rtp,
                                          const double* d_w, const double* nbar_b, int32_t win_kind,
                                          const double* d_win_a, const double* d_win_b, double* d_out, void* stream)
@@ -4344,21 +4392,60 @@ extern "C" int sfb_cat2amln_batch_device(sfb_plan* P, int32_t B, const int64_t* 
     if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
     if (B < 0 || (B > 0 && (!N_b || !nbar_b || !d_out))) return fail(SFB_ERR_ARG, "bad batch arguments");
     int64_t total = 0;
+    std::vector<int64_t> off((size_t)std::max(B, 0));
     for (int32_t b = 0; b < B; ++b) {
         if (N_b[b] < 0) return fail(SFB_ERR_ARG, "catalogue %d has N < 0", b);
+        off[(size_t)b] = total;
         total += N_b[b];
     }
     if (total > 0 && !d_rtp) return fail(SFB_ERR_ARG, "rtp is NULL");
-    int64_t off = 0;
-    for (int32_t b = 0; b < B; ++b) {
+    if (B == 0) return SFB_OK;
+    int rc = check_window(P, win_kind, d_win_a, d_win_b);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    // Every mock is enqueued back to back with no host synchronisation; each leaves its flags
+    // in d_bflag and the host reads them all once at the end.  The largest catalogue goes
+    // first so that its call grows every work buffer to the batch's largest size while the
+    // stream is otherwise idle; later calls reuse them (no allocation behind queued work).
+    std::vector<int32_t> order((size_t)B);
+    for (int32_t b = 0; b < B; ++b) order[(size_t)b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return N_b[x] > N_b[y]; });
+    if ((rc = P->d_bflag.ensure(sizeof(int32_t) * 4 * (size_t)B))) return rc;
+    for (int32_t ib = 0; ib < B; ++ib) {
+        const int32_t b = order[(size_t)ib];
         const int64_t n = N_b[b];
-        int rc = sfb_cat2amln_device(P, n, n ? d_rtp + 3 * off : nullptr, (d_w && n) ? d_w + off : nullptr, nbar_b[b],
-                                     win_kind, d_win_a, d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, stream);
-        if (rc) {
+        if ((rc = cat2amln_enqueue(P, n, n ? d_rtp + 3 * off[(size_t)b] : nullptr,
+                                   (d_w && n) ? d_w + off[(size_t)b] : nullptr, nbar_b[b], win_kind, d_win_a,
+                                   d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, st))) {
             std::string msg = sfb_last_error();
             return fail(rc, "catalogue %d: %s", b, msg.c_str());
         }
-        off += n;
+        hipLaunchKernelGGL(k_save_flags, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(),
+                           P->d_bflag.as<int32_t>() + 4 * (int64_t)b);
+        LAUNCH_CHECK();
+    }
+    std::vector<int32_t> flags(4 * (size_t)B);
+    HIP_TRY(hipMemcpyAsync(flags.data(), P->d_bflag.p, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (P->prof) {
+        prof_collect(P);
+        P->prof_calls += B;
+    }
+    for (int32_t b = 0; b < B; ++b) {
+        const int32_t* f = flags.data() + 4 * (size_t)b;
+        if (f[1]) return fail(SFB_ERR_ARG, "catalogue %d: InexactError: a galaxy angle (theta or phi) is not finite", b);
+        if (f[2]) {  // a crowded mock outgrew the slot bound: redone alone with exact sizes
+            const int64_t n = N_b[b];
+            if ((rc = sfb_cat2amln_device(P, n, n ? d_rtp + 3 * off[(size_t)b] : nullptr,
+                                          (d_w && n) ? d_w + off[(size_t)b] : nullptr, nbar_b[b], win_kind, d_win_a,
+                                          d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, stream))) {
+                std::string msg = sfb_last_error();
+                return fail(rc, "catalogue %d: %s", b, msg.c_str());
+            }
+        } else if (f[0]) {
+            return fail(SFB_ERR_NONFINITE, "catalogue %d: AssertionError: all(isfinite.(anlm)) failed", b);
+        }
     }
     return SFB_OK;
 }
@@ -4805,8 +4892,16 @@ extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const do
     HIP_TRY(hipSetDevice(P->device));
     const double *d_rtp, *d_w, *d_wa, *d_wb;
     if ((rc = stage_inputs(P, N, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
-    if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream))) return rc;
-    if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream))) return rc;
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream))) return rc;
+        if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream))) return rc;
+        int32_t over = 0;
+        HIP_TRY(hipMemcpyAsync(&over, P->d_flag.as<int32_t>() + 2, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        if (!over) break;
+        if (pass == 1) return fail(SFB_ERR_ARG, "internal: slot overflow after resizing");
+        if ((rc = take_slot_total(P))) return rc;
+    }
     HIP_TRY(hipMemcpyAsync(map_out, P->d_delta.as<double>() + q * P->npix, sizeof(double) * (size_t)P->npix,
                            hipMemcpyDeviceToHost, P->stream));
     if (P->delta_perm) {  // slot order: pixel p sits at slot perm[p]

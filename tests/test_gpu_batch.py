@@ -1,7 +1,10 @@
 """Batched mocks (sfb_cat2amln_batch): B catalogues of one survey geometry through
-one plan.  Every mock of the batch must equal its own cat2amln call bit for bit
-(same kernels, same summation order), including an empty catalogue, and one mock is
-checked against the oracle.  An error in one mock names it."""
+one plan, enqueued back to back with no host synchronisation between them.  Every
+mock of the batch must equal its own cat2amln call bit for bit (same kernels, same
+summation order), including an empty catalogue and a crowded one whose sliced layout
+outgrows the slot bound (it is redone alone with exact sizes), and one mock is checked
+against the oracle.  An error in one mock names it.  At the nside-128 geometry
+(kmax 0.1), eight Poisson mocks with means drawn in [1e5, 1e6], both precisions."""
 import numpy as np
 import pytest
 
@@ -24,6 +27,9 @@ def setup():
     sizes = [int(rng.poisson(mu)) for mu in rng.uniform(1e4, 1e5, 4)] + [0]
     cats = [uniform_shell_catalog(n, 500.0, 1000.0, seed=100 + b) if n else np.zeros((3, 0))
             for b, n in enumerate(sizes)]
+    # crowded: 30000 galaxies in one pixel (64 slots per galaxy row: far past the slot bound)
+    crowd = np.stack([rng.uniform(500.0, 1000.0, 30000), np.full(30000, 1.0), np.full(30000, 2.0)])
+    cats.append(crowd)
     weights = [rng.uniform(0.5, 1.5, c.shape[1]) for c in cats]
     nbars = [max(c.shape[1], 1) / shell_volume(500.0, 1000.0) for c in cats]
     return am, win, cats, weights, nbars
@@ -62,3 +68,26 @@ def test_batch_error_names_catalogue(setup):
     with pytest.raises(AssertionError) as e:
         S.get_plan(am).cat2amln_batch(cats[:3], nbars[:3], win, bad)
     assert "catalogue 2" in str(e.value)
+
+
+def test_batch_c2_geometry_poisson_mocks():
+    am = S.AnlmModes(0.1, 500.0, 1500.0, nside=128)
+    win, _ = separable_unit_window(S, am)
+    rng = np.random.default_rng(41)
+    sizes = [int(rng.poisson(mu)) for mu in rng.uniform(1e5, 1e6, 8)]
+    cats = [uniform_shell_catalog(n, 500.0, 1500.0, seed=200 + b) for b, n in enumerate(sizes)]
+    nbars = [c.shape[1] / shell_volume(500.0, 1500.0) for c in cats]
+    say("C2 batch of", sizes)
+    for prec in ("fp64", "mixed"):
+        plan = S.get_plan(am, precision=prec)
+        batch = plan.cat2amln_batch(cats, nbars, win)
+        for b, c in enumerate(cats):
+            assert np.array_equal(batch[b], plan.cat2amln(c, nbars[b], win)), (prec, b)
+        if prec == "fp64":
+            om = O.OracleModes.__new__(O.OracleModes)
+            om.nside, om.lmax, om.nmax, om.lmax_n = am.nside, am.lmax, am.nmax, am.lmax_n
+            om.knot_off, om.knot_x, om.knot_y, om.knot_ypp = am.knot_off, am.knot_x, am.knot_y, am.knot_ypp
+            ref = O.cat2amln(om, cats[3], nbars[3], 1, win.mask, win.w_ln.reshape(-1, order="F"), q_range=(0, 12))
+            n12 = int(np.sum(np.arange(1, 13)))  # maps 0..11 are n = 1, l = 0..11
+            d = np.abs(batch[3][:n12] - ref[:n12])
+            assert d.max() < 1e-10 * np.abs(ref[:n12]).max()
