@@ -141,6 +141,15 @@ int sfb_cat2amln_batch_device(sfb_plan* p, int32_t B, const int64_t* N_b, const 
                               const double* nbar_b, int32_t win_kind, const double* d_win_a,
                               const double* d_win_b, double* d_anlm_out, void* stream);
 
+/* Batched mocks as one captured HIP graph (no reference counterpart): with enable = 1 the
+ * batch entry points capture the whole batch into a graph and launch it, and replay that
+ * graph when a later batch call repeats the same sizes, n-bar values, pointers, window kind
+ * and precision (no plan buffer reallocated since).  Before a capture the largest mock runs
+ * once eagerly so that no buffer grows under capture.  enable = 0 enqueues the mocks
+ * directly (the default); enable < 0 leaves the setting.  *capture_ms (if not NULL)
+ * receives the wall time of the last capture and instantiation. */
+int sfb_plan_set_batch_graph(sfb_plan* p, int32_t enable, double* capture_ms);
+
 /* ---- catalogue preparation and the C_l(n, n') consumer ------------------------ */
 
 /* xyz2rtp(xyz) (src/SphericalFourierBesselDecompositions.jl:517-532): 3 x N Cartesian

@@ -27,6 +27,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -2518,12 +2519,17 @@ __global__ void k_amln2clnn(int64_t nlnn, const int64_t* __restrict__ lnn, const
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Bumped whenever a device buffer is (re)allocated: a captured batch graph holds buffer
+// addresses and is replayed only while this count is unchanged.
+static std::atomic<uint64_t> g_alloc_gen{0};
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     int ensure(size_t need)
     {
         if (need <= bytes && p) return SFB_OK;
+        g_alloc_gen.fetch_add(1);
         if (p) {
             (void)hipFree(p);
             p = nullptr;
@@ -2616,6 +2622,16 @@ struct sfb_plan {
     int32_t sc_nm[3] = {1, 1, 1}, sc_per_xcd[3] = {0, 0, 0};  // [0] FP64, [1] FP32, [2] FP32 split records
     DevBuf d_reclo, d_sblk32s, d_mrange;  // split records' d - RN32(d); the separable mask's range
     DevBuf d_bflag;                       // batched mocks: each mock's flags
+    // batched mocks as one captured HIP graph (sfb_plan_set_batch_graph): the last batch's
+    // graph and what it was captured for, replayed when a batch repeats that call exactly
+    bool batch_graph = false;
+    hipGraphExec_t bg_exec = nullptr;
+    std::vector<int64_t> bg_n;
+    std::vector<double> bg_nbar;
+    std::vector<const void*> bg_ptr;
+    int32_t bg_kind = -1, bg_prec = -1;
+    uint64_t bg_gen = 0;
+    double bg_capture_ms = 0.0;  // capture + instantiation time of the last graph
     bool unit_w = false;  // the last catalogue came without weights (w = 1)
     // per-phase event timing (sfb_plan_profile)
     bool prof = false;
@@ -3315,6 +3331,7 @@ extern "C" void sfb_plan_destroy(sfb_plan* P)
                       &P->d_desc_ana32, &P->d_desc_syn32, &P->d_qbase, &P->d_lmaxn, &P->d_gtab, &P->d_wB, &P->d_winT,
                       &P->d_rr, &P->d_desc_tmp, &P->h_fa, &P->h_fb, &P->h_fo, &P->d_mono, &P->d_desc_synf32,
                       &P->d_reclo, &P->d_sblk32s, &P->d_mrange, &P->d_bflag};
+    if (P->bg_exec) (void)hipGraphExecDestroy(P->bg_exec);
     for (DevBuf* b : bufs) b->release();
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -4412,18 +4429,78 @@ extern "C" int sfb_cat2amln_batch_device(sfb_plan* P, int32_t B, const int64_t* 
     for (int32_t b = 0; b < B; ++b) order[(size_t)b] = b;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return N_b[x] > N_b[y]; });
     if ((rc = P->d_bflag.ensure(sizeof(int32_t) * 4 * (size_t)B))) return rc;
-    for (int32_t ib = 0; ib < B; ++ib) {
-        const int32_t b = order[(size_t)ib];
-        const int64_t n = N_b[b];
-        if ((rc = cat2amln_enqueue(P, n, n ? d_rtp + 3 * off[(size_t)b] : nullptr,
-                                   (d_w && n) ? d_w + off[(size_t)b] : nullptr, nbar_b[b], win_kind, d_win_a,
-                                   d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, st))) {
-            std::string msg = sfb_last_error();
-            return fail(rc, "catalogue %d: %s", b, msg.c_str());
+    auto enqueue_all = [&]() -> int {
+        for (int32_t ib = 0; ib < B; ++ib) {
+            const int32_t b = order[(size_t)ib];
+            const int64_t n = N_b[b];
+            int r;
+            if ((r = cat2amln_enqueue(P, n, n ? d_rtp + 3 * off[(size_t)b] : nullptr,
+                                      (d_w && n) ? d_w + off[(size_t)b] : nullptr, nbar_b[b], win_kind, d_win_a,
+                                      d_win_b, d_out + (int64_t)b * 2 * P->nlmsize, st))) {
+                std::string msg = sfb_last_error();
+                return fail(r, "catalogue %d: %s", b, msg.c_str());
+            }
+            hipLaunchKernelGGL(k_save_flags, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(),
+                               P->d_bflag.as<int32_t>() + 4 * (int64_t)b);
+            HIP_TRY(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_save_flags, dim3(1), dim3(64), 0, st, P->d_flag.as<int32_t>(),
-                           P->d_bflag.as<int32_t>() + 4 * (int64_t)b);
-        LAUNCH_CHECK();
+        return SFB_OK;
+    };
+    if (!P->batch_graph) {
+        if ((rc = enqueue_all())) return rc;
+    } else {
+        // One HIP graph for the whole batch: replayed when the call repeats the previous one
+        // (same sizes, n-bar, pointers, precision, and no buffer reallocated since), else
+        // captured anew.  Before a capture the largest mock runs once eagerly so every work
+        // buffer already has the batch's size (nothing may allocate under capture).
+        std::vector<int64_t> key_n(N_b, N_b + B);
+        std::vector<double> key_nb(nbar_b, nbar_b + B);
+        std::vector<const void*> key_p = {d_rtp, d_w, d_win_a, d_win_b, d_out, (const void*)st};
+        const bool same = P->bg_exec && key_n == P->bg_n && key_nb == P->bg_nbar && key_p == P->bg_ptr &&
+                          win_kind == P->bg_kind && P->precision == P->bg_prec;
+        if (!(same && g_alloc_gen.load() == P->bg_gen)) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (P->bg_exec) {
+                (void)hipGraphExecDestroy(P->bg_exec);
+                P->bg_exec = nullptr;
+            }
+            const int32_t b0 = order[0];
+            if ((rc = cat2amln_enqueue(P, N_b[b0], N_b[b0] ? d_rtp + 3 * off[(size_t)b0] : nullptr,
+                                       (d_w && N_b[b0]) ? d_w + off[(size_t)b0] : nullptr, nbar_b[b0], win_kind,
+                                       d_win_a, d_win_b, d_out + (int64_t)b0 * 2 * P->nlmsize, st)))
+                return rc;
+            HIP_TRY(hipStreamSynchronize(st));
+            const bool prof = P->prof;
+            P->prof = false;  // no event records inside the graph
+            const uint64_t gen = g_alloc_gen.load();
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            rc = enqueue_all();
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            P->prof = prof;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess || !g) return fail(SFB_ERR_DEVICE, "batch graph capture failed: %s", hipGetErrorString(ec));
+            if (g_alloc_gen.load() != gen) {
+                (void)hipGraphDestroy(g);
+                return fail(SFB_ERR_DEVICE, "internal: a buffer was reallocated during the batch capture");
+            }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return fail(SFB_ERR_DEVICE, "batch graph instantiation failed: %s", hipGetErrorString(ei));
+            P->bg_exec = ex;
+            P->bg_n = key_n;
+            P->bg_nbar = key_nb;
+            P->bg_ptr = key_p;
+            P->bg_kind = win_kind;
+            P->bg_prec = P->precision;
+            P->bg_gen = gen;
+            P->bg_capture_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        HIP_TRY(hipGraphLaunch(P->bg_exec, st));
     }
     std::vector<int32_t> flags(4 * (size_t)B);
     HIP_TRY(hipMemcpyAsync(flags.data(), P->d_bflag.p, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost, st));
@@ -4447,6 +4524,14 @@ extern "C" int sfb_cat2amln_batch_device(sfb_plan* P, int32_t B, const int64_t* 
             return fail(SFB_ERR_NONFINITE, "catalogue %d: AssertionError: all(isfinite.(anlm)) failed", b);
         }
     }
+    return SFB_OK;
+}
+
+extern "C" int sfb_plan_set_batch_graph(sfb_plan* P, int32_t enable, double* capture_ms)
+{
+    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (capture_ms) *capture_ms = P->bg_capture_ms;
+    if (enable >= 0) P->batch_graph = enable != 0;
     return SFB_OK;
 }
 

@@ -46,6 +46,7 @@ SIGNATURES = {
     "sfb_cat2amln_device": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _i32, _P, _P, _P, _P]),
     "sfb_cat2amln_batch": (ctypes.c_int, [_P, _i32, _P, _P, _P, _P, _i32, _P, _P, _P]),
     "sfb_cat2amln_batch_device": (ctypes.c_int, [_P, _i32, _P, _P, _P, _P, _i32, _P, _P, _P, _P]),
+    "sfb_plan_set_batch_graph": (ctypes.c_int, [_P, _i32, _P]),
     "sfb_win_rhat_ln": (ctypes.c_int, [_P, _i64, _P, _dbl, _i32, _P, _P]),
     "sfb_win_rhat_ln_device": (ctypes.c_int, [_P, _i64, _P, _dbl, _i32, _P, _P, _P]),
     "sfb_field2anlm": (ctypes.c_int, [_P, _i64, _P, _dbl, _P, _P]),

@@ -62,6 +62,14 @@ class Plan:
         L.check(L.lib().sfb_plan_phase_times(self._h, L.ptr(ms), L.ptr(n), ctypes.byref(calls)))
         return {ph: (float(ms[i]), int(n[i])) for i, ph in enumerate(L.PHASES)}, calls.value
 
+    def set_batch_graph(self, enable=True):
+        """Batched mocks as one captured HIP graph (replayed when a batch call repeats);
+        returns the wall time (ms) of the last capture and instantiation."""
+        ms = ctypes.c_double()
+        L.check(L.lib().sfb_plan_set_batch_graph(self._h, -1 if enable is None else int(bool(enable)),
+                                                 ctypes.byref(ms)))
+        return ms.value
+
     def tiles(self):
         """The phi-transform tile shapes the plan's tile search chose, and its time (ms)."""
         t = np.zeros(6, dtype=np.int32)

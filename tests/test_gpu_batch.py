@@ -91,3 +91,22 @@ def test_batch_c2_geometry_poisson_mocks():
             n12 = int(np.sum(np.arange(1, 13)))  # maps 0..11 are n = 1, l = 0..11
             d = np.abs(batch[3][:n12] - ref[:n12])
             assert d.max() < 1e-10 * np.abs(ref[:n12]).max()
+
+
+def test_batch_graph_replays_match_eager(setup):
+    """The batch captured as one HIP graph: its first launch and two replays (with other
+    eager work on the plan in between) return the eager batch bit for bit."""
+    am, win, cats, weights, nbars = setup
+    plan = S.Plan(am)
+    ws = [w if c.shape[1] else [] for c, w in zip(cats, weights)]
+    eager = plan.cat2amln_batch(cats, nbars, win, ws)
+    plan.set_batch_graph(True)
+    small = cats[1][:, :500]
+    for rep in range(3):
+        say("graph batch, launch", rep)
+        got = plan.cat2amln_batch(cats, nbars, win, ws)
+        assert np.array_equal(got, eager), rep
+        plan.cat2amln(small, nbars[1], win)  # eager work between launches
+    assert plan.set_batch_graph(None) > 0.0
+    plan.set_batch_graph(False)
+    assert np.array_equal(plan.cat2amln_batch(cats, nbars, win, ws), eager)
