@@ -243,7 +243,7 @@ int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const doub
 
 #define SFB_PHASE_CATALOG 0    /* ang2pix, the two stable sorts, gathers, pixel segments */
 #define SFB_PHASE_SCATTER 1    /* spline evaluation + ordered per-pixel sums -> delta maps */
-#define SFB_PHASE_PHI 2        /* per-ring phi-transform */
+#define SFB_PHASE_PHI 2        /* phi-transform of the rings with 4 nside pixels (one FFT kernel) */
 #define SFB_PHASE_ANALYSIS 3   /* Legendre analysis GEMMs (1 + niter launches) */
 #define SFB_PHASE_SYNTHESIS 4  /* Legendre synthesis GEMMs (niter launches) */
 #define SFB_PHASE_RESIDUAL 5   /* ring-Fourier residual with aliasing (niter launches) */
@@ -251,7 +251,8 @@ int sfb_anlm2field_device(sfb_plan* p, int64_t nr, const double* d_r, const doub
 #define SFB_PHASE_SEGMENT 7    /* per knot group and galaxy: spline segment and t (shared by the group's maps) */
 #define SFB_PHASE_MONOPOLE 8   /* mixed precision: FP64 monopole correction of the l = 0 maps */
 #define SFB_PHASE_ADJOINT 9    /* sfb_cat2amln_vjp(_device) after its catalogue stage */
-#define SFB_NPHASE 10
+#define SFB_PHASE_PHI_POLAR 10 /* phi-transform of the polar rings (one DFT kernel) */
+#define SFB_NPHASE 11
 
 /* Turns per-phase HIP-event timing of sfb_cat2amln(_device) on or off and
  * clears the totals.  No reference counterpart (the reference prints @time). */

@@ -3771,6 +3771,7 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     // (runs of 2, 4 and 8 were timed: FP32 0.92, 0.87, 0.82 ms; the polar rings keep runs of 4)
     const int32_t nring_f = K - k_eq0, ntile_f = (int32_t)nblk(nq, tqf);
     const dim3 grid_f((unsigned)(64 * (((int64_t)nring_f * ntile_f + 63) / 64)));
+    PhaseTimer t_fft(P, SFB_PHASE_PHI, st);
     if (f32) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_phi_fft<double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));
         hipLaunchKernelGGL((k_phi_fft<double, float>), grid_f, dim3(P->fft_nt32), shf, st,
@@ -3783,7 +3784,9 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
                            P->d_tw.as<double2>(), P->d_X0.as<double>(), tqf, k_eq0, P->logM, mall, P->delta_perm, nring_f, ntile_f);
     }
     LAUNCH_CHECK();
+    t_fft.end();
     if (k_eq0 == 0) return SFB_OK;
+    PhaseTimer t_polar(P, SFB_PHASE_PHI_POLAR, st);
     int32_t gy4;
     size_t shp;
     if (!polar4_fits(P, vb, &shp, nq, &gy4))
@@ -3984,7 +3987,6 @@ static int sht_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false, 
     const int32_t L = (int32_t)P->lmax;
     const double domega = 4.0 * kPi / (double)P->npix;
     {
-        PhaseTimer timer(P, SFB_PHASE_PHI, st);
         int rc = phi_transform(P, nq, st, f32, true, src);
         if (rc) return rc;
     }

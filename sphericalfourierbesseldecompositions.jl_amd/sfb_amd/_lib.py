@@ -72,7 +72,7 @@ SIGNATURES = {
     "sfb_plan_tiles": (ctypes.c_int, [_P, _P, _P]),
 }
 
-PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment", "monopole", "adjoint"]
+PHASES = ["catalog", "scatter", "phi", "analysis", "synthesis", "residual", "pack", "segment", "monopole", "adjoint", "phi_polar"]
 
 _lib = None
 

@@ -115,7 +115,7 @@ class Plan:
         out = np.empty(2 * self.nlmsize)
         L.check(L.lib().sfb_cat2amln(self._h, N, L.ptr(host), L.ptr(w), float(nbar), kind, L.ptr(a), L.ptr(b),
                                      L.ptr(out)))
-        return out[0::2] + 1j * out[1::2]
+        return out.view(np.complex128)  # (re, im) pairs are numpy's complex layout
 
     def cat2amln_device(self, N, d_rtp, d_w, nbar, win_kind, d_win_a, d_win_b, d_out, stream=None):
         """All arrays are device pointers (ints) or torch CUDA tensors."""
