@@ -265,7 +265,8 @@ int sfb_plan_phase_times(const sfb_plan* p, double* ms, int64_t* launches, int64
 /* The phi-transform tile shapes the plan's tile search chose at creation (no reference
  * counterpart; SFB_TUNE=0 in the environment skips the search and keeps the defaults):
  * tiles[0..5] = maps per FFT block FP64, FFT threads per block FP64, the same two for FP32,
- * at most maps per polar block FP64, FP32; *tune_ms = the search's wall time. */
+ * maps per polar block at the largest polar ring FP64, FP32 (fixed by the ring sizes, not
+ * searched); *tune_ms = the search's wall time. */
 int sfb_plan_tiles(const sfb_plan* p, int32_t* tiles, double* tune_ms);
 
 /* ---- pieces of the path, exposed for parity checks ------------------------ */

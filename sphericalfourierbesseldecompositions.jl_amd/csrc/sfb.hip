@@ -827,12 +827,16 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
                                              int npr, int shift, const TW* __restrict__ twr, int L, int kf,
                                              int q, R xnr, R xni, R xsr, R xsi)
 {
-    const int n2 = 2 * npr;
+    // the phase index (m shift) mod 2 npr, advanced by npr shift with each fold of m (32-bit:
+    // m shift < 2^31 for every ring the plans accept)
+    const unsigned un2 = 2u * (unsigned)npr, dph = ((unsigned)npr * (unsigned)shift) % un2;
     for (int fam = 0; fam < 2; ++fam) {
         if (fam == 1 && (kf == 0 || 2 * kf == npr)) break;
         const double cj = fam ? -1.0 : 1.0;
-        for (int m = fam ? npr - kf : kf; m <= L; m += npr) {
-            const TW ph64 = twr[(int)(((int64_t)m * shift) % n2)];  // e^{-i m phi0} = (c, -s)
+        const int m0 = fam ? npr - kf : kf;
+        unsigned ph = ((unsigned)m0 * (unsigned)shift) % un2;
+        for (int m = m0; m <= L; m += npr, ph = min(ph + dph, ph + dph - un2)) {
+            const TW ph64 = twr[ph];  // e^{-i m phi0} = (c, -s)
             const R phx = (R)ph64.x, phy = (R)ph64.y;
             const R anr = xnr, ani = (R)cj * xni, asr = xsr, asi = (R)cj * xsi;
             const R fnr = anr * phx + ani * phy, fni = ani * phx - anr * phy;
@@ -1098,7 +1102,7 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const R* __restrict__ X, int64
             }
         }
         const R c = m ? (R)2 : (R)1;
-        const double2 ph64 = twg[(int)(((int64_t)m * Rg.shift) % (2 * N))];  // e^{i m phi_0}
+        const double2 ph64 = twg[((unsigned)m * (unsigned)Rg.shift) % (2u * (unsigned)N)];  // e^{i m phi_0}
         const R phx = (R)ph64.x, phy = (R)ph64.y;
         const R nr = c * (er + orr), ni = c * (ei + oi), sr = c * (er - orr), si = c * (ei - oi);
         R2 hn, hsv;
@@ -1217,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const R* __restrict__ X, in
                     od.y = 0;
                     if (m + 1 <= L) od = *(const R2*)(X + ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q);
                     const R c = m ? (R)2 : (R)1;
-                    const double2 ph64 = twl[(int)(((int64_t)m * Rg.shift) % n2)];  // e^{i m phi_0}
+                    const double2 ph64 = twl[((unsigned)m * (unsigned)Rg.shift) % (unsigned)n2];  // e^{i m phi_0}
                     const R phx = (R)ph64.x, phy = (R)ph64.y;
                     const R gnr = c * (ev.x + od.x), gni = c * (ev.y + od.y);
                     const R gsr = c * (ev.x - od.x), gsi = c * (ev.y - od.y);
@@ -1255,7 +1259,7 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const R* __restrict__ X, in
         for (int t = 0; t < 4; ++t) {
             const int bb = u + t * ni;
             const R2 x = xf[bb];
-            const double2 w64 = twl[(int)(((int64_t)2 * bb * ss) % n2)];  // e^{2 pi i bb s/npr}
+            const double2 w64 = twl[((unsigned)(2 * bb) * (unsigned)ss) % (unsigned)n2];  // e^{2 pi i bb s/npr}
             const R wx = (R)w64.x, wy = (R)w64.y;
             vr += x.x * wx - x.y * wy;
             vi += x.x * wy + x.y * wx;
@@ -1303,25 +1307,143 @@ __global__ __launch_bounds__(256) void k_phi_ipolar4(const R* __restrict__ X, in
         for (int ss = 0; ss < 4; ++ss) o[4 * j1 + ss] = sc * acc[1][ss];
 }
 
+// Result-row map: v_mfma_f64_16x16x4f64 leaves row (l/16) + 4v in register v of
+// lane l, v_mfma_f32_16x16x4f32 row 4(l/16) + v (both checked by sfb_gemm_f64/f32).
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <bool F32> struct MfmaT;
+template <> struct MfmaT<false> {
+    typedef double T;
+    typedef double4_t V;
+    static __device__ __forceinline__ V zero() { return (V){0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct MfmaT<true> {
+    typedef float T;
+    typedef float4_t V;
+    static __device__ __forceinline__ V zero() { return (V){0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 // Polar ring pairs (npr = 4i, i < nside): one radix-4 decimation-in-time step,
 // X_kf = sum_{s<4} W^{kf s} Y_s[kf mod i], W = e^{-2 pi i/npr}, with Y_s the length-i
 // DFT of the decimated samples delta[4j + s], computed directly for u <= i/2
-// (real input: Y_s[i - u] = conj Y_s[u]).  A block takes ring pair k and TQ maps
-// (2 TQ sequences, north and south); each Y work item accumulates one sequence x
-// 2 frequencies x 4 phases (16 fma per 64 bytes of LDS).  Y overwrites the staged
-// samples (row stride npr + 8 doubles holds 4 (i/2 + 1) complex values).  The
-// LDS footprint stays ~35 KB so that four blocks share a CU.
-template <typename R>
+// (real input: Y_s[i - u] = conj Y_s[u]).
+//
+// The direct DFTs of a block's 8 TQ columns c = (sequence f = 2 ql + side, phase s) are one
+// product on MFMA, Yt[r][c] = sum_j T[r][j] x_f[4j + s], with T[u][j] = cos(2 pi u j/i) and
+// T[nu + u][j] = -sin(2 pi u j/i) for u < nu = i/2 + 1 (Y_s[u] = Yt[u] + i Yt[nu + u]):
+// 16-row tiles r (MT of them) x 16-column tiles x 4-sample k-steps.  A wave owns one
+// column tile and every msplit-th row tile (at most MTWMAX: 4, or 8 at nside 512, whose
+// largest rings have 32 row tiles; the register count and so the occupancy follow it);
+// the T fragment of a lane is read
+// from the ring's (cos, -sin) table in LDS at (r' j) mod i, an index that each k-step
+// advances by 4 r' mod i.  The FP32 path (mixed precision) flushes its FP32 accumulators
+// into FP64 sums every four k-steps (16 samples): a length-i FP32 sum errs by ~sqrt(i/2)
+// ulp, 1e-6 of the coefficients at nside 256, the mixed path's whole error budget.  Y
+// overwrites the staged samples after the product.
+__host__ __device__ inline int p4_mt(int ni) { return (2 * (ni / 2 + 1) + 15) / 16; }
+// row tiles per wave at most, for the rings of an nside (i < nside)
+__host__ __device__ inline int p4_mtwmax(int64_t nside) { return p4_mt((int)(nside > 1 ? nside - 1 : 1)) <= 16 ? 4 : 8; }
+// maps per block: 8 while a wave holds every row tile, then 4 and 2
+__host__ __device__ inline int p4_tq(int ni, int mtwmax)
+{
+    const int mt = p4_mt(ni);
+    return mt <= mtwmax ? 8 : (mt <= 2 * mtwmax ? 4 : 2);
+}
+// LDS row of one sequence (elements of R): the 4 i samples, then the 4 (i/2 + 1) complex Y
+// values (<= 4 i + 8 elements), padded to a stride of 4 elements mod 128 bytes
+__host__ __device__ inline int p4_ld(int ni, int vb)
+{
+    const int need = 4 * ni + 8, m = 128 / vb;
+    return need + ((4 - need % m) % m + m) % m;
+}
+// LDS bytes of a block: 2 TQ sample rows and the ring's (cos, -sin) table
+__host__ __device__ inline int64_t p4_lds(int ni, int vb, int mtwmax)
+{
+    return (int64_t)vb * (2 * (int64_t)p4_tq(ni, mtwmax) * p4_ld(ni, vb) + 2 * (int64_t)ni);
+}
+
+// One wave's share of the product (MTW row tiles, columns 16 nt ..), then Y into LDS.
+template <typename R, int MTW>
+__device__ __forceinline__ void p4_product(R* __restrict__ D, const typename Vec2T<R>::T* __restrict__ twi, int ni,
+                                           int ld, int nt, int mpart, int msplit)
+{
+    typedef MfmaT<sizeof(R) == 4> M;
+    typedef typename M::V V;
+    const int lane = threadIdx.x & 63, nu = ni / 2 + 1;
+    // B fragment: lane holds column c at sample j = 4 kq + (lane >> 4)
+    const int c = 16 * nt + (lane & 15);
+    const R* bcol = D + (c >> 2) * ld + (c & 3);
+    // T fragment: lane holds row r = 16 mt + (lane & 15) of each of its row tiles
+    int idx[MTW], stp[MTW], comp[MTW];
+#pragma unroll
+    for (int mi = 0; mi < MTW; ++mi) {
+        const int r = 16 * (mpart + mi * msplit) + (lane & 15);
+        const int rr = r < nu ? r : r - nu;  // rows past 2 nu are computed and dropped
+        comp[mi] = r < nu ? 0 : 1;
+        stp[mi] = (int)((4LL * rr) % ni);
+        idx[mi] = (int)(((int64_t)rr * (lane >> 4)) % ni);
+    }
+    const R* tw1 = (const R*)twi;  // (cos, -sin) pairs
+    V acc[MTW];
+    double accd[MTW][4];
+#pragma unroll
+    for (int mi = 0; mi < MTW; ++mi) {
+        acc[mi] = M::zero();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) accd[mi][v] = 0.0;
+    }
+    // chunks of four k-steps (16 samples; past the last sample B is zero), so that the
+    // FP32 flush follows every chunk with no branch around the accumulators
+    const int KC = (ni + 15) / 16;
+    for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+            const int j = 16 * kc + 4 * q4 + (lane >> 4);
+            const R bv = j < ni ? bcol[4 * j] : (R)0;
+#pragma unroll
+            for (int mi = 0; mi < MTW; ++mi) {
+                const R av = tw1[2 * idx[mi] + comp[mi]];
+                acc[mi] = M::mma(av, bv, acc[mi]);
+                const unsigned nx = (unsigned)(idx[mi] + stp[mi]);  // (idx + step) mod i
+                idx[mi] = (int)min(nx, nx - (unsigned)ni);
+            }
+        }
+        if (sizeof(R) == 4) {
+#pragma unroll
+            for (int mi = 0; mi < MTW; ++mi) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) accd[mi][v] += (double)acc[mi][v];
+                acc[mi] = M::zero();
+            }
+        }
+    }
+    __syncthreads();  // every sample read; Y overwrites them
+#pragma unroll
+    for (int mi = 0; mi < MTW; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * (mpart + mi * msplit) + M::row(lane, v);
+            if (r >= 2 * nu) continue;
+            const R val = sizeof(R) == 4 ? (R)accd[mi][v] : (R)acc[mi][v];
+            const int u = r < nu ? r : r - nu;
+            D[(c >> 2) * ld + 2 * ((c & 3) * nu + u) + (r < nu ? 0 : 1)] = val;
+        }
+}
+
+template <typename R, int MTWMAX>
 __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta, int64_t npix, int32_t nq,
                                                     int64_t M2, int32_t K, int32_t L,
                                                     const RingDev* __restrict__ rings, const double2* __restrict__ tw,
-                                                    R* __restrict__ X, int32_t TQMAX, const int32_t* __restrict__ mall,
+                                                    R* __restrict__ X, const int32_t* __restrict__ mall,
                                                     const uint32_t* __restrict__ perm, int32_t nring, int32_t ntile)
 {
     typedef typename Vec2T<R>::T R2;
     typedef typename Vec4T<R>::T R4;
     extern __shared__ double2 ps_raw[];
-    R2* ps = (R2*)ps_raw;
     // 1-D grid over (map tile, ring pair) units as k_phi_fft, in runs of four per XCD
     const int b = blockIdx.x, loc = b >> 3;
     const int unit = (((loc >> 2) << 3) + (b & 7)) * 4 + (loc & 3);
@@ -1329,19 +1451,18 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
     const int k = unit % nring;
     const RingDev Rg = rings[k];
     const int npr = Rg.npr, n2 = 2 * npr, ni = npr >> 2, nu = ni / 2 + 1;
-    // maps per block: as many as fill the work items (2 TQ ceil(nu / 2) <= blockDim.x),
-    // so small rings take many maps per block; blocks past the last map leave at once
-    const int nup = (nu + 1) / 2;
-    const int TQ = min(TQMAX, (int)blockDim.x / (2 * nup));
+    // maps per block (the grid is sized for the largest ring's tiles; blocks past the last
+    // map leave at once)
+    const int TQ = p4_tq(ni, MTWMAX);
     const int q0 = (unit / nring) * TQ;
     if (q0 >= nq) return;
-    const int ld = npr + 8;                       // doubles per sequence row
+    const int ld = p4_ld(ni, (int)sizeof(R));     // elements per sequence row
     const double2* twl = tw + Rg.tw_off;          // (cos, sin)(pi kk/npr), kk < 2 npr (global, cached)
-    R2* twi = ps;                                 // (cos, sin)(2 pi v/i), v < i
-    R* D = (R*)(ps + ((ni + 1) & ~1));            // [2 TQ][ld]: samples, then Y (aligned rows)
+    R* D = (R*)ps_raw;                            // [2 TQ][ld]: samples, then Y
+    R2* twi = (R2*)(D + 2 * TQ * ld);             // (cos, -sin)(2 pi v/i), v < i
     for (int e = threadIdx.x; e < ni; e += blockDim.x) {  // 2 pi v/i = pi (8v)/npr
         twi[e].x = (R)twl[8 * e].x;
-        twi[e].y = (R)twl[8 * e].y;
+        twi[e].y = (R)-twl[8 * e].y;
     }
     // samples in fours (npr = 4i and every polar ring starts at a multiple of 4), PH_UL
     // loads in flight per thread before their LDS stores
@@ -1389,80 +1510,20 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         }
     }
     __syncthreads();
-    // Y_s[u] of sequence f and frequencies (2 up, 2 up + 1) for work item e = threadIdx.x
-    const int e = threadIdx.x;
-    const bool active = e < 2 * TQ * nup;
-    const int up = active ? e % nup : 0, f = active ? e / nup : 0;
-    R a[2][4][2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int ss = 0; ss < 4; ++ss) a[u][ss][0] = a[u][ss][1] = 0.0;
-    if (active) {
-        const int u0 = 2 * up, u1 = u0 + 1;
-        const R* d0 = D + f * ld;
-        int i0 = 0, i1 = 0;
-        // FP32 (mixed path): the running sums restart every PH_CHUNK samples and are flushed
-        // into FP64 sums, so a rounding is relative to at most PH_CHUNK terms rather than to
-        // up to 255 (a length-i sequential FP32 sum errs by ~sqrt(i/2) ulp: 1e-6 of the
-        // coefficients at nside 256, the mixed path's whole error budget)
-        constexpr int PH_CHUNK = sizeof(R) == 4 ? 16 : (1 << 30);
-        double A[2][4][2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int ss = 0; ss < 4; ++ss) A[u][ss][0] = A[u][ss][1] = 0.0;
-        for (int jc = 0; jc < ni; jc += PH_CHUNK) {
-            const int je = min(ni, jc + PH_CHUNK);
-            for (int j = jc; j < je; ++j) {
-                const R2 w0 = twi[i0], w1 = twi[i1];
-                const R4 x = *(const R4*)(d0 + 4 * j);
-                const R xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                for (int ss = 0; ss < 4; ++ss) {
-                    a[0][ss][0] = fma(xs[ss], w0.x, a[0][ss][0]);
-                    a[0][ss][1] = fma(-xs[ss], w0.y, a[0][ss][1]);
-                    a[1][ss][0] = fma(xs[ss], w1.x, a[1][ss][0]);
-                    a[1][ss][1] = fma(-xs[ss], w1.y, a[1][ss][1]);
-                }
-                i0 += u0;
-                if (i0 >= ni) i0 -= ni;
-                i1 += u1;
-                if (i1 >= ni) i1 -= ni;
-            }
-            if (sizeof(R) == 4) {
-#pragma unroll
-                for (int u = 0; u < 2; ++u)
-#pragma unroll
-                    for (int ss = 0; ss < 4; ++ss)
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            A[u][ss][c] += (double)a[u][ss][c];
-                            a[u][ss][c] = 0;
-                        }
-            }
-        }
-        if (sizeof(R) == 4) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int ss = 0; ss < 4; ++ss)
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) a[u][ss][c] = (R)A[u][ss][c];
-        }
-    }
-    __syncthreads();  // every sample read; Y overwrites them
-    if (active) {
-        R2* Y = (R2*)(D + f * ld);  // Y[s * nu + u]
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int uu = 2 * up + u;
-            if (uu < nu)
-#pragma unroll
-                for (int ss = 0; ss < 4; ++ss) {
-                    Y[ss * nu + uu].x = a[u][ss][0];
-                    Y[ss * nu + uu].y = a[u][ss][1];
-                }
+    // the product: waves split the column tiles (TQ / 2 of 16 columns) and the row tiles
+    {
+        const int MT = p4_mt(ni), ntb = TQ / 2, msplit = 4 / ntb;
+        const int wv = threadIdx.x >> 6, nt = wv % ntb, mpart = wv / ntb;
+        const int mtw = (MT + msplit - 1) / msplit;  // row tiles per wave (<= MTWMAX)
+        switch (mtw) {
+            case 1: p4_product<R, 1>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 2: p4_product<R, 2>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 3: p4_product<R, 3>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 4: p4_product<R, 4>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 5: if (MTWMAX > 4) p4_product<R, MTWMAX < 5 ? 4 : 5>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 6: if (MTWMAX > 4) p4_product<R, MTWMAX < 6 ? 4 : 6>(D, twi, ni, ld, nt, mpart, msplit); break;
+            case 7: if (MTWMAX > 4) p4_product<R, MTWMAX < 7 ? 4 : 7>(D, twi, ni, ld, nt, mpart, msplit); break;
+            default: if (MTWMAX > 4) p4_product<R, MTWMAX>(D, twi, ni, ld, nt, mpart, msplit); break;
         }
     }
     __syncthreads();
@@ -1477,6 +1538,18 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
         int u = kf % ni;
         const bool cj = u > ni / 2;
         if (cj) u = ni - u;
+        // W^{kf s} = (c, -s) at index 2 kf s mod 2 npr (2 kf <= npr + 2 < 2 npr)
+        R wx[4], wy[4];
+        {
+            unsigned iw = 0;
+            const unsigned st2 = 2u * (unsigned)kf, un2 = (unsigned)n2;
+#pragma unroll
+            for (int ss = 0; ss < 4; ++ss, iw = min(iw + st2, iw + st2 - un2)) {
+                const double2 w64 = twl[iw];
+                wx[ss] = (R)w64.x;
+                wy[ss] = (R)w64.y;
+            }
+        }
         R xr[2], xi[2];
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
@@ -1486,10 +1559,8 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
             for (int ss = 0; ss < 4; ++ss) {
                 R2 y = Y[ss * nu + u];
                 if (cj) y.y = -y.y;
-                const double2 w64 = twl[(int)(((int64_t)2 * kf * ss) % n2)];  // W^{kf s} = (c, -s)
-                const R wx = (R)w64.x, wy = (R)w64.y;
-                sr += y.x * wx + y.y * wy;
-                si += y.y * wx - y.x * wy;
+                sr += y.x * wx[ss] + y.y * wy[ss];
+                si += y.y * wx[ss] - y.x * wy[ss];
             }
             xr[side] = sr;
             xi[side] = si;
@@ -1507,26 +1578,6 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
 // F32: operands rounded to FP32 on the way into LDS and accumulated in FP32 on
 // v_mfma_f32_16x16x4f32 (used for the Jacobi corrections, K <= ~2 lmax); the
 // epilogue alpha*acc + beta*Cin is FP64 either way.
-// Result-row map: v_mfma_f64_16x16x4f64 leaves row (l/16) + 4v in register v of
-// lane l, v_mfma_f32_16x16x4f32 row 4(l/16) + v (both checked by sfb_gemm_f64/f32).
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
-template <bool F32> struct MfmaT;
-template <> struct MfmaT<false> {
-    typedef double T;
-    typedef double4_t V;
-    static __device__ __forceinline__ V zero() { return (V){0.0, 0.0, 0.0, 0.0}; }
-    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-template <> struct MfmaT<true> {
-    typedef float T;
-    typedef float4_t V;
-    static __device__ __forceinline__ V zero() { return (V){0.f, 0.f, 0.f, 0.f}; }
-    static __device__ __forceinline__ V mma(T a, T b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
 constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
@@ -2563,7 +2614,6 @@ struct sfb_plan {
     int32_t niter = 3, precision = 0, TQ = 8;
     int32_t TQF = 1, TQF32 = 2, logM = 1;  // phi-transform FFT tiles (FP64 / FP32)
     int32_t fft_nt = 512, fft_nt32 = 256;           // FFT threads per block (FP64 / FP32)
-    int32_t p4_tq = 64, p4_tq32 = 64;               // polar maps per block, at most (FP64 / FP32)
     double tune_ms = 0.0;                           // time the plan-time tile search took
     std::vector<MapInfo> maps;
     std::vector<int64_t> out_base, aoff, lam_off;
@@ -3675,38 +3725,32 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
 }
 
 // k_phi_polar4 launch shape for the polar ring pairs of an nside (i = 1 .. nside - 1):
-// each ring takes min(tqmax, 256 / (2 nup)) maps per block (nup = ceil((i/2 + 1)/2) work
-// items per map and ring), so its largest ring (nup <= 128) holds one map; the LDS is that
-// of the largest block over the rings, tqmin the fewest maps per block.  Host arithmetic
+// ring i takes p4_tq(i) maps per block; the LDS is that of the largest block over the
+// rings, tqmin the fewest maps per block (the grid is sized by it).  Host arithmetic
 // only; false when a block does not fit.
-static bool polar4_shape(int64_t nside, int32_t tqmax, int64_t lds_max, size_t vbytes, size_t* shp, int64_t* tqmin_out)
+static bool polar4_shape(int64_t nside, int64_t lds_max, size_t vbytes, size_t* shp, int64_t* tqmin_out)
 {
-    const int64_t npr_max = 4 * (nside - 1);
-    const int64_t nup_max = ((npr_max / 4) / 2 + 2) / 2;  // ceil((i/2 + 1) / 2) at the largest ring
-    *shp = 0;
-    *tqmin_out = tqmax;
-    if (2 * nup_max > 256) return false;
     size_t sh = 0;
-    int64_t tqmin = tqmax;
+    int64_t tqmin = 8;
+    *shp = 0;
+    *tqmin_out = 1;
+    const int mtwmax = p4_mtwmax(nside);
+    if (p4_mt((int)std::max<int64_t>(nside - 1, 1)) > 4 * mtwmax) return false;  // 2 maps, 4 waves of row tiles
     for (int64_t i = 1; i < nside; ++i) {
-        const int64_t nup = ((i / 2 + 1) + 1) / 2;
-        const int64_t tq = std::min<int64_t>(tqmax, 256 / (2 * nup));
-        sh = std::max(sh, 2 * vbytes * (size_t)(i + 2) + vbytes * 2 * (size_t)tq * (size_t)(4 * i + 8));
-        tqmin = std::min(tqmin, tq);
+        sh = std::max(sh, (size_t)p4_lds((int)i, (int)vbytes, mtwmax));
+        tqmin = std::min<int64_t>(tqmin, p4_tq((int)i, mtwmax));
     }
     *shp = sh;
     *tqmin_out = tqmin;
-    return tqmin >= 1 && (int64_t)sh <= lds_max;
+    return (int64_t)sh <= lds_max;
 }
 
-// The plan's k_phi_polar4 shape: maps per block up to its tile (FP64 / FP32) and the grid
-// height for nq maps (set by the ring with the fewest maps per block).  Every nside the
-// plan accepts (<= 512) fits.
+// The plan's k_phi_polar4 shape and the grid height for nq maps (set by the ring with the
+// fewest maps per block).  Every nside the plan accepts (<= 512) fits.
 static bool polar4_fits(const sfb_plan* P, size_t vbytes, size_t* shp, int32_t nq = 1, int32_t* gy = nullptr)
 {
-    const int32_t tqmax = vbytes == 8 ? P->p4_tq : P->p4_tq32;
     int64_t tqmin = 1;
-    const bool ok = polar4_shape(P->nside, tqmax, P->lds_max, vbytes, shp, &tqmin);
+    const bool ok = polar4_shape(P->nside, P->lds_max, vbytes, shp, &tqmin);
     if (gy) *gy = (int32_t)((nq + tqmin - 1) / std::max<int64_t>(tqmin, 1));
     return ok;
 }
@@ -3792,17 +3836,21 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     if (!polar4_fits(P, vb, &shp, nq, &gy4))
         return fail(SFB_ERR_ARG, "internal: the polar phi-transform blocks do not fit in LDS");
     const dim3 grid_p((unsigned)(32 * (((int64_t)k_eq0 * gy4 + 31) / 32)));
-    if (f32) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<float>, grid_p, dim3(256), shp, st,
-                           (const float*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<float>(), P->p4_tq32, mall, P->delta_perm, k_eq0, gy4);
-    } else {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_phi_polar4<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
-        hipLaunchKernelGGL(k_phi_polar4<double>, grid_p, dim3(256), shp, st,
-                           (const double*)dsrc, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
-                           P->d_tw.as<double2>(), P->d_X0.as<double>(), P->p4_tq, mall, P->delta_perm, k_eq0, gy4);
-    }
+    auto go = [&](auto kern, const auto* src, auto* x) -> int {
+        HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shp));
+        hipLaunchKernelGGL(kern, grid_p, dim3(256), shp, st, src, P->npix, nq, P->M2, K, L, P->d_rings.as<RingDev>(),
+                           P->d_tw.as<double2>(), x, mall, P->delta_perm, k_eq0, gy4);
+        return SFB_OK;
+    };
+    const bool wide = p4_mtwmax(nside) > 4;
+    int rc;
+    if (f32)
+        rc = wide ? go(k_phi_polar4<float, 8>, (const float*)dsrc, P->d_X0.as<float>())
+                  : go(k_phi_polar4<float, 4>, (const float*)dsrc, P->d_X0.as<float>());
+    else
+        rc = wide ? go(k_phi_polar4<double, 8>, (const double*)dsrc, P->d_X0.as<double>())
+                  : go(k_phi_polar4<double, 4>, (const double*)dsrc, P->d_X0.as<double>());
+    if (rc) return rc;
     LAUNCH_CHECK();
     return SFB_OK;
 }
@@ -3879,11 +3927,9 @@ static int autotune_phi(sfb_plan* P)
     };
     const int64_t M = 2 * P->nside;
     for (int f = 0; f < 2 && !rc; ++f) {
-        const size_t vb = f ? 4 : 8;
         int32_t& tq = f ? P->TQF32 : P->TQF;
         int32_t& nt = f ? P->fft_nt32 : P->fft_nt;
-        int32_t& p4 = f ? P->p4_tq32 : P->p4_tq;
-        // FFT tiles (the polar kernel at its default meanwhile)
+        // FFT tiles (the polar kernel's shape is fixed by the ring sizes)
         int32_t btq = tq, bnt = nt;
         float bms = 1e30f;
         for (int32_t cq = 1; cq <= 8 && !rc; cq *= 2) {
@@ -3903,22 +3949,6 @@ static int autotune_phi(sfb_plan* P)
         }
         tq = btq;
         nt = bnt;
-        // polar tiles
-        int32_t bp = p4;
-        bms = 1e30f;
-        for (int32_t cp : {32, 64}) {
-            if (rc) break;
-            p4 = cp;
-            size_t shp;
-            if (!polar4_fits(P, vb, &shp)) continue;
-            float ms = 0.0f;
-            if ((rc = timed(f == 1, &ms))) break;
-            if (ms < bms) {
-                bms = ms;
-                bp = cp;
-            }
-        }
-        p4 = bp;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -4053,7 +4083,7 @@ extern "C" int sfb_ring_kernel_shapes(int64_t nside, int64_t lmax, int64_t lds_b
     if (!out || nside < 1 || lmax < 0 || lds_bytes < 1) return fail(SFB_ERR_ARG, "bad ring-shape arguments");
     size_t sh4 = 0;
     int64_t tq4 = 0;
-    const bool fwd = polar4_shape(nside, 64, lds_bytes, sizeof(double), &sh4, &tq4);
+    const bool fwd = polar4_shape(nside, lds_bytes, sizeof(double), &sh4, &tq4);
     const InvShapes S = inv_shapes(nside, lmax, lds_bytes);
     out[0] = fwd ? nside - 1 : 0;
     out[1] = (int64_t)sh4;
@@ -5133,8 +5163,12 @@ extern "C" int sfb_plan_tiles(const sfb_plan* P, int32_t* tiles, double* tune_ms
     tiles[1] = P->fft_nt;
     tiles[2] = P->TQF32;
     tiles[3] = P->fft_nt32;
-    tiles[4] = P->p4_tq;
-    tiles[5] = P->p4_tq32;
+    for (int f = 0; f < 2; ++f) {  // the polar kernel's maps per block at the largest ring
+        size_t shp = 0;
+        int64_t tqmin = 1;
+        (void)polar4_shape(P->nside, P->lds_max, f ? 4 : 8, &shp, &tqmin);
+        tiles[4 + f] = (int32_t)tqmin;
+    }
     if (tune_ms) *tune_ms = P->tune_ms;
     return SFB_OK;
 }

@@ -58,3 +58,35 @@ def test_phi_transform_matches_direct_dft():
                 got_o = out[2 * m + 1, k, 2 * q] + 1j * out[2 * m + 1, k, 2 * q + 1]
                 assert abs(got_e - E) < 1e-12 * (1 + abs(E)), (k, m, q, got_e, E)
                 assert abs(got_o - Od) < 1e-12 * (1 + abs(Od)), (k, m, q, got_o, Od)
+
+
+@pytest.mark.parametrize("nside,lmax,nmap", [(64, 200, 37), (256, 400, 19)])
+def test_phi_transform_large_rings(nside, lmax, nmap):
+    """Every ring pair against numpy's FFT (F_m = e^{-i m phi_0} FFT(ring)[m mod npr]) at
+    ring sizes where a polar block holds one to sixty-four sequence batches (frequency
+    counts from 1 to 128) and a map count that leaves the last tile of each ring part-full."""
+    am = S.AnlmModes.from_nl(2, lmax, 500.0, 1000.0, nside=nside)
+    plan = S.Plan(am)
+    npix = 12 * nside * nside
+    maps = np.random.default_rng(nside).normal(size=(nmap, npix))
+    K = 2 * nside
+    out = np.empty((2 * (lmax + 1), K, 2 * nmap))
+    L.check(L.lib().sfb_phi_transform(plan.handle, nmap, L.ptr(maps), L.ptr(out)))
+    got = out[:, :, 0::2] + 1j * out[:, :, 1::2]  # [(m, parity), k, q]
+    m = np.arange(lmax + 1)
+    worst = 0.0
+    for k in range(K):
+        st, npr, z, phi0 = O.ring_info(nside, k + 1)
+        eq = k == K - 1
+        sts = st if eq else npix - npr - st
+        # e^{-i m phi_0} with m phi_0 reduced mod 2 pi in integers (phi_0 = shift pi / npr;
+        # m phi_0 in floating point errs by up to 1e-13 at m = 400 on small rings)
+        shift = int(round(phi0 * npr / np.pi))
+        ph = np.exp(-1j * np.pi * ((m * shift) % (2 * npr)) / npr)[:, None]
+        FN = ph * np.fft.fft(maps[:, st:st + npr], axis=1)[:, m % npr].T
+        FS = np.zeros_like(FN) if eq else ph * np.fft.fft(maps[:, sts:sts + npr], axis=1)[:, m % npr].T
+        E, Od = (FN, FN) if eq else (FN + FS, FN - FS)
+        scale = np.sqrt(npr)
+        worst = max(worst, np.abs(got[0::2, k] - E).max() / scale, np.abs(got[1::2, k] - Od).max() / scale)
+    print(f"nside {nside}: max |F - FFT| / sqrt(npr) = {worst:.2e}", flush=True)
+    assert worst < 1e-13
