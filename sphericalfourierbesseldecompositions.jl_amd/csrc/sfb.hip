@@ -818,11 +818,16 @@ template <typename R> struct Vec2T;
 template <> struct Vec2T<double> { typedef double2 T; };
 template <> struct Vec2T<float> { typedef float2 T; };
 
+// Twiddle tables in LDS hold kk at twpad(kk): one pad entry per 16 (tw_len entries for kk <= N).
+__host__ __device__ inline int twpad(int kk) { return kk + (kk >> 4); }
+__host__ __device__ inline int64_t tw_len(int64_t N) { return N + (N >> 4) + 1; }
+
 // Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
 // every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
 // each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
-// twr[kk] = (cos, sin)(pi kk/npr) for every index (m shift) mod 2 npr that occurs.
-template <typename R, typename TW>
+// twr[kk] = (cos, sin)(pi kk/npr) for every index (m shift) mod 2 npr that occurs (at
+// twpad(kk) when TWPAD: the padded table in LDS).
+template <typename R, bool TWPAD, typename TW>
 __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int32_t K, int32_t k, bool eq,
                                              int npr, int shift, const TW* __restrict__ twr, int L, int kf,
                                              int q, R xnr, R xni, R xsr, R xsi)
@@ -836,7 +841,7 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
         const int m0 = fam ? npr - kf : kf;
         unsigned ph = ((unsigned)m0 * (unsigned)shift) % un2;
         for (int m = m0; m <= L; m += npr, ph = min(ph + dph, ph + dph - un2)) {
-            const TW ph64 = twr[ph];  // e^{-i m phi0} = (c, -s)
+            const TW ph64 = twr[TWPAD ? twpad((int)ph) : (int)ph];  // e^{-i m phi0} = (c, -s)
             const R phx = (R)ph64.x, phy = (R)ph64.y;
             const R anr = xnr, ani = (R)cj * xni, asr = xsr, asi = (R)cj * xsi;
             const R fnr = anr * phx + ani * phy, fni = ani * phx - anr * phy;
@@ -863,15 +868,24 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
 // 32 complex FP32): element x of a sequence sits at fpad(x) = x + (x >> FFT_PAD_SH),
 // and a sequence takes fft_plen(M) slots.  Unpadded, the bit-reversed stores put
 // the 16 lanes of an LDS cycle on one bank group, as do the first butterfly stages.
+// Consecutive sequences start 64 bytes apart modulo 256 (fft_plen pads the stride), so the
+// output phase, whose lanes read one element of several maps' sequences, spreads them too.
+// The twiddle table in LDS is padded alike (kk at twpad(kk)): the radix passes read it at
+// strides of 4 to 32 entries, which unpadded fall on one bank group.
 constexpr bool FFT_PAD = true;
 template <typename R> struct FftPad { static constexpr int SH = sizeof(R) == 8 ? 4 : 5; };
 template <typename R> __device__ __forceinline__ int fpad(int x) { return FFT_PAD ? x + (x >> FftPad<R>::SH) : x; }
-static inline int64_t fft_plen(int64_t M, size_t vbytes) { return FFT_PAD ? M + (M >> (vbytes == 8 ? 4 : 5)) : M; }
+__host__ __device__ inline int64_t fft_plen(int64_t M, size_t vbytes)
+{
+    const int64_t e = 128 / (int64_t)vbytes, t = 32 / (int64_t)vbytes;  // per 256 B, per 64 B
+    const int64_t p = FFT_PAD ? M + (M >> (vbytes == 8 ? 4 : 5)) : M;
+    return p + ((t - p % e) % e + e) % e;
+}
 
 // In-place forward DFT (e^{-i}) of nf sequences of length M = N/2 held bit-reversed
 // (and padded, stride Mp) in LDS: the radix-2 stages fused in pairs (one radix-4 pass
 // of 4 loads / 3 twiddles / 4 butterflies per thread item between barriers), then a
-// last radix-2 stage when log2 M is odd.  twr[kk] = (cos, sin)(pi kk / N), kk <= N.
+// last radix-2 stage when log2 M is odd.  twr[twpad(kk)] = (cos, sin)(pi kk / N), kk <= N.
 template <typename R>
 __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int M, int Mp, int N, int logM,
                                            const typename Vec2T<R>::T* twr)
@@ -882,15 +896,21 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
         const int half = 1 << (st - 1);
         const int tstep = N >> (st - 1);  // stage st twiddle step; stage st+1 uses tstep/2
         const int lq4 = logM - 2;         // items per sequence: M/4 = 2^lq4
+        // lanes walk the butterfly groups while a group spans fewer than 16 elements (their
+        // four reads then fall on distinct banks, and the twiddles are one broadcast read),
+        // the positions inside a group above that
+        const bool gfast = half < 16;
+        const int lg = lq4 - (st - 1);  // log2 of the group count
         for (int e = threadIdx.x; e < (nf << lq4); e += blockDim.x) {
             const int f = e >> lq4, b = e & ((1 << lq4) - 1);
-            const int grp = b >> (st - 1), pos = b & (half - 1);
+            const int grp = gfast ? (b & ((1 << lg) - 1)) : (b >> (st - 1));
+            const int pos = gfast ? (b >> lg) : (b & (half - 1));
             const int l0 = grp * 4 * half + pos;
             const int a0 = f * Mp + fpad<R>(l0), a1 = f * Mp + fpad<R>(l0 + half);
             const int a2 = f * Mp + fpad<R>(l0 + 2 * half), a3 = f * Mp + fpad<R>(l0 + 3 * half);
-            const R2 w1 = twr[pos * tstep];
-            const R2 w2 = twr[pos * (tstep >> 1)];
-            const R2 w3 = twr[(pos + half) * (tstep >> 1)];
+            const R2 w1 = twr[twpad(pos * tstep)];
+            const R2 w2 = twr[twpad(pos * (tstep >> 1))];
+            const R2 w3 = twr[twpad((pos + half) * (tstep >> 1))];
             R2 x0 = zs[a0], x1 = zs[a1], x2 = zs[a2], x3 = zs[a3];
             // stage st: (x0, x1) and (x2, x3) with w1 (e^{-i theta} = (c, -s))
             R cr = x1.x * w1.x + x1.y * w1.y, ci = x1.y * w1.x - x1.x * w1.y;
@@ -915,7 +935,7 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
             const int grp = b >> (st - 1), pos = b & (half - 1);
             const int l0 = grp * 2 * half + pos;
             const int i0 = f * Mp + fpad<R>(l0), i1 = f * Mp + fpad<R>(l0 + half);
-            const R2 w = twr[pos * tstep];
+            const R2 w = twr[twpad(pos * tstep)];
             const R2 a = zs[i0], c = zs[i1];
             const R cr = c.x * w.x + c.y * w.y;
             const R ci = c.y * w.x - c.x * w.y;
@@ -955,12 +975,12 @@ __global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, i
     const int k = k0 + unit % nring;
     const int q0 = (unit / nring) * TQ;
     const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
+    const int N = Rg.npr, M = N >> 1, Mp = (int)fft_plen(M, sizeof(R));
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
     const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1 to 8, the tile search's candidates)
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
-    R2* twr = zs + nf * Mp;               // kk <= N staged in LDS
+    R2* twr = zs + nf * Mp;               // kk <= N staged in LDS (at twpad(kk))
     for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
         double2 w[PH_UL];
 #pragma unroll
@@ -972,8 +992,8 @@ __global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, i
         for (int u = 0; u < PH_UL; ++u) {
             const int e = e0 + u * blockDim.x;
             if (e <= N) {
-                twr[e].x = (R)w[u].x;
-                twr[e].y = (R)w[u].y;
+                twr[twpad(e)].x = (R)w[u].x;
+                twr[twpad(e)].y = (R)w[u].y;
             }
         }
     }
@@ -1030,12 +1050,13 @@ __global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, i
             const R2 zc = z[fpad<R>((M - m) & (M - 1))];
             const R er = (R)0.5 * (zm.x + zc.x), ei = (R)0.5 * (zm.y - zc.y);
             const R orr = (R)0.5 * (zm.y + zc.y), oi = (R)-0.5 * (zm.x - zc.x);
-            const R2 w = twr[2 * m];
+            const R2 w = twr[twpad(2 * m)];
             xr[side] = er + (w.x * orr + w.y * oi);
             xi[side] = ei + (w.x * oi - w.y * orr);
         }
         // ring phases from the LDS table: (m' shift) mod 2N <= L <= N for every m' <= L
-        store_folded<TS>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, (TS)xr[0], (TS)xi[0], (TS)xr[1], (TS)xi[1]);
+        store_folded<TS, true>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, (TS)xr[0], (TS)xi[0], (TS)xr[1],
+                               (TS)xi[1]);
     }
 }
 
@@ -1066,13 +1087,13 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const R* __restrict__ X, int64
     const int k = k0 + unit / nqt;
     const int q0 = (unit - (unit / nqt) * nqt) * TQ;
     const RingDev Rg = rings[k];
-    const int N = Rg.npr, M = N >> 1, Mp = M + (M >> FftPad<R>::SH);
+    const int N = Rg.npr, M = N >> 1, Mp = (int)fft_plen(M, sizeof(R));
     const bool eq = (k == K - 1);
     const int nf = 2 * TQ;
     const int ltq = __ffs(TQ) - 1;  // TQ is a power of two (1 or 2)
     R2* zs = (R2*)zi_raw;                 // nf x Mp (padded sequences)
-    R2* twr = zs + nf * Mp;               // N + 1 twiddles
-    R2* hs = twr + N + 1;                 // nf x (L + 1): H_m per sequence
+    R2* twr = zs + nf * Mp;               // N + 1 twiddles at twpad(kk)
+    R2* hs = twr + tw_len(N);             // nf x (L + 1): H_m per sequence
     const double2* twg = tw + Rg.tw_off;  // (cos, sin)(pi kk / N), kk < 2N
     for (int e0 = threadIdx.x; e0 <= N; e0 += PH_UL * blockDim.x) {  // PH_UL loads in flight
         double2 w[PH_UL];
@@ -1082,8 +1103,8 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const R* __restrict__ X, int64
 #pragma unroll
         for (int u = 0; u < PH_UL; ++u)
             if (e0 + u * (int)blockDim.x <= N) {
-                twr[e0 + u * blockDim.x].x = (R)w[u].x;
-                twr[e0 + u * blockDim.x].y = (R)w[u].y;
+                twr[twpad(e0 + u * blockDim.x)].x = (R)w[u].x;
+                twr[twpad(e0 + u * blockDim.x)].y = (R)w[u].y;
             }
     }
     const int Lk = mall ? mall[k] : L;  // X of this ring pair holds m <= Lk (zero beyond)
@@ -1135,7 +1156,7 @@ __global__ __launch_bounds__(512) void k_phi_ifft(const R* __restrict__ X, int64
         const R2 xa = spec(kk), xb = spec(M - kk);  // X_k, X_{M-k}
         const R er = (R)0.5 * (xa.x + xb.x), ei = (R)0.5 * (xa.y - xb.y);
         const R dr = (R)0.5 * (xa.x - xb.x), di = (R)0.5 * (xa.y + xb.y);
-        const R2 w = twr[2 * kk];  // e^{2 pi i k/N}
+        const R2 w = twr[twpad(2 * kk)];  // e^{2 pi i k/N}
         const R orr = dr * w.x - di * w.y, oi = dr * w.y + di * w.x;
         // Z = E + i O; conj(Z) goes in bit-reversed
         const int rk = (int)(__brev((unsigned)kk) >> (32 - logM));
@@ -1565,7 +1586,7 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
             xr[side] = sr;
             xi[side] = si;
         }
-        store_folded<R>(X, M2, K, k, false, npr, Rg.shift, twl, Lk, kf, q, xr[0], xi[0], xr[1], xi[1]);
+        store_folded<R, false>(X, M2, K, k, false, npr, Rg.shift, twl, Lk, kf, q, xr[0], xi[0], xr[1], xi[1]);
     }
 }
 
@@ -3342,7 +3363,7 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
         const int64_t fit = (budget - 2 * N4 * 16) / (2 * M * 16);
         P->TQF = (int32_t)std::max<int64_t>(1, std::min<int64_t>(2, fit));
         while (P->TQF < 4 && 2 * P->TQF <= fit &&
-               16 * (2 * 2 * P->TQF * fft_plen(M, 8) + 2 * M + 1) <= 49152)
+               16 * (2 * 2 * P->TQF * fft_plen(M, 8) + tw_len(2 * M)) <= 49152)
             P->TQF *= 2;
         P->TQF32 = P->TQF;
     }
@@ -3774,7 +3795,7 @@ static InvShapes inv_shapes(int64_t nside, int64_t L, int64_t lds_max)
     InvShapes S{};
     const int64_t N = 4 * nside, M = N / 2;
     auto ifft_lds = [&](int32_t tq) {
-        return sizeof(double2) * (size_t)(2 * tq * fft_plen(M, sizeof(double)) + N + 1 + 2 * tq * (L + 1));
+        return sizeof(double2) * (size_t)(2 * tq * fft_plen(M, sizeof(double)) + tw_len(N) + 2 * tq * (L + 1));
     };
     S.tqi = 2;
     while (S.tqi > 0 && (int64_t)ifft_lds(S.tqi) > lds_max) --S.tqi;
@@ -3810,7 +3831,7 @@ static int phi_transform(sfb_plan* P, int32_t nq, hipStream_t st, bool f32, bool
     const size_t vb = f32 ? sizeof(float) : sizeof(double);
     // the FFT computes in FP64 on both paths (FP32 maps and X on the mixed one)
     const int32_t tqf = f32 ? P->TQF32 : P->TQF;
-    const size_t shf = 2 * 8 * (2 * (size_t)tqf * (size_t)fft_plen(M, 8) + 2 * (size_t)M + 1);
+    const size_t shf = 2 * 8 * (2 * (size_t)tqf * (size_t)fft_plen(M, 8) + (size_t)tw_len(2 * M));
     // 1-D grid over (map tile, ring pair) units, rounded up to whole runs of 8 units on 8 XCDs
     // (runs of 2, 4 and 8 were timed: FP32 0.92, 0.87, 0.82 ms; the polar rings keep runs of 4)
     const int32_t nring_f = K - k_eq0, ntile_f = (int32_t)nblk(nq, tqf);
@@ -3933,7 +3954,7 @@ static int autotune_phi(sfb_plan* P)
         int32_t btq = tq, bnt = nt;
         float bms = 1e30f;
         for (int32_t cq = 1; cq <= 8 && !rc; cq *= 2) {
-            const size_t sh = 2 * 8 * (2 * (size_t)cq * (size_t)fft_plen(M, 8) + 2 * (size_t)M + 1);
+            const size_t sh = 2 * 8 * (2 * (size_t)cq * (size_t)fft_plen(M, 8) + (size_t)tw_len(2 * M));
             if ((int64_t)sh > P->lds_max) break;
             for (int32_t cn = 256; cn <= 512 && !rc; cn *= 2) {
                 tq = cq;
