@@ -184,6 +184,15 @@ int sfb_apply_window_noalloc(int32_t device, int64_t nside, int64_t nr, double r
                              const double* d_win, int64_t N, const double* d_rtp, const double* d_u,
                              double* d_rtp_out, int64_t* d_n_out, void* stream);
 
+/* The HEALPix pixel window w_l, l = 0 .. lmax, that pixwin(cmodes)
+ * (src/SphericalFourierBesselDecompositions.jl:158-167) reads from HEALPix's tables, computed
+ * from its definition: w_l^2 = (1/Npix) sum_p (4 pi/(2l+1)) sum_m |<Y_lm>_p|^2, the pixel
+ * means taken over the centres of the 4^depth sub-pixels at nside 2^depth and extrapolated
+ * from depths depth-1 and depth (depth 0: 5 up to nside 32, 4 up to 256, 3 above; the
+ * work grows as nside^2 16^depth lmax).  lmax <= min(4 nside, 3000), nside <= 4096.
+ * Runs on `device`; synchronous. */
+int sfb_pixwin(int64_t nside, int64_t lmax, int32_t depth, int32_t device, double* w_out);
+
 /* amln2clnn(anlm1, anlm2, cmodes) (src/SphericalFourierBesselDecompositions.jl:130-150)
  * for the (l, n1, n2) triples lnn[3 x nlnn] of a ClnnModes over the plan's modes:
  * C = (Re(a1_l0 a2_l0) + 2 sum_{m>=1} Re(a1_lm conj(a2_lm))) / (2l + 1). */

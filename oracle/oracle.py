@@ -491,3 +491,61 @@ def delta_map(om: OracleModes, rtp, nbar, win_kind, win_a, win_b, w, q):
                         _p(om.knot_y), _p(om.knot_ypp), N, _p(rtp_c), _p(w), float(nbar), int(win_kind),
                         _p(win_a), _p(win_b), int(q), _p(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# Pixel window (what src/SphericalFourierBesselDecompositions.jl:158-167 reads from HEALPix's
+# tables through Healpix.pixwin), restated from its definition (Gorski et al. 2005):
+# w_l^2 = (1/Npix) sum_p (4 pi / (2l + 1)) sum_{m=-l..l} |<Y_lm>_p|^2, <.>_p the mean over
+# pixel p.  The means are taken over the centres of every pixel's 4^k sub-pixels at nside 2^k
+# (assigned to pixels by ang2pix), for every pixel of the map, and depths k - 1 and k are
+# combined as (4 W_k - W_{k-1}) / 3.
+# --------------------------------------------------------------------------
+def _subpixels(nside, k):
+    nh = nside << k
+    th, ph = pix2ang_ring(nh, np.arange(12 * nh * nh))
+    par = ang2pix_ring(nside, th, ph)
+    if not np.all(np.bincount(par, minlength=12 * nside * nside) == 4 ** k):
+        raise AssertionError("sub-pixels do not tile the pixels")
+    return th, ph, par
+
+
+def pixwin_ylm(nside, lmax, depth):
+    """w_l from the pixel means of Y_lm (scipy sph_harm_y), every pixel."""
+    from scipy import special
+    npix = 12 * nside * nside
+
+    def W(k):
+        th, ph, par = _subpixels(nside, k)
+        out = np.zeros(lmax + 1)
+        for l in range(lmax + 1):
+            s = np.zeros(npix)
+            for m in range(-l, l + 1):
+                y = special.sph_harm_y(l, m, th, ph)
+                mr = np.bincount(par, weights=y.real, minlength=npix) / 4 ** k
+                mi = np.bincount(par, weights=y.imag, minlength=npix) / 4 ** k
+                s += mr * mr + mi * mi
+            out[l] = np.mean(4 * np.pi / (2 * l + 1) * s)
+        return out
+
+    return np.sqrt(np.maximum((4 * W(depth) - W(depth - 1)) / 3, 0.0))
+
+
+def pixwin_pairs(nside, lmax, depth):
+    """w_l from the addition theorem: w_l^2(p) = mean of P_l(n . n') over the pairs of
+    sub-pixel centres of p (scipy eval_legendre), every pixel."""
+    from scipy import special
+    npix = 12 * nside * nside
+
+    def W(k):
+        th, ph, par = _subpixels(nside, k)
+        v = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+        out = np.zeros(lmax + 1)
+        for p in range(npix):
+            u = v[par == p]
+            x = np.clip(u @ u.T, -1.0, 1.0)
+            for l in range(lmax + 1):
+                out[l] += special.eval_legendre(l, x).mean()
+        return out / npix
+
+    return np.sqrt(np.maximum((4 * W(depth) - W(depth - 1)) / 3, 0.0))

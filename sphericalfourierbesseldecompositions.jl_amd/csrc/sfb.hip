@@ -5019,6 +5019,307 @@ extern "C" int sfb_amln2clnn(sfb_plan* P, int64_t nlnn, const int64_t* lnn, cons
     return SFB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Pixel window (the w_l that src/SphericalFourierBesselDecompositions.jl:158-167 reads from
+// HEALPix's tables).  Definition (Gorski et al. 2005): w_l^2 = (1/Npix) sum_p w_l^2(p),
+// w_l^2(p) = (4 pi / (2l + 1)) sum_m |<Y_lm>_p|^2 with <.>_p the mean over pixel p.  By the
+// addition theorem w_l^2(p) = <P_l(n . n')>_{n, n' in p}, the mean of P_l over pairs of
+// points of the pixel; here the points are the centres of its 4^k sub-pixels at nside 2^k
+// (the midpoint rule on equal-area cells, error ~4^-k), and the depths k - 1 and k are
+// combined as (4 S_k - S_{k-1}) / 3.  A southern ring mirrors a northern one.  Rings wholly
+// in the equatorial zone (nside < i <= 2 nside) hold congruent pixels (translates in phi), so
+// one pixel stands for its ring; a ring that reaches into a polar cap (i <= nside) holds
+// congruent pixels only up to the quarter turns and the mirror inside a quarter, so the
+// first ceil(i'/2) pixels of its first quarter (i' pixels per quarter) stand for it.
+// ---------------------------------------------------------------------------
+namespace {
+
+int64_t isqrt64(int64_t v)
+{
+    int64_t r = (int64_t)std::sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+// pixel centre (z = cos theta, phi) in the ring scheme
+void pix_zphi_ring(int64_t nside, int64_t p, double* z, double* phi)
+{
+    const int64_t ncap = 2 * nside * (nside - 1), npix = 12 * nside * nside;
+    const double fact2 = 4.0 / (double)npix;
+    if (p < ncap) {
+        const int64_t ir = (1 + isqrt64(1 + 2 * p)) >> 1;
+        const int64_t ip = p + 1 - 2 * ir * (ir - 1);
+        *z = 1.0 - (double)(ir * ir) * fact2;
+        *phi = ((double)ip - 0.5) * (0.5 * kPi) / (double)ir;
+    } else if (p < npix - ncap) {
+        const int64_t nl4 = 4 * nside, q = p - ncap, t = q / nl4;
+        const int64_t ir = t + nside, ip = q - nl4 * t + 1;
+        const double fodd = ((ir + nside) & 1) ? 1.0 : 0.5;
+        *z = (double)(2 * nside - ir) * 2.0 / (3.0 * (double)nside);
+        *phi = ((double)ip - fodd) * kPi / (2.0 * (double)nside);
+    } else {
+        const int64_t q = npix - p;
+        const int64_t ir = (1 + isqrt64(2 * q - 1)) >> 1;
+        const int64_t ip = 4 * ir + 1 - (q - 2 * ir * (ir - 1));
+        *z = -1.0 + (double)(ir * ir) * fact2;
+        *phi = ((double)ip - 0.5) * (0.5 * kPi) / (double)ir;
+    }
+}
+
+// ring-scheme pixel of (z, phi); used only on sub-pixel centres, which lie inside a pixel
+int64_t zphi_pix_ring(int64_t nside, double z, double phi)
+{
+    const int64_t ncap = 2 * nside * (nside - 1), npix = 12 * nside * nside, nl4 = 4 * nside;
+    double tt = std::fmod(phi, 2.0 * kPi);
+    if (tt < 0.0) tt += 2.0 * kPi;
+    tt /= 0.5 * kPi;  // in [0, 4)
+    const double za = std::fabs(z);
+    if (za <= 2.0 / 3.0) {
+        const double t1 = (double)nside * (0.5 + tt), t2 = (double)nside * z * 0.75;
+        const int64_t jp = (int64_t)std::floor(t1 - t2), jm = (int64_t)std::floor(t1 + t2);
+        const int64_t ir = nside + 1 + jp - jm, kshift = 1 - (ir & 1);
+        int64_t ip = (jp + jm - nside + kshift + 1) / 2;
+        ip = ((ip % nl4) + nl4) % nl4;
+        return ncap + (ir - 1) * nl4 + ip;
+    }
+    const double tp = tt - std::floor(tt);
+    const double tmp = (double)nside * std::sqrt(3.0 * (1.0 - za));
+    const int64_t jp = (int64_t)std::floor(tp * tmp), jm = (int64_t)std::floor((1.0 - tp) * tmp);
+    const int64_t ir = jp + jm + 1;
+    int64_t ip = (int64_t)std::floor(tt * (double)ir);
+    ip = ((ip % (4 * ir)) + 4 * ir) % (4 * ir);
+    return z > 0 ? 2 * ir * (ir - 1) + ip : npix - 2 * ir * (ir + 1) + ip;
+}
+
+// first pixel and pixel count of ring i (1 <= i <= 4 nside - 1)
+void ring_span(int64_t nside, int64_t i, int64_t* start, int64_t* npr)
+{
+    const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    if (i < nside) {
+        *start = 2 * i * (i - 1);
+        *npr = 4 * i;
+    } else if (i <= 3 * nside) {
+        *start = ncap + (i - nside) * 4 * nside;
+        *npr = 4 * nside;
+    } else {
+        const int64_t j = 4 * nside - i;
+        *start = npix - 2 * j * (j + 1);
+        *npr = 4 * j;
+    }
+}
+
+// The representative pixels (ring, first pixel + j) and the number of pixels each stands for
+struct PwRep {
+    int64_t ring, pix;
+    double weight;
+};
+std::vector<PwRep> pixwin_reps(int64_t nside)
+{
+    std::vector<PwRep> reps;
+    for (int64_t i = 1; i <= 2 * nside; ++i) {
+        int64_t st, npr;
+        ring_span(nside, i, &st, &npr);
+        const double ns = i < 2 * nside ? 2.0 : 1.0;  // the southern mirror ring
+        if (i > nside) {
+            reps.push_back({i, st, ns * (double)npr});
+            continue;
+        }
+        const int64_t q = npr / 4;  // pixels per quarter turn
+        for (int64_t j = 0; j < (q + 1) / 2; ++j) reps.push_back({i, st + j, ns * 4.0 * (j == q - 1 - j ? 1.0 : 2.0)});
+    }
+    return reps;
+}
+
+// unit vectors of the 4^k sub-pixel centres of each representative pixel ([rep][child][3]);
+// false when a pixel does not yield exactly 4^k of them
+bool pixwin_children(int64_t nside, int k, const std::vector<PwRep>& reps, std::vector<double>& vec)
+{
+    const int64_t f = (int64_t)1 << k, nh = nside * f, nch = f * f;
+    vec.assign(reps.size() * (size_t)nch * 3, 0.0);
+    for (size_t r = 0; r < reps.size(); ++r) {
+        const int64_t i = reps[r].ring, st = reps[r].pix;
+        int64_t st0, npr;
+        ring_span(nside, i, &st0, &npr);
+        (void)st0;
+        double zc, phic;
+        pix_zphi_ring(nside, st, &zc, &phic);
+        const double half = 3.0 * kPi / (double)npr;  // beyond a pixel's phi extent (< one spacing)
+        int64_t got = 0;
+        const int64_t ih0 = std::max<int64_t>(1, (i - 1) * f), ih1 = std::min<int64_t>(4 * nh - 1, (i + 1) * f);
+        for (int64_t ih = ih0; ih <= ih1; ++ih) {
+            int64_t sh, nprh;
+            ring_span(nh, ih, &sh, &nprh);
+            double z0, phi0;
+            pix_zphi_ring(nh, sh, &z0, &phi0);
+            const double dphi = 2.0 * kPi / (double)nprh;
+            const int64_t j0 = (int64_t)std::floor((phic - half - phi0) / dphi) - 1;
+            const int64_t j1 = (int64_t)std::ceil((phic + half - phi0) / dphi) + 1;
+            for (int64_t j = j0; j <= j1 && j - j0 < nprh; ++j) {
+                const int64_t jj = ((j % nprh) + nprh) % nprh;
+                double z, phi;
+                pix_zphi_ring(nh, sh + jj, &z, &phi);
+                if (zphi_pix_ring(nside, z, phi) != st) continue;
+                if (got >= nch) return false;
+                const double s = std::sqrt((1.0 - z) * (1.0 + z));
+                double* v = &vec[(r * (size_t)nch + (size_t)got) * 3];
+                v[0] = s * std::cos(phi);
+                v[1] = s * std::sin(phi);
+                v[2] = z;
+                ++got;
+            }
+        }
+        if (got != nch) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+constexpr int PW_PP = 8, PW_LC = 16, PW_BPR = 16;  // pairs per thread item, l per chunk, blocks per ring
+
+// S[rep][blk][l] = sum of P_l(n . n') over the block's share of the representative pixel's
+// ordered pairs of sub-pixel centres.  A thread takes PW_PP pairs at a time through the Legendre
+// recurrence in chunks of PW_LC degrees; each chunk is summed over the wave (fixed shuffle
+// order) into the wave's LDS row, and the four rows are added at the end: the sums do not
+// depend on timing.
+__global__ __launch_bounds__(256) void k_pixwin_pairs(int32_t nch, int32_t lmax, const double* __restrict__ vec,
+                                                      double* __restrict__ part)
+{
+    extern __shared__ double pw_sm[];
+    const int64_t ring = blockIdx.x / PW_BPR;
+    const int blk = (int)(blockIdx.x - ring * PW_BPR);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nl = lmax + 1;
+    double* ca = pw_sm;           // (2l + 1) / (l + 1)
+    double* cb = pw_sm + nl;      // l / (l + 1)
+    double* wacc = pw_sm + 2 * nl + wv * nl;
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        ca[l] = (double)(2 * l + 1) / (double)(l + 1);
+        cb[l] = (double)l / (double)(l + 1);
+    }
+    for (int l = lane; l < nl; l += 64) wacc[l] = 0.0;
+    __syncthreads();
+    const double* v = vec + (int64_t)ring * nch * 3;
+    const int64_t npair = (int64_t)nch * nch;
+    const int64_t per = (npair + PW_BPR - 1) / PW_BPR;
+    const int64_t p0 = (int64_t)blk * per, p1 = min(npair, p0 + per);
+    const int64_t nit = p1 > p0 ? (p1 - p0 + 256 * PW_PP - 1) / (256 * PW_PP) : 0;  // uniform trip count
+    for (int64_t it = 0; it < nit; ++it) {
+        double x[PW_PP], pm[PW_PP], pc[PW_PP];
+        bool ok[PW_PP];
+#pragma unroll
+        for (int u = 0; u < PW_PP; ++u) {
+            const int64_t e = p0 + (it * 256 + threadIdx.x) * PW_PP + u;
+            ok[u] = e < p1;
+            double d = 0.0;
+            if (ok[u]) {
+                const int64_t c = e / nch, cc = e - c * nch;
+                const double* a = v + 3 * c;
+                const double* b = v + 3 * cc;
+                d = fmin(1.0, fmax(-1.0, fma(a[0], b[0], fma(a[1], b[1], a[2] * b[2]))));
+            }
+            x[u] = d;
+            pm[u] = 0.0;
+            pc[u] = 1.0;
+        }
+        for (int l0 = 0; l0 < nl; l0 += PW_LC) {
+            double acc[PW_LC];
+#pragma unroll
+            for (int j = 0; j < PW_LC; ++j) {
+                const int l = min(l0 + j, lmax);
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < PW_PP; ++u) s += ok[u] ? pc[u] : 0.0;
+                acc[j] = s;
+                const double al = ca[l], bl = cb[l];
+#pragma unroll
+                for (int u = 0; u < PW_PP; ++u) {
+                    const double pn = fma(al * x[u], pc[u], -bl * pm[u]);
+                    pm[u] = pc[u];
+                    pc[u] = pn;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PW_LC; ++j) {
+                double t = acc[j];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+                if (lane == 0 && l0 + j < nl) wacc[l0 + j] += t;
+            }
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        const double* w0 = pw_sm + 2 * nl;
+        part[((int64_t)ring * PW_BPR + blk) * nl + l] = ((w0[l] + w0[nl + l]) + w0[2 * nl + l]) + w0[3 * nl + l];
+    }
+}
+
+// w_l = sqrt((4 W_k - W_{k-1}) / 3), W = (1/Npix) sum_reps weight S / nch^2
+__global__ void k_pixwin_final(int64_t nside, int32_t lmax, int32_t nrep, const double* __restrict__ weight,
+                               int32_t nch1, const double* __restrict__ part1, int32_t nch0,
+                               const double* __restrict__ part0, double* __restrict__ w)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > lmax) return;
+    const int nl = lmax + 1;
+    const double npix = 12.0 * (double)nside * (double)nside;
+    double W[2];
+    for (int d = 0; d < 2; ++d) {
+        const double* part = d ? part1 : part0;
+        const double nch = (double)(d ? nch1 : nch0);
+        double sum = 0.0;
+        for (int64_t r = 0; r < nrep; ++r) {
+            double s = 0.0;
+            for (int b = 0; b < PW_BPR; ++b) s += part[(r * PW_BPR + b) * nl + l];
+            sum += weight[r] * s;
+        }
+        W[d] = sum / (npix * nch * nch);
+    }
+    const double w2 = (4.0 * W[1] - W[0]) / 3.0;
+    w[l] = sqrt(fmax(w2, 0.0));
+}
+
+extern "C" int sfb_pixwin(int64_t nside, int64_t lmax, int32_t depth, int32_t device, double* w_out)
+{
+    sfb::clear_error();
+    if (nside < 1 || nside > 4096 || lmax < 0 || lmax > 4 * nside || lmax > 3000 || !w_out)
+        return fail(SFB_ERR_ARG, "pixwin needs 1 <= nside <= 4096 and 0 <= lmax <= min(4 nside, 3000)");
+    const int k = depth != 0 ? depth : (nside <= 32 ? 5 : (nside <= 256 ? 4 : 3));
+    if (k < 2 || k > 6) return fail(SFB_ERR_ARG, "pixwin depth must be 2 .. 6 (0: 5)");
+    const std::vector<PwRep> reps = pixwin_reps(nside);
+    std::vector<double> v1, v0, wt;
+    if (!pixwin_children(nside, k, reps, v1) || !pixwin_children(nside, k - 1, reps, v0))
+        return fail(SFB_ERR_ARG, "pixwin: the sub-pixel search did not find 4^k centres per pixel");
+    for (const PwRep& r : reps) wt.push_back(r.weight);
+    HIP_TRY(hipSetDevice(device));
+    const int64_t nring = (int64_t)reps.size(), nl = lmax + 1;
+    DevBuf dv1, dv0, dp1, dp0, dw, dwt;
+    int rc;
+    if ((rc = dv1.ensure(8 * v1.size())) || (rc = dv0.ensure(8 * v0.size())) ||
+        (rc = dp1.ensure(8 * (size_t)(nring * PW_BPR * nl))) || (rc = dp0.ensure(8 * (size_t)(nring * PW_BPR * nl))) ||
+        (rc = dw.ensure(8 * (size_t)nl)) || (rc = dwt.ensure(8 * wt.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(dv1.p, v1.data(), 8 * v1.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dv0.p, v0.data(), 8 * v0.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dwt.p, wt.data(), 8 * wt.size(), hipMemcpyHostToDevice));
+    const size_t shm = 8 * (size_t)(6 * nl);
+    HIP_TRY(hipFuncSetAttribute((const void*)k_pixwin_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const int32_t nch1 = 1 << (2 * k), nch0 = 1 << (2 * (k - 1));
+    hipLaunchKernelGGL(k_pixwin_pairs, dim3((unsigned)(PW_BPR * nring)), dim3(256), shm, 0, nch1, (int32_t)lmax,
+                       dv1.as<double>(), dp1.as<double>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pixwin_pairs, dim3((unsigned)(PW_BPR * nring)), dim3(256), shm, 0, nch0, (int32_t)lmax,
+                       dv0.as<double>(), dp0.as<double>());
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pixwin_final, dim3(nblk(nl, 128)), dim3(128), 0, 0, nside, (int32_t)lmax, (int32_t)nring,
+                       dwt.as<double>(), nch1, dp1.as<double>(), nch0, dp0.as<double>(), dw.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpy(w_out, dw.p, 8 * (size_t)nl, hipMemcpyDeviceToHost));
+    return SFB_OK;
+}
+
 extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const double* w, double nbar,
                              int32_t win_kind, const double* win_a, const double* win_b, int64_t q, double* map_out)
 {

@@ -19,7 +19,7 @@ module SFBMI355X
 import ChainRulesCore
 import SphericalFourierBesselDecompositions as SFB
 
-export cat2amln, cat2nlm, cat2amln_vjp, cat2amln_batch, win_rhat_ln, field2anlm, anlm2field, amln2clnn
+export cat2amln, cat2nlm, cat2amln_vjp, cat2amln_batch, win_rhat_ln, field2anlm, anlm2field, amln2clnn, pixwin
 
 const libsfb = get(ENV, "SFB_MI355X_LIB",
                    joinpath(@__DIR__, "..", "sfb_amd", "libsfb_mi355x.so"))
@@ -233,6 +233,18 @@ function amln2clnn(anlm1, anlm2, cmodes; device::Integer=0)
     return out
 end
 amln2clnn(anlm, cmodes; kw...) = amln2clnn(anlm, anlm, cmodes; kw...)
+
+"""pixwin(cmodes) (src/SphericalFourierBesselDecompositions.jl:158-167): the pixel window w_l of
+every (l, n1, n2) entry, computed on the GPU from the window's definition (HEALPix's tables,
+which the reference reads, are not used); `depth` sets the sub-pixel depth (0: by nside)."""
+function pixwin(cmodes; depth::Integer=0, device::Integer=0)
+    lnn = Matrix{Int64}(cmodes.lnn)
+    lmax = maximum(lnn[1, :])
+    w = Vector{Float64}(undef, lmax + 1)
+    check(ccall((:sfb_pixwin, libsfb), Cint, (Int64, Int64, Int32, Int32, Ptr{Float64}),
+                cmodes.amodesA.nside, lmax, Int32(depth), Int32(device), w))
+    return [w[lnn[1, i] + 1] for i in 1:size(lnn, 2)]
+end
 
 function ChainRulesCore.rrule(::typeof(cat2amln), rθϕ, amodes, nbar, win_rhat_ln,
                               weight=ones(eltype(rθϕ), size(rθϕ, 2)); kw...)

@@ -15,7 +15,7 @@ from ._lib import SFBError, lib  # noqa: F401
 from .cat2anlm import (Plan, anlm2field, ang2pix_ring, cat2amln, cat2amln_vjp, cat2nlm, field2anlm,  # noqa: F401
                        get_plan)  # noqa: F401
 from .catalog import apply_window, winweights2galweights, xyz2rtp  # noqa: F401
-from .clnn import ClnnModes, amln2clnn, anlm2clnn, getlnn, getlnnsize  # noqa: F401
+from .clnn import ClnnModes, amln2clnn, anlm2clnn, getlnn, getlnnsize, healpix_pixwin, pixwin  # noqa: F401
 from .modes import (AnlmModes, GNL, Spline1D, estimate_nside, getidx, getlmsize, getnlm,  # noqa: F401
                     getnlmsize, isvalidnlm)
 from .windows import (ConfigurationSpaceModes, SeparableArray, get_rbounds, integrate_window,  # noqa: F401
@@ -25,5 +25,5 @@ __all__ = [
     "AnlmModes", "GNL", "Spline1D", "estimate_nside", "getidx", "getlmsize", "getnlm", "getnlmsize",
     "isvalidnlm", "ConfigurationSpaceModes", "SeparableArray", "get_rbounds", "integrate_window",
     "win_rhat_ln", "window_r", "xyz2rtp", "apply_window", "winweights2galweights", "ClnnModes",
-    "amln2clnn", "anlm2clnn", "getlnn", "getlnnsize", "cat2amln", "cat2amln_vjp", "cat2nlm", "field2anlm", "anlm2field", "Plan", "get_plan", "ang2pix_ring", "SFBError",
+    "amln2clnn", "anlm2clnn", "getlnn", "getlnnsize", "pixwin", "healpix_pixwin", "cat2amln", "cat2amln_vjp", "cat2nlm", "field2anlm", "anlm2field", "Plan", "get_plan", "ang2pix_ring", "SFBError",
 ]

@@ -79,3 +79,24 @@ def amln2clnn(anlm1, *args, device=0):
 
 
 anlm2clnn = amln2clnn
+
+
+def healpix_pixwin(nside, lmax=None, *, depth=0, device=0):
+    """The HEALPix pixel window w_l, l = 0..lmax (default 4 nside): what Healpix.pixwin(nside)
+    returns from HEALPix's tables, here computed on the device from its definition
+    (w_l^2 = mean over pixels of (4 pi/(2l+1)) sum_m |<Y_lm>_pixel|^2, pixel means over the
+    4^depth sub-pixel centres, extrapolated from depths depth-1 and depth; depth 0 picks 5 up
+    to nside 32, 4 up to nside 256 and 3 above)."""
+    nside = int(nside)
+    lmax = 4 * nside if lmax is None else int(lmax)
+    out = np.empty(lmax + 1)
+    L.check(L.lib().sfb_pixwin(nside, lmax, int(depth), int(device), L.ptr(out)))
+    return out
+
+
+def pixwin(cmodes: ClnnModes, *, depth=0, device=0):
+    """pixwin(cmodes) (src/SphericalFourierBesselDecompositions.jl:158-167): the angular pixel
+    window w_l of every (l, n1, n2) entry of a C_l(n, n') vector."""
+    w = healpix_pixwin(cmodes.amodes.nside, int(cmodes.lnn[:, 0].max()) if len(cmodes.lnn) else 0,
+                       depth=depth, device=device)
+    return w[cmodes.lnn[:, 0]]
