@@ -199,6 +199,12 @@ int sfb_pixwin(int64_t nside, int64_t lmax, int32_t depth, int32_t device, doubl
 int sfb_amln2clnn(sfb_plan* p, int64_t nlnn, const int64_t* lnn, const double* anlm1, const double* anlm2,
                   double* clnn);
 
+/* The same for a ClnnModes over two AnlmModes (src/modes.jl:396-399, ClnnModes{false}):
+ * anlm1 in the order of plan pa's modes, anlm2 in that of pb's, n1 a mode of pa and n2 of
+ * pb (src/SphericalFourierBesselDecompositions.jl:133-134).  Both plans on one device. */
+int sfb_amln2clnn_cross(sfb_plan* pa, sfb_plan* pb, int64_t nlnn, const int64_t* lnn, const double* anlm1,
+                        const double* anlm2, double* clnn);
+
 /* ---- adjoint ----------------------------------------------------------------- */
 
 /* Vector-Jacobian product of cat2amln at (rθϕ, w, nbar): given dL_danlm (2*nlmsize

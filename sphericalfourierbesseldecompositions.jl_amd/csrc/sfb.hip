@@ -2575,14 +2575,16 @@ __global__ __launch_bounds__(1024) void k_keep_scatter(int64_t N, const int32_t*
 
 // clnn[i] = (Re(a1_l0 a2_l0) + 2 sum_{m>=1} Re(a1_lm conj(a2_lm))) / (2l + 1) for
 // (l, n1, n2) = lnn[:, i] (alm2cl, src/SphericalFourierBesselDecompositions.jl:114-128)
-__global__ void k_amln2clnn(int64_t nlnn, const int64_t* __restrict__ lnn, const int32_t* __restrict__ qbase,
-                            const int64_t* __restrict__ out_base, const double* __restrict__ a1,
+// a1 in the a_nlm order of modes A (qbase1, out_base1), a2 in that of modes B
+__global__ void k_amln2clnn(int64_t nlnn, const int64_t* __restrict__ lnn, const int32_t* __restrict__ qbase1,
+                            const int64_t* __restrict__ out_base1, const int32_t* __restrict__ qbase2,
+                            const int64_t* __restrict__ out_base2, const double* __restrict__ a1,
                             const double* __restrict__ a2, double* __restrict__ clnn)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nlnn) return;
     const int64_t l = lnn[3 * i], n1 = lnn[3 * i + 1], n2 = lnn[3 * i + 2];
-    const int64_t b1 = 2 * out_base[qbase[n1 - 1] + l], b2 = 2 * out_base[qbase[n2 - 1] + l];
+    const int64_t b1 = 2 * out_base1[qbase1[n1 - 1] + l], b2 = 2 * out_base2[qbase2[n2 - 1] + l];
     double c = a1[b1] * a2[b2] - a1[b1 + 1] * a2[b2 + 1];
     for (int64_t m = 1; m <= l; ++m) c += 2.0 * (a1[b1 + 2 * m] * a2[b2 + 2 * m] + a1[b1 + 2 * m + 1] * a2[b2 + 2 * m + 1]);
     clnn[i] = c / (double)(2 * l + 1);
@@ -4989,34 +4991,41 @@ extern "C" int sfb_apply_window_noalloc(int32_t device, int64_t nside, int64_t n
     return SFB_OK;
 }
 
-extern "C" int sfb_amln2clnn(sfb_plan* P, int64_t nlnn, const int64_t* lnn, const double* anlm1, const double* anlm2,
-                             double* clnn)
+extern "C" int sfb_amln2clnn_cross(sfb_plan* PA, sfb_plan* PB, int64_t nlnn, const int64_t* lnn,
+                                   const double* anlm1, const double* anlm2, double* clnn)
 {
     sfb::clear_error();
-    if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (!PA || !PB) return fail(SFB_ERR_ARG, "plan is NULL");
+    if (PA->device != PB->device) return fail(SFB_ERR_ARG, "amln2clnn: the two plans are on different devices");
     if (nlnn < 0 || (nlnn > 0 && (!lnn || !anlm1 || !anlm2 || !clnn))) return fail(SFB_ERR_ARG, "bad amln2clnn arguments");
     for (int64_t i = 0; i < nlnn; ++i) {
         const int64_t l = lnn[3 * i], n1 = lnn[3 * i + 1], n2 = lnn[3 * i + 2];
-        if (n1 < 1 || n2 < 1 || n1 > P->nmax || n2 > P->nmax || l < 0 || l > P->lmaxn_h[(size_t)(n1 - 1)] ||
-            l > P->lmaxn_h[(size_t)(n2 - 1)])
-            return fail(SFB_ERR_ARG, "lnn entry %lld is not a mode of the plan", (long long)i);
+        if (n1 < 1 || n2 < 1 || n1 > PA->nmax || n2 > PB->nmax || l < 0 || l > PA->lmaxn_h[(size_t)(n1 - 1)] ||
+            l > PB->lmaxn_h[(size_t)(n2 - 1)])
+            return fail(SFB_ERR_ARG, "lnn entry %lld is not a mode pair of the plans", (long long)i);
     }
     if (nlnn == 0) return SFB_OK;
-    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipSetDevice(PA->device));
     DevBuf dl, d1, d2, dc;
     int rc;
-    if ((rc = dl.ensure(24 * (size_t)nlnn)) || (rc = d1.ensure(16 * (size_t)P->nlmsize)) ||
-        (rc = d2.ensure(16 * (size_t)P->nlmsize)) || (rc = dc.ensure(8 * (size_t)nlnn)))
+    if ((rc = dl.ensure(24 * (size_t)nlnn)) || (rc = d1.ensure(16 * (size_t)PA->nlmsize)) ||
+        (rc = d2.ensure(16 * (size_t)PB->nlmsize)) || (rc = dc.ensure(8 * (size_t)nlnn)))
         return rc;
     HIP_TRY(hipMemcpy(dl.p, lnn, 24 * (size_t)nlnn, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d1.p, anlm1, 16 * (size_t)P->nlmsize, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d2.p, anlm2, 16 * (size_t)P->nlmsize, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d1.p, anlm1, 16 * (size_t)PA->nlmsize, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2.p, anlm2, 16 * (size_t)PB->nlmsize, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_amln2clnn, dim3(nblk(nlnn, 256)), dim3(256), 0, 0, nlnn, dl.as<int64_t>(),
-                       P->d_qbase.as<int32_t>(), P->d_outbase.as<int64_t>(), d1.as<double>(), d2.as<double>(),
-                       dc.as<double>());
+                       PA->d_qbase.as<int32_t>(), PA->d_outbase.as<int64_t>(), PB->d_qbase.as<int32_t>(),
+                       PB->d_outbase.as<int64_t>(), d1.as<double>(), d2.as<double>(), dc.as<double>());
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpy(clnn, dc.p, 8 * (size_t)nlnn, hipMemcpyDeviceToHost));
     return SFB_OK;
+}
+
+extern "C" int sfb_amln2clnn(sfb_plan* P, int64_t nlnn, const int64_t* lnn, const double* anlm1, const double* anlm2,
+                             double* clnn)
+{
+    return sfb_amln2clnn_cross(P, P, nlnn, lnn, anlm1, anlm2, clnn);
 }
 
 // ---------------------------------------------------------------------------

@@ -221,14 +221,15 @@ function anlm2field(f_nlm, wmodes, amodes; device::Integer=0)
 end
 
 """amln2clnn(anlm1, anlm2, cmodes) (src/SphericalFourierBesselDecompositions.jl:130-150) on
-the GPU, for a ClnnModes over a single AnlmModes."""
+the GPU, for a ClnnModes over one AnlmModes or two (anlm1 of amodesA, anlm2 of amodesB)."""
 function amln2clnn(anlm1, anlm2, cmodes; device::Integer=0)
-    p = plan(cmodes.amodesA; device)
+    pa = plan(cmodes.amodesA; device)
+    pb = plan(cmodes.amodesB; device)  # the same plan for a ClnnModes over one AnlmModes
     lnn = Matrix{Int64}(cmodes.lnn)
     a1, a2 = Vector{ComplexF64}(anlm1), Vector{ComplexF64}(anlm2)
     out = Vector{Float64}(undef, size(lnn, 2))
-    rc = ccall((:sfb_amln2clnn, libsfb), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64},
-               Ptr{Float64}), p.ptr, size(lnn, 2), lnn, a1, a2, out)
+    rc = ccall((:sfb_amln2clnn_cross, libsfb), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{ComplexF64},
+               Ptr{ComplexF64}, Ptr{Float64}), pa.ptr, pb.ptr, size(lnn, 2), lnn, a1, a2, out)
     check(rc)
     return out
 end

@@ -58,6 +58,7 @@ SIGNATURES = {
     "sfb_apply_window": (ctypes.c_int, [_i32, _i64, _i64, _dbl, _dbl, _dbl, _P, _i64, _P, _P, _P, _P]),
     "sfb_apply_window_noalloc": (ctypes.c_int, [_i32, _i64, _i64, _dbl, _dbl, _dbl, _P, _i64, _P, _P, _P, _P, _P]),
     "sfb_amln2clnn": (ctypes.c_int, [_P, _i64, _P, _P, _P, _P]),
+    "sfb_amln2clnn_cross": (ctypes.c_int, [_P, _P, _i64, _P, _P, _P, _P]),
     "sfb_pixwin": (ctypes.c_int, [_i64, _i64, _i32, _i32, _P]),
     "sfb_cat2amln_vjp": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _P, _P, _P, _P]),
     "sfb_cat2amln_vjp_device": (ctypes.c_int, [_P, _i64, _P, _P, _dbl, _P, _P, _P, _P]),

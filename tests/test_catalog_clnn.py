@@ -97,3 +97,52 @@ def test_amln2clnn():
         y = a2[S.getidx(am, n2, l, 0) - 1:S.getidx(am, n2, l, l)]
         expect = (np.real(x[0] * y[0]) + 2 * np.sum(np.real(x[1:] * np.conj(y[1:])))) / (2 * l + 1)
         assert c12[i] == pytest.approx(expect, rel=1e-13, abs=1e-15)
+
+
+def test_clnn_modes_cross():
+    """ClnnModes over two AnlmModes (src/modes.jl:396-399): every (n1 of A, n2 of B) for
+    l <= min(lmax_A, lmax_B), sorted by l, n2 - n1, n1; n1 and n2 are not swapped."""
+    amA = S.AnlmModes.from_nl(3, 2, 500.0, 1000.0, nside=8)
+    amB = S.AnlmModes.from_nl(2, 3, 400.0, 1000.0, nside=8)
+    cm = S.ClnnModes(amA, amB)
+    assert not cm.symmetric and not hasattr(cm, "amodes")
+    lmax = min(amA.lmax, amB.lmax)
+    assert S.getlnnsize(cm) == sum(int(amA.nmax_l[l]) * int(amB.nmax_l[l]) for l in range(lmax + 1))
+    keys = [(l, n2 - n1, n1) for l, n1, n2 in cm.lnn.tolist()]
+    assert keys == sorted(keys) and len(set(map(tuple, cm.lnn.tolist()))) == len(keys)
+    for i in range(1, S.getlnnsize(cm) + 1):
+        l, n1, n2 = S.getlnn(cm, i)
+        assert C.getidx(cm, l, n1, n2) == i
+    assert (0, 2, 1) in map(tuple, cm.lnn.tolist()) and (0, 1, 2) in map(tuple, cm.lnn.tolist())
+    # over one AnlmModes given twice: every pair, of which the symmetric ClnnModes holds n1 <= n2
+    both = set(map(tuple, S.ClnnModes(amA, amA).lnn.tolist()))
+    assert set(map(tuple, S.ClnnModes(amA).lnn.tolist())) == {t for t in both if t[1] <= t[2]}
+    dk = 0.004
+    cmk = S.ClnnModes(amA, amB, Δkmax=dk)
+    for l, n1, n2 in cmk.lnn.tolist():
+        assert abs(amB.knl[n2 - 1, l] - amA.knl[n1 - 1, l]) <= dk
+
+
+@pytest.mark.gpu
+def test_amln2clnn_cross():
+    """Cross spectra between two mode sets against the reference's alm2cl over getidx of each
+    (src/SphericalFourierBesselDecompositions.jl:114-150), and the cross form over one
+    AnlmModes equal to the symmetric one on its entries."""
+    amA = S.AnlmModes(0.05, 500.0, 1000.0, nside=32)
+    amB = S.AnlmModes(0.04, 400.0, 1000.0, nside=32)
+    rng = np.random.default_rng(53)
+    a1 = rng.normal(size=S.getnlmsize(amA)) + 1j * rng.normal(size=S.getnlmsize(amA))
+    a2 = rng.normal(size=S.getnlmsize(amB)) + 1j * rng.normal(size=S.getnlmsize(amB))
+    cm = S.ClnnModes(amA, amB, Δnmax=3)
+    c = S.amln2clnn(a1, a2, cm)
+    ref = np.empty_like(c)
+    for i, (l, n1, n2) in enumerate(cm.lnn.tolist()):
+        x = a1[[S.getidx(amA, n1, l, m) - 1 for m in range(l + 1)]]
+        y = a2[[S.getidx(amB, n2, l, m) - 1 for m in range(l + 1)]]
+        ref[i] = (np.real(x[0] * y[0]) + 2 * np.sum(np.real(x[1:] * np.conj(y[1:])))) / (2 * l + 1)
+    assert np.abs(c - ref).max() <= 1e-13 * np.abs(ref).max()
+    cs, cx = S.ClnnModes(amA), S.ClnnModes(amA, amA)
+    b = rng.normal(size=S.getnlmsize(amA)) + 1j * rng.normal(size=S.getnlmsize(amA))
+    vs = dict(zip(map(tuple, cs.lnn.tolist()), S.amln2clnn(a1, b, cs)))
+    vx = dict(zip(map(tuple, cx.lnn.tolist()), S.amln2clnn(a1, b, cx)))
+    assert all(vs[t] == vx[t] for t in vs)
