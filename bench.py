@@ -274,6 +274,15 @@ def main():
     err_mixed_vs_fp64 = errors(anlm["mixed"], anlm["fp64"], am)
     log(f"mixed vs FP64 GPU over {err_mixed_vs_fp64['blocks']} blocks: {err_mixed_vs_fp64}")
 
+    # -- split-BF16 Jacobi GEMMs (hi + mid + lo and hi + lo bf16 terms), device-resident -----
+    split_bf16 = {}
+    for prec in ("bf16x3", "bf16x2"):
+        plan.set_precision(prec)
+        ms = 1e3 * timed(dev_call, args.steps, args.warmup, sync)
+        e = errors(d_out.cpu().numpy().view(np.complex128).copy(), anlm["fp64"], am)
+        split_bf16[prec] = {"device_resident_ms": ms, "max_blk_vs_fp64": e["max_blk"], "rms_blk_vs_fp64": e["rms_blk"]}
+        log(f"[C2 {prec} device-resident] {ms:.2f} ms/call, max_blk vs FP64 GPU {e['max_blk']:.2e}")
+
     # -- C5: adjoint (dL/dr, dL/dw for a random cotangent), device-resident ------------------
     adjoint = None
     if not args.no_adjoint:
@@ -476,6 +485,7 @@ def main():
         "max_blk_err": err_oracle.get("mixed", {}).get("max_blk"),
         "errors_vs_oracle": err_oracle,
         "errors_mixed_vs_fp64_gpu_all_maps": err_mixed_vs_fp64,
+        "split_bf16_jacobi": split_bf16,
         "error_definitions": "max_rel / rms_rel: per mode |a - ref| / |ref|; max_blk: max over (n,l) blocks of "
                              "max_m |a - ref| / RMS_m |ref|; rms_blk: RMS over modes of |a - ref| / RMS_block |ref|",
         "phases_ms": phases,

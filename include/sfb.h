@@ -48,6 +48,11 @@ extern "C" {
                                 on FP32 MFMA (FP32 iterate and aliasing buffers); FP64 setup, segment
                                 search, pixel sums, lambda tables, GEMM epilogues, aliasing sums and
                                 the l = 0 maps' monopoles.  Every (n,l) block within 1e-6 of FP64. */
+#define SFB_PREC_BF16X3 2    /* SFB_PREC_MIXED with the Jacobi GEMMs on bf16 MFMA: each FP32 operand split
+                                into hi + mid + lo bf16 terms, six products, FP32 accumulation */
+#define SFB_PREC_BF16X2 3    /* the same with hi + lo terms, three products (~2^-16 per product; the
+                                Jacobi steps only correct the FP64-MFMA first analysis, so every
+                                (n,l) block still stays within 1e-6 of FP64) */
 
 #define SFB_WIN_DENSE 0      /* win_a = W[npix, lmax+1, nmax] */
 #define SFB_WIN_SEPARABLE 1  /* win_a = mask[npix], win_b = w_ln[lmax+1, nmax] */
@@ -310,6 +315,11 @@ int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const doubl
 /* The same GEMM through the FP32-operand MFMA kernel used for the Jacobi corrections
  * of SFB_PREC_MIXED (operands rounded to FP32, FP64 accumulation across k-tiles). */
 int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
+
+/* The same GEMM with split-BF16 operands in `terms` bf16 terms (2: hi + lo, three products;
+ * 3: hi + mid + lo, six products), the kernel of the SFB_PREC_BF16X2 / X3 Jacobi steps. */
+int sfb_gemm_bf16(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C,
+                  int32_t terms);
 
 /* Launch shapes of the ring transforms for a geometry, computed on the host (no device):
  * out[0] polar ring pairs on the forward radix-4 kernel (nside - 1), out[1] its LDS bytes

@@ -1602,11 +1602,42 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
 constexpr int GT_BM = 64, GT_BK = 16, GT_NTM = 5, GT_BNW = 16 * GT_NTM;
 constexpr int GT_ALD = GT_BM * GT_BK / 256;  // A elements per thread per k-step
 
-template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0, bool BF = false>
+// Split-BF16 operands (SB = 2: hi + lo, SB = 3: hi + mid + lo).  An FP32 value v splits
+// exactly: hi = RN_bf16(v), mid = RN_bf16(v - hi), lo = RN_bf16(v - hi - mid) (each
+// difference is exact in FP32); a product is hi*hi + hi*mid + mid*hi (SB 2: with lo in
+// place of mid) and, for SB 3, + hi*lo + lo*hi + mid*mid, the dropped terms ~2^-24 of it.
+// The bf16 products are exact in the MFMA's FP32 accumulation.
+typedef short short4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ short bf16_rn(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+template <int SB>
+__device__ __forceinline__ void bf16_split(const float (&v)[4], short4_t& h, short4_t& m, short4_t& l)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const short hj = bf16_rn(v[j]);
+        const float r = v[j] - bf16_f(hj);
+        const short mj = bf16_rn(r);
+        h[j] = hj;
+        m[j] = mj;
+        l[j] = SB == 3 ? bf16_rn(r - bf16_f(mj)) : (short)0;
+    }
+}
+__device__ __forceinline__ float4_t mfma_bf16(short4_t a, short4_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
+    static_assert(SB == 0 || F32, "split-BF16 operands replace FP32 ones");
     typedef MfmaT<F32> M;
     typedef typename M::T T;
     constexpr int BNW = 16 * NT;  // active columns of this block
@@ -1672,12 +1703,38 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
         __syncthreads();
         if (k0 + GT_BK < d.K) fetch(k0 + GT_BK);
+        if constexpr (SB == 0) {
 #pragma unroll
-        for (int ks = 0; ks < GT_BK; ks += 4) {
-            const int kk = ks + (lane >> 4);
-            const T fb = As[kk][wi + (lane & 15)];
+            for (int ks = 0; ks < GT_BK; ks += 4) {
+                const int kk = ks + (lane >> 4);
+                const T fb = As[kk][wi + (lane & 15)];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = M::mma(Bs[kk][t * 16 + (lane & 15)], fb, acc[t]);
+                for (int t = 0; t < NT; ++t) acc[t] = M::mma(Bs[kk][t * 16 + (lane & 15)], fb, acc[t]);
+            }
+        } else {
+            // the whole 16-deep tile per bf16 MFMA: lane holds k = 4 (lane >> 4) + j, j < 4
+            const int g = lane >> 4, c = lane & 15;
+            float av[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) av[j] = (float)As[4 * g + j][wi + c];
+            short4_t ah, am, al;
+            bf16_split<SB>(av, ah, am, al);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                float bv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bv[j] = (float)Bs[4 * g + j][t * 16 + c];
+                short4_t bh, bm, bl;
+                bf16_split<SB>(bv, bh, bm, bl);
+                if (SB == 3) {  // the small products first
+                    acc[t] = mfma_bf16(bm, am, acc[t]);
+                    acc[t] = mfma_bf16(bl, ah, acc[t]);
+                    acc[t] = mfma_bf16(bh, al, acc[t]);
+                }
+                acc[t] = mfma_bf16(bm, ah, acc[t]);
+                acc[t] = mfma_bf16(bh, am, acc[t]);
+                acc[t] = mfma_bf16(bh, ah, acc[t]);
+            }
         }
         __syncthreads();
     }
@@ -1701,18 +1758,18 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32, bool AF, bool CF = false, int SPL = 0, bool BF = false>
+template <bool F32, bool AF, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
 __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
@@ -1720,7 +1777,7 @@ __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, d
 // the second A FP64 (1) or FP32 (2).  The GEMMs are short-K and bound by the loads in
 // flight per CU: FP64 at least 3 waves per SIMD (<= 168 VGPRs; more spills), the
 // all-FP32 mixed-path ones 6 (<= 80 VGPRs: 0.32 -> 0.29 ms for the merged step).
-template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false>
+template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
 __global__ __launch_bounds__(256, (F32 && AF) ? 6 : 3) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
@@ -1731,7 +1788,7 @@ __global__ __launch_bounds__(256, (F32 && AF) ? 6 : 3) void k_gemm_tall(const Ge
     const int i0 = blockIdx.x * GT_BM;
     const int n0 = blockIdx.y * GT_BNW;
     if (n0 >= d.N) return;
-    gemm_dispatch<F32, AF, CF, SPL, BF>(d, rows, alpha, beta, i0, n0, As, Bs);
+    gemm_dispatch<F32, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs);
 }
 
 // Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
@@ -1764,12 +1821,27 @@ __global__ __launch_bounds__(256) void k_gemm_rowshare(const GemmDesc* __restric
 // x32: A and C stored as float, B as double, FP64 MFMA and accumulation (the mixed path's
 // first analysis: its sum over the ring pairs cancels to ~1/4 of its largest terms, which
 // an FP32 accumulation turned into errors up to 2.7e-6 of an (n,l) block at nside 128).
+// sb (with f32, af32 and cf32, or f32 alone): split-BF16 operands in sb terms (2 or 3).
 static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t rows, int32_t maxN, double alpha,
                                double beta, hipStream_t st, bool f32 = false, bool af32 = false, bool cf32 = false,
-                               int spl = 0, bool x32 = false)
+                               int spl = 0, bool x32 = false, int sb = 0)
 {
     dim3 g((unsigned)((rows + GT_BM - 1) / GT_BM), (unsigned)((maxN + GT_BNW - 1) / GT_BNW), nbat);
-    if (x32)
+    if (sb && f32 && af32 && cf32) {
+        if (sb == 3 && spl == 2)
+            hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true, 3>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        else if (sb == 3)
+            hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true, 3>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        else if (spl == 2)
+            hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        else
+            hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    } else if (sb && f32) {
+        if (sb == 3)
+            hipLaunchKernelGGL((k_gemm_tall<true, false, false, 0, false, 3>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        else
+            hipLaunchKernelGGL((k_gemm_tall<true, false, false, 0, false, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    } else if (x32)
         hipLaunchKernelGGL((k_gemm_tall<false, true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
         hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
@@ -3324,7 +3396,7 @@ extern "C" int sfb_plan_create(sfb_plan** out, int64_t nside, int64_t lmax, int6
     if (lmax > 4 * nside)
         return fail(SFB_ERR_LMAX, "alm.lmax > 4*nside is a poor choice (lmax=%lld, nside=%lld)", (long long)lmax,
                     (long long)nside);
-    if (precision != SFB_PREC_FP64 && precision != SFB_PREC_MIXED) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
+    if (precision < SFB_PREC_FP64 || precision > SFB_PREC_BF16X2) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
     int64_t nmaps = 0;
     for (int64_t n = 0; n < nmax; ++n) {
         if (lmax_n[n] < 0 || lmax_n[n] > lmax) return fail(SFB_ERR_ARG, "lmax_n[%lld] out of range", (long long)n);
@@ -3422,7 +3494,7 @@ extern "C" int sfb_plan_sizes(const sfb_plan* P, int64_t* nmaps, int64_t* nlmsiz
 extern "C" int sfb_plan_set_precision(sfb_plan* P, int32_t precision)
 {
     if (!P) return fail(SFB_ERR_ARG, "plan is NULL");
-    if (precision != SFB_PREC_FP64 && precision != SFB_PREC_MIXED) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
+    if (precision < SFB_PREC_FP64 || precision > SFB_PREC_BF16X2) return fail(SFB_ERR_ARG, "unknown precision %d", precision);
     P->precision = precision;
     return SFB_OK;
 }
@@ -3989,6 +4061,8 @@ static int autotune_phi(sfb_plan* P)
 // below kcut[m] are skipped by the descriptors, and k_alias works on m <= mall[k].
 static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = false)
 {
+    // split-BF16 precisions: the Jacobi GEMMs take their FP32 operands as bf16 terms
+    const int sb = P->precision == SFB_PREC_BF16X3 ? 3 : (P->precision == SFB_PREC_BF16X2 ? 2 : 0);
     const int32_t K = P->K, L = (int32_t)P->lmax;
     const int32_t rows = 2 * nq;
     const unsigned nbat = (unsigned)(2 * (L + 1));
@@ -4007,7 +4081,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
         if (Kal > 0) {
             {
                 PhaseTimer timer(P, SFB_PHASE_SYNTHESIS, st);
-                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, f32, f32, f32);
+                launch_gemm(syn, nbat, rows, Kal, 1.0, 0.0, st, f32, f32, f32, 0, false, f32 ? sb : 0);
                 LAUNCH_CHECK();
             }
             {
@@ -4026,7 +4100,7 @@ static int jacobi_stage(sfb_plan* P, int32_t nq, hipStream_t st, bool f32 = fals
         {
             // a_out = b + (I - C0) a_in + dOmega Lam_al R in one GEMM over K = n + K_al
             PhaseTimer timer(P, SFB_PHASE_ANALYSIS, st);
-            launch_gemm(cal, nbat, rows, maxnb, 1.0, 1.0, st, f32, f32, f32, f32 ? 2 : 1);
+            launch_gemm(cal, nbat, rows, maxnb, 1.0, 1.0, st, f32, f32, f32, f32 ? 2 : 1, false, f32 ? sb : 0);
             LAUNCH_CHECK();
         }
         P->a_final = (slot == 1) ? P->d_a2.as<double>() : P->d_a1.as<double>();
@@ -4197,7 +4271,7 @@ static int cat2amln_enqueue(sfb_plan* P, int64_t N, const double* d_rtp, const d
     if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     // mixed path: FP32 spline values and delta maps, FP32 MFMA for the Jacobi passes
     size_t shp4;
-    const bool f32 = P->precision == SFB_PREC_MIXED && polar4_fits(P, sizeof(float), &shp4);
+    const bool f32 = P->precision != SFB_PREC_FP64 && polar4_fits(P, sizeof(float), &shp4);
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st, f32))) return rc;
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st, f32))) return rc;
     {
@@ -4316,7 +4390,7 @@ extern "C" int sfb_cat2amln_vjp_device(sfb_plan* P, int64_t N, const double* d_r
     const size_t shv64 = 32 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
     const size_t shv32 = 16 * (size_t)(P->tbl_cap + 1) + 16 * (size_t)P->tbl_cap;
     const bool grouped = P->sell_ok && P->nvrange > 0;
-    const bool f32 = P->precision == SFB_PREC_MIXED && grouped && (int64_t)shv32 <= P->lds_max;
+    const bool f32 = P->precision != SFB_PREC_FP64 && grouped && (int64_t)shv32 <= P->lds_max;
     PhaseTimer tj(P, SFB_PHASE_ADJOINT, st);
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(P->a_size, 256)), dim3(256), 0, st, P->d_a.as<double>(), P->a_size, 0.0);
     LAUNCH_CHECK();
@@ -5416,22 +5490,29 @@ extern "C" int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const 
 }
 
 static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
-                      double* C, bool f32);
+                      double* C, bool f32, int sb);
 
 extern "C" int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
                             double* C)
 {
-    return gemm_check(device, rows, K, N, A, B, C, false);
+    return gemm_check(device, rows, K, N, A, B, C, false, 0);
 }
 
 extern "C" int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
                             double* C)
 {
-    return gemm_check(device, rows, K, N, A, B, C, true);
+    return gemm_check(device, rows, K, N, A, B, C, true, 0);
+}
+
+extern "C" int sfb_gemm_bf16(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
+                             double* C, int32_t terms)
+{
+    if (terms != 2 && terms != 3) return fail(SFB_ERR_ARG, "split-BF16 terms must be 2 or 3");
+    return gemm_check(device, rows, K, N, A, B, C, true, terms);
 }
 
 static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
-                      double* C, bool f32)
+                      double* C, bool f32, int sb)
 {
     sfb::clear_error();
     if (rows < 1 || K < 1 || N < 1 || rows > (1 << 24) || N > (1 << 24) || !A || !B || !C)
@@ -5456,7 +5537,7 @@ static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const 
     d.ldc = rows;
     if (e == hipSuccess) e = hipMemcpy(dD.p, &d, sizeof d, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, 0.0, 0, f32);
+        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, 0.0, 0, f32, false, false, 0, false, sb);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(C, dC.p, sizeof(double) * (size_t)(N * rows), hipMemcpyDeviceToHost);

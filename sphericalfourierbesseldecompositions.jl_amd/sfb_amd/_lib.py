@@ -22,6 +22,8 @@ SFB_ERR_NOMEM = 6
 
 SFB_PREC_FP64 = 0
 SFB_PREC_MIXED = 1
+SFB_PREC_BF16X3 = 2
+SFB_PREC_BF16X2 = 3
 SFB_WIN_DENSE = 0
 SFB_WIN_SEPARABLE = 1
 
@@ -67,6 +69,7 @@ SIGNATURES = {
     "sfb_map2alm": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_gemm_f64": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
     "sfb_gemm_f32": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
+    "sfb_gemm_bf16": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P, _i32]),
     "sfb_phi_transform": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_ring_kernel_shapes": (ctypes.c_int, [_i64, _i64, _i64, _P]),
     "sfb_plan_profile": (ctypes.c_int, [_P, _i32]),

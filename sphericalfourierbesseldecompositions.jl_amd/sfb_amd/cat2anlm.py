@@ -18,7 +18,8 @@ from . import _lib as L
 from .modes import AnlmModes, getnlmsize
 from .windows import SeparableArray
 
-_PRECISIONS = {"fp64": L.SFB_PREC_FP64, "mixed": L.SFB_PREC_MIXED}
+_PRECISIONS = {"fp64": L.SFB_PREC_FP64, "mixed": L.SFB_PREC_MIXED, "bf16x3": L.SFB_PREC_BF16X3,
+               "bf16x2": L.SFB_PREC_BF16X2}
 
 
 class Plan:

@@ -120,3 +120,19 @@ def test_c3_mixed_every_block(c3):
         eo = block_errors(amx, ref, am, set(range(q0, q1)))
         say(f"C3 mixed vs oracle, maps {q0}..{q1 - 1}: {eo}")
         assert max(eo.values()) <= 1e-6
+
+
+def test_c3_bf16x3_every_block(c3):
+    """Split-BF16 Jacobi GEMMs (hi + mid + lo) at nside 256 under the hemisphere mask, every
+    (n, l) block against the FP64 GPU path, to the mixed path's rule above."""
+    am, win, cat, w, nbar = c3
+    say("C3 bf16x3 vs FP64")
+    a64 = S.get_plan(am).cat2amln(cat, nbar, win, w)
+    abf = S.Plan(am, precision="bf16x3").cat2amln(cat, nbar, win, w)
+    e = block_errors(abf, a64, am)
+    rms = block_rms(a64, am)
+    med = float(np.median(list(rms.values())))
+    say(f"C3 bf16x3 vs FP64 GPU over {len(e)} blocks: worst {sorted(e.items(), key=lambda kv: -kv[1])[:3]}")
+    assert len(e) == am.nmaps
+    assert max(v for k, v in e.items() if rms[k] >= 0.2 * med) <= 1e-6
+    assert max(e.values()) <= 2e-6

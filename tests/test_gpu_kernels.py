@@ -35,6 +35,24 @@ def test_gemm_f32_matches_numpy(rows, K, N):
     assert np.max(np.abs(C - ref)) <= 1e-6 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (70, 37, 23), (130, 256, 71), (300, 80, 200), (64, 1, 1)])
+@pytest.mark.parametrize("terms", [2, 3])
+def test_gemm_bf16_split_matches_numpy(rows, K, N, terms):
+    """Split-BF16 operands: the FP32-rounded operands' product to ~2^-24 (hi + mid + lo) or
+    ~2^-16 (hi + lo) of sum |a||b|; also checks the bf16 MFMA's operand and result layout."""
+    rng = np.random.default_rng(rows * 5 + K + N + terms)
+    A = rng.normal(size=(K, rows))
+    B = rng.normal(size=(K, N))
+    C = np.empty((N, rows))
+    L.check(L.lib().sfb_gemm_bf16(0, rows, K, N, L.ptr(A), L.ptr(B), L.ptr(C), terms))
+    A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    ref = B32.T @ A32
+    scale = np.abs(B32).T @ np.abs(A32)
+    err = np.max(np.abs(C - ref) / scale)
+    print(f"terms {terms}: max |C - ref| / (|B|^T |A|) = {err:.2e}", flush=True)
+    assert err <= (4e-7 if terms == 3 else 4e-5)
+
+
 def test_phi_transform_matches_direct_dft():
     nside, lmax = 8, 16
     am = S.AnlmModes.from_nl(2, lmax, 500.0, 1000.0, nside=nside)
