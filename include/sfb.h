@@ -295,8 +295,10 @@ int sfb_plan_tiles(const sfb_plan* p, int32_t* tiles, double* tune_ms);
 int sfb_ang2pix_ring(int32_t device, int64_t nside, int64_t N, const double* theta, const double* phi,
                      int32_t* pix_out);
 
-/* The FP64 delta map of map q (the (n,l) pair in map order) exactly as the
- * reference forms it before map2alm (src/cat2anlm.jl:284-291), length npix. */
+/* The delta map of map q (the (n,l) pair in map order) as the reference forms it
+ * before map2alm (src/cat2anlm.jl:284-291), length npix: FP64 and bit-exact on an
+ * SFB_PREC_FP64 plan; on the other precisions the FP32 map their scatter writes,
+ * widened to double. */
 int sfb_delta_map(sfb_plan* p, int64_t N, const double* rtp, const double* w, double nbar,
                   int32_t win_kind, const double* win_a, const double* win_b, int64_t q,
                   double* map_out);

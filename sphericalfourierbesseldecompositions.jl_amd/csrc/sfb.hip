@@ -595,7 +595,23 @@ __device__ __forceinline__ float4 lds_rec_get(const float4* sm, int, int i) { re
 template <bool UNIT, typename R> struct RowLoad {
     int i[4];
     R t[4], w[4];
+    double x[4];  // uniform knots: the slots' radii, resolved to (i, t) when summed
 };
+
+// Segment and t of a radius on uniform knots x0 + j H (j = 0 .. nk-1), for record tables
+// whose slot 0 holds the y_first pseudo-record, slots 1 .. nk-1 the segments and slot nk
+// the y_last one: u = (r - x0)/H as fma(r, 1/H, -x0/H), slot = clamp(floor(u), -1, nk-1) + 1,
+// t = u - floor(u) inside the knots and 0 outside (the boundary values, also for r = +-inf
+// and the padding's -inf); a NaN radius keeps t = NaN (the reference's NaN result).
+__device__ __forceinline__ void seg_uniform(double r, double ainv, double c0, int nkm1, double fnkm1, int& slot,
+                                            float& t)
+{
+    const double u = fma(r, ainv, c0);
+    const double fl = floor(u);
+    const int ic = (int)fmin(fmax(fl, -1.0), fnkm1);  // NaN -> -1
+    slot = ic + 1;
+    t = (unsigned)ic < (unsigned)nkm1 ? (float)(u - fl) : (r == r ? 0.0f : (float)r);
+}
 
 // Row and slot indices are 32-bit (the host keeps 8 T below 2^32): the slot arrays are
 // read as raw buffers (a wave-uniform descriptor and a 32-bit per-lane byte offset), which
@@ -619,15 +635,19 @@ template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsr
 // Rows r .. r + 3: one per-lane offset and the row steps as immediate offsets.  Rows past
 // the wave's run read its neighbours' slots (not used); past the last slot the buffer's
 // range check returns zeros.
-template <bool UNIT, typename R>
+template <bool UNIT, typename R, bool ONFLY>
 __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_rsrc_t gi, __amdgpu_buffer_rsrc_t gt,
                                           __amdgpu_buffer_rsrc_t sw, int r, int lane)
 {
     const uint32_t o = (uint32_t)(r * 64 + lane);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        L.i[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(gi, o * 2u, 128 * u, 0);
-        L.t[u] = buf_load<R>(gt, o, u);
+        if (ONFLY) {  // gi: the slot radii
+            L.x[u] = buf_load<double>(gi, o, u);
+        } else {
+            L.i[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(gi, o * 2u, 128 * u, 0);
+            L.t[u] = buf_load<R>(gt, o, u);
+        }
         if (!UNIT) L.w[u] = buf_load<R>(sw, o, u);
     }
 }
@@ -645,7 +665,9 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_r
 // R = double: the reference's arithmetic, bit-exact.  R = float (mixed path): t,
 // w and the records rounded to FP32, FP32 Horner and w g products, FP64 pixel sums
 // (an FP32 running sum cost up to 4e-7 of an (n,l) block at nside 128), FP32 delta.
-template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false>
+// ONFLY (FP32, knots uniform to 1e-9 of their spacing): no (i, t) arrays; each row's
+// radii (it_r) are resolved by seg_uniform against record tables staged one slot up.
+template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false, bool ONFLY = false>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t Tcap,
                                                      const int64_t* __restrict__ Tdev, const double* __restrict__ msel,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
@@ -659,8 +681,10 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                                                      const double* __restrict__ win_a,
                                                      const double* __restrict__ win_b, int64_t lmax,
                                                      R* __restrict__ delta, int32_t slot_order,
-                                                     const float* __restrict__ reclo)
+                                                     const float* __restrict__ reclo,
+                                                     const double* __restrict__ it_r)
 {
+    static_assert(!ONFLY || sizeof(R) == 4, "radii resolved in the scatter: FP32 records only");
     typedef typename Vec4T<R>::T R4;
     // msel (mixed path): the window's mask range (min, max); the split-record kernel runs when
     // the mask varies, the plain FP32 one when it does not (both are launched, one leaves)
@@ -675,24 +699,35 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     double wcol[NM];
     const R4* tab[NM];
     int pad = 0;
+    int nkm1 = 0;  // ONFLY: the group's uniform knots (nk - 1 segments, x0, 1/H)
+    double ainv = 0.0, ux0 = 0.0;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
         has[k] = B.q[k] >= 0;
         // a missing map reads the first map's table and writes nothing
         const int q = has[k] ? B.q[k] : B.q[0];
         const MapInfo m = maps[q];
-        if (k == 0) pad = m.pad;
+        if (k == 0) {
+            pad = m.pad;
+            nkm1 = m.nk - 1;
+            ainv = m.invdx;
+            ux0 = -m.xfirst * m.invdx;
+        }
         if (has[k])
             for (int j = threadIdx.x; j <= m.nk; j += blockDim.x) {
-                lds_rec_put(sm4 + k * tstride, tstride, j, rec4[m.koff + q + j]);
-                if (SPLIT) lo_sm[k * tstride + j] = reclo[m.koff + q + j];
+                // ONFLY: segments one slot up, y_first (record nk - 1) at slot 0
+                const int sj = ONFLY ? (j < m.nk - 1 ? j + 1 : (j == m.nk - 1 ? 0 : j)) : j;
+                lds_rec_put(sm4 + k * tstride, tstride, sj, rec4[m.koff + q + j]);
+                if (SPLIT) lo_sm[k * tstride + sj] = reclo[m.koff + q + j];
             }
         tab[k] = sm4 + (has[k] ? k : 0) * tstride;
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
     }
-    const __amdgpu_buffer_rsrc_t gi = slot_rsrc(it_i + (int64_t)pad * Tcap, (uint32_t)(2 * T));
-    const __amdgpu_buffer_rsrc_t gt = slot_rsrc(it_t + (int64_t)pad * Tcap, (uint32_t)(sizeof(R) * T));
+    const __amdgpu_buffer_rsrc_t gi = ONFLY ? slot_rsrc(it_r, (uint32_t)(8 * T))
+                                            : slot_rsrc(it_i + (int64_t)pad * Tcap, (uint32_t)(2 * T));
+    const __amdgpu_buffer_rsrc_t gt = ONFLY ? gi : slot_rsrc(it_t + (int64_t)pad * Tcap, (uint32_t)(sizeof(R) * T));
+    const double fnkm1 = (double)nkm1;
     const __amdgpu_buffer_rsrc_t gw = slot_rsrc(sw, (uint32_t)(sizeof(R) * T));
     // wave-uniform slice bookkeeping (scalar registers and loads)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -765,9 +800,16 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     if (r > rlast) return;
 
     RowLoad<UNIT, R> cur, nxt;
-    load_rows<UNIT, R>(cur, gi, gt, gw, r, lane);
+    load_rows<UNIT, R, ONFLY>(cur, gi, gt, gw, r, lane);
     for (; r <= rlast; r += 4) {
-        load_rows<UNIT, R>(nxt, gi, gt, gw, r + 4, lane);
+        load_rows<UNIT, R, ONFLY>(nxt, gi, gt, gw, r + 4, lane);
+        if (ONFLY)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float tf;
+                seg_uniform(cur.x[u], ainv, ux0, nkm1, fnkm1, cur.i[u], tf);
+                cur.t[u] = (R)tf;
+            }
         R4 c[4][NM];
         float clo[4][NM];
 #pragma unroll
@@ -2762,6 +2804,9 @@ struct sfb_plan {
     int32_t nvrange = 0;
     // knot groups (maps with bit-identical knots) and the shared segment search
     int32_t ngroup = 0, nk_max_group = 0;
+    // every group's knots uniform to 1e-9 of their spacing: the mixed-path scatter finds
+    // (i, t) from the radii itself (no k_seg_t arrays); SFB_SEG_SEARCH=1 keeps the search
+    bool seg_uniform = false;
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
     int32_t sc_nchunk_it = 0;
     int32_t sc_nm[3] = {1, 1, 1}, sc_per_xcd[3] = {0, 0, 0};  // [0] FP64, [1] FP32, [2] FP32 split records
@@ -2904,6 +2949,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         std::vector<double> gx, grh, rec4;
         std::vector<int32_t> gid((size_t)P->nmaps);
         rec4.reserve((size_t)(knot_off[P->nmaps] + P->nmaps) * 4);
+        const char* seg_env = std::getenv("SFB_SEG_SEARCH");
+        P->seg_uniform = !(seg_env && seg_env[0] == '1');
         for (int64_t qq = 0; qq < P->nmaps; ++qq) {
             const int64_t o = knot_off[qq], nk = knot_off[qq + 1] - knot_off[qq];
             int32_t g = -1;
@@ -2920,10 +2967,15 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
                 G.xfirst = kx[o];
                 G.xlast = kx[o + nk - 1];
                 G.invdx = (double)(nk - 1) / (kx[o + nk - 1] - kx[o]);
+                const double H = (kx[o + nk - 1] - kx[o]) / (double)(nk - 1);
                 for (int64_t i = 0; i < nk; ++i) {
                     gx.push_back(kx[o + i]);
                     grh.push_back(i + 1 < nk ? 1.0 / (kx[o + i + 1] - kx[o + i]) : 0.0);
+                    // uniform knots: x_i within 1e-9 H of x_0 + i H (a segment chosen by the
+                    // uniform grid then evaluates at most 1e-9 outside its interval)
+                    if (!(std::fabs(kx[o + i] - (kx[o] + (double)i * H)) <= 1e-9 * H)) P->seg_uniform = false;
                 }
+                if (nk < 2 || nk > 65535) P->seg_uniform = false;
                 g = (int32_t)groups.size();
                 groups.push_back(G);
             }
@@ -3638,10 +3690,16 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
                            P->d_soff.as<int64_t>(), sw,
                            P->d_it_i.as<uint16_t>(), P->d_it_t.as<R>(), P->d_maps.as<MapInfo>(), rec, inv, d_wa,
                            d_wb, P->lmax, P->d_delta.as<R>(), (int32_t)(P->delta_perm != nullptr),
-                           P->d_reclo.as<float>());
+                           P->d_reclo.as<float>(), P->d_sr.as<double>());
         LAUNCH_CHECK();
         return SFB_OK;
     };
+    constexpr bool F = sizeof(R) == 4;
+    if (F && P->seg_uniform) {
+        if (dense)
+            return unit ? go(k_scatter_it<true, NM, true, R, SPLIT, F>) : go(k_scatter_it<true, NM, false, R, SPLIT, F>);
+        return unit ? go(k_scatter_it<false, NM, true, R, SPLIT, F>) : go(k_scatter_it<false, NM, false, R, SPLIT, F>);
+    }
     if (dense)
         return unit ? go(k_scatter_it<true, NM, true, R, SPLIT>) : go(k_scatter_it<true, NM, false, R, SPLIT>);
     return unit ? go(k_scatter_it<false, NM, true, R, SPLIT>) : go(k_scatter_it<false, NM, false, R, SPLIT>);
@@ -3738,8 +3796,10 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         const int64_t* Tdev = P->d_soff.as<int64_t>() + P->sell_nslice;
         int rc;
         PhaseTimer tseg(P, SFB_PHASE_SEGMENT, st);
-        if ((rc = P->d_it_i.ensure(sizeof(uint16_t) * (size_t)(T * P->ngroup))) ||
-            (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup))))
+        // mixed path on uniform knots: the scatter resolves each slot's radius itself
+        const bool onfly = f32 && P->seg_uniform;
+        if (!onfly && ((rc = P->d_it_i.ensure(sizeof(uint16_t) * (size_t)(T * P->ngroup))) ||
+                       (rc = P->d_it_t.ensure(sizeof(double) * (size_t)(T * P->ngroup)))))
             return rc;
         // 64 slot ranges per group; LDS holds the group's knots and 1/h
         const dim3 gseg(64, (unsigned)P->ngroup);
@@ -3755,9 +3815,10 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
                                    P->d_swf.as<float>());
                 LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(1024), shm_seg, st, T, Tdev, P->d_sr.as<double>(),
-                               P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
-                               P->d_it_i.as<uint16_t>(), P->d_it_t.as<float>());
+            if (!onfly)
+                hipLaunchKernelGGL(k_seg_t<float>, gseg, dim3(1024), shm_seg, st, T, Tdev, P->d_sr.as<double>(),
+                                   P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
+                                   P->d_it_i.as<uint16_t>(), P->d_it_t.as<float>());
         } else {
             hipLaunchKernelGGL(k_seg_t<double>, gseg, dim3(1024), shm_seg, st, T, Tdev, P->d_sr.as<double>(),
                                P->d_groups.as<GroupInfo>(), P->d_gx.as<double>(), P->d_grh.as<double>(),
@@ -5414,9 +5475,10 @@ extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const do
     HIP_TRY(hipSetDevice(P->device));
     const double *d_rtp, *d_w, *d_wa, *d_wb;
     if ((rc = stage_inputs(P, N, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
+    const bool f32 = P->precision != SFB_PREC_FP64;
     for (int pass = 0; pass < 2; ++pass) {
         if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream))) return rc;
-        if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream))) return rc;
+        if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream, f32))) return rc;
         int32_t over = 0;
         HIP_TRY(hipMemcpyAsync(&over, P->d_flag.as<int32_t>() + 2, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
         HIP_TRY(hipStreamSynchronize(P->stream));
@@ -5424,8 +5486,16 @@ extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const do
         if (pass == 1) return fail(SFB_ERR_ARG, "internal: slot overflow after resizing");
         if ((rc = take_slot_total(P))) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(map_out, P->d_delta.as<double>() + q * P->npix, sizeof(double) * (size_t)P->npix,
-                           hipMemcpyDeviceToHost, P->stream));
+    if (f32) {
+        std::vector<float> m32((size_t)P->npix);
+        HIP_TRY(hipMemcpyAsync(m32.data(), P->d_delta.as<float>() + q * P->npix, sizeof(float) * m32.size(),
+                               hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (int64_t p = 0; p < P->npix; ++p) map_out[p] = (double)m32[(size_t)p];
+    } else {
+        HIP_TRY(hipMemcpyAsync(map_out, P->d_delta.as<double>() + q * P->npix, sizeof(double) * (size_t)P->npix,
+                               hipMemcpyDeviceToHost, P->stream));
+    }
     if (P->delta_perm) {  // slot order: pixel p sits at slot perm[p]
         std::vector<uint32_t> perm((size_t)P->npix);
         HIP_TRY(hipMemcpyAsync(perm.data(), P->delta_perm, sizeof(uint32_t) * perm.size(), hipMemcpyDeviceToHost,

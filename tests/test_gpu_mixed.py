@@ -161,3 +161,42 @@ def test_mixed_c1_masked_dense_window():
     e = block_errors(a, ref, am)
     say("C1 mixed, masked dense window, worst blocks", worst(e))
     assert max(e.values()) <= BLOCK_BAR
+
+
+def test_mixed_delta_maps_uniform_knots_against_oracle(monkeypatch):
+    """The mixed scatter finds each galaxy's spline segment and t on the plan's uniform knots
+    itself (no per-group search arrays).  Its FP32 delta maps against the oracle's FP64 maps
+    and against the same path with the per-group segment search (SFB_SEG_SEARCH=1), with
+    radii on every knot of two maps, exactly at rmin and rmax, outside the knots on both
+    sides and at +-inf (the boundary values), plus given weights: per map,
+    max |delta - ref| <= 2e-6 max |ref|.  A NaN radius still gives the non-finite error."""
+    am = S.AnlmModes(0.05, 500.0, 1000.0, nside=32)
+    win, _ = separable_unit_window(S, am)
+    rng = np.random.default_rng(77)
+    qa, qb = am.nmaps // 3, am.nmaps - 1
+    radii = [am.knot_x[am.knot_off[q]:am.knot_off[q + 1]] for q in (qa, qb)]
+    edge = np.concatenate(radii + [np.array([500.0, 1000.0, 499.999, 1000.001, 100.0, 3000.0, np.inf, -np.inf])])
+    edge = np.repeat(edge, 3)
+    parts = [np.stack([edge, np.arccos(rng.uniform(-1, 1, edge.size)), rng.uniform(0, 2 * np.pi, edge.size)]),
+             uniform_shell_catalog(60_000, 500.0, 1000.0, seed=78)]
+    cat = np.concatenate(parts, axis=1)
+    cat = cat[:, rng.permutation(cat.shape[1])]
+    nbar = cat.shape[1] / shell_volume(500.0, 1000.0)
+    om = oracle_view(am)
+    onfly = S.Plan(am, precision="mixed")
+    monkeypatch.setenv("SFB_SEG_SEARCH", "1")
+    search = S.Plan(am, precision="mixed")
+    monkeypatch.delenv("SFB_SEG_SEARCH")
+    for w in (None, rng.uniform(0.5, 1.5, cat.shape[1])):
+        for q in (0, qa, qb):
+            ref = O.delta_map(om, cat, nbar, 1, win.mask, win.w_ln.reshape(-1, order="F"), w, q)
+            g = onfly.delta_map(cat, nbar, win, w, q)
+            s = search.delta_map(cat, nbar, win, w, q)
+            scale = np.max(np.abs(ref))
+            e_ref, e_s = np.max(np.abs(g - ref)) / scale, np.max(np.abs(g - s)) / scale
+            say(f"map {q} {'weights' if w is not None else 'unit'}: vs oracle {e_ref:.2e}, vs search {e_s:.2e}")
+            assert e_ref <= 2e-6 and e_s <= 2e-6
+    bad = cat.copy()
+    bad[0, 11] = np.nan
+    with pytest.raises(AssertionError):
+        onfly.cat2amln(bad, nbar, win)
