@@ -601,16 +601,15 @@ template <bool UNIT, typename R> struct RowLoad {
 // Segment and t of a radius on uniform knots x0 + j H (j = 0 .. nk-1), for record tables
 // whose slot 0 holds the y_first pseudo-record, slots 1 .. nk-1 the segments and slot nk
 // the y_last one: u = (r - x0)/H as fma(r, 1/H, -x0/H), slot = clamp(floor(u), -1, nk-1) + 1,
-// t = u - floor(u) inside the knots and 0 outside (the boundary values, also for r = +-inf
-// and the padding's -inf); a NaN radius keeps t = NaN (the reference's NaN result).
-__device__ __forceinline__ void seg_uniform(double r, double ainv, double c0, int nkm1, double fnkm1, int& slot,
-                                            float& t)
+// t = u - floor(u).  The pseudo-records (y, 0, 0, 0) return y for every finite t, so only
+// u = +-inf (r = +-inf, the padding's -inf) needs t = 0; a NaN radius keeps t = NaN (the
+// reference's NaN result) and reads slot 0.  Branch-free: a v_cmp_class and a select.
+__device__ __forceinline__ void seg_uniform(double r, double ainv, double c0, double fnkm1, int& slot, float& t)
 {
     const double u = fma(r, ainv, c0);
     const double fl = floor(u);
-    const int ic = (int)fmin(fmax(fl, -1.0), fnkm1);  // NaN -> -1
-    slot = ic + 1;
-    t = (unsigned)ic < (unsigned)nkm1 ? (float)(u - fl) : (r == r ? 0.0f : (float)r);
+    slot = (int)fmin(fmax(fl, -1.0), fnkm1) + 1;  // NaN -> slot 0
+    t = __builtin_isinf(u) ? 0.0f : (float)(u - fl);
 }
 
 // Row and slot indices are 32-bit (the host keeps 8 T below 2^32): the slot arrays are
@@ -807,7 +806,7 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 float tf;
-                seg_uniform(cur.x[u], ainv, ux0, nkm1, fnkm1, cur.i[u], tf);
+                seg_uniform(cur.x[u], ainv, ux0, fnkm1, cur.i[u], tf);
                 cur.t[u] = (R)tf;
             }
         R4 c[4][NM];
@@ -3554,8 +3553,11 @@ extern "C" int sfb_plan_set_precision(sfb_plan* P, int32_t precision)
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // Sort the catalogue and build the pixel segments.  Leaves r_s, w_s, pstart and,
-// when the records fit in LDS, the sliced layout sr, sw (soff, perm).
-static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, hipStream_t st)
+// when the records fit in LDS, the sliced layout sr, sw (soff, perm).  sort_r: galaxies
+// of a pixel in the reference's order (by r, then input order: the FP64 path's bit-exact
+// sums); without it in input order (one sort fewer).
+static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const double* d_w, hipStream_t st,
+                         bool sort_r = true)
 {
     int rc;
     if (N > 0x7fffffff) return fail(SFB_ERR_ARG, "N > 2^31-1 galaxies");
@@ -3582,7 +3584,7 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, N, d_rtp, P->nside, P->d_keyr.as<uint64_t>(),
                        P->d_idx.as<uint32_t>(), P->d_pix.as<uint32_t>(), P->d_flag.as<int32_t>());
     LAUNCH_CHECK();
-    // 1) stable sort by r
+    // 1) stable sort by r (sort_r), 2) stable sort by pixel: temporary sizes
     size_t tmp1 = 0, tmp2 = 0;
     HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
                                       P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
@@ -3613,18 +3615,29 @@ static int catalog_stage(sfb_plan* P, int64_t N, const double* d_rtp, const doub
     // every temporary size is known before the first sort is enqueued: the buffer
     // is never reallocated under work in flight
     if ((rc = P->d_sorttmp.ensure(std::max(std::max(tmp1, tmp2), tmp4)))) return rc;
-    tmp1 = P->d_sorttmp.bytes;
-    HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
-                                      P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
-    hipLaunchKernelGGL(k_gather_pix, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx2.as<uint32_t>(),
-                       P->d_pix.as<uint32_t>(), P->d_pixkey.as<uint32_t>());
-    LAUNCH_CHECK();
-    // 2) stable sort by pixel: (pixel, r, original index) order
-    tmp2 = P->d_sorttmp.bytes;
-    HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp2, P->d_pixkey.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
-                                      P->d_idx2.as<uint32_t>(), P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
-    hipLaunchKernelGGL(k_gather, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx.as<uint32_t>(), d_rtp, d_w,
-                       P->d_rs.as<double>(), P->d_ws.as<double>());
+    if (sort_r) {
+        tmp1 = P->d_sorttmp.bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp1, P->d_keyr.as<uint64_t>(), P->d_keyr2.as<uint64_t>(),
+                                          P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, 64, st));
+        hipLaunchKernelGGL(k_gather_pix, dim3(nblk(N, 256)), dim3(256), 0, st, N, P->d_idx2.as<uint32_t>(),
+                           P->d_pix.as<uint32_t>(), P->d_pixkey.as<uint32_t>());
+        LAUNCH_CHECK();
+        // 2) stable sort by pixel: (pixel, r, original index) order
+        tmp2 = P->d_sorttmp.bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp2, P->d_pixkey.as<uint32_t>(),
+                                          P->d_pixkey2.as<uint32_t>(), P->d_idx2.as<uint32_t>(),
+                                          P->d_idx.as<uint32_t>(), (unsigned)N, 0, pbits, st));
+    } else {
+        // mixed path: one stable sort by pixel, (pixel, original index) order; its FP64 pixel
+        // sums of FP32 values do not need the reference's order inside a pixel
+        tmp2 = P->d_sorttmp.bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(P->d_sorttmp.p, tmp2, P->d_pix.as<uint32_t>(), P->d_pixkey2.as<uint32_t>(),
+                                          P->d_idx.as<uint32_t>(), P->d_idx2.as<uint32_t>(), (unsigned)N, 0, pbits,
+                                          st));
+    }
+    const uint32_t* sidx = (sort_r ? P->d_idx : P->d_idx2).as<uint32_t>();
+    hipLaunchKernelGGL(k_gather, dim3(nblk(N, 256)), dim3(256), 0, st, N, sidx, d_rtp, d_w, P->d_rs.as<double>(),
+                       P->d_ws.as<double>());
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pixel_start, dim3(nblk(P->npix + 1, 256)), dim3(256), 0, st, P->npix, N,
                        P->d_pixkey2.as<uint32_t>(), P->d_pstart.as<int32_t>());
@@ -4329,10 +4342,10 @@ static int cat2amln_enqueue(sfb_plan* P, int64_t N, const double* d_rtp, const d
                             hipStream_t st)
 {
     int rc;
-    if ((rc = catalog_stage(P, N, d_rtp, d_w, st))) return rc;
     // mixed path: FP32 spline values and delta maps, FP32 MFMA for the Jacobi passes
     size_t shp4;
     const bool f32 = P->precision != SFB_PREC_FP64 && polar4_fits(P, sizeof(float), &shp4);
+    if ((rc = catalog_stage(P, N, d_rtp, d_w, st, !f32))) return rc;
     if ((rc = scatter_stage(P, nbar, win_kind, d_win_a, d_win_b, st, f32))) return rc;
     if ((rc = sht_stage(P, (int32_t)P->nmaps, st, f32))) return rc;
     {
@@ -5477,7 +5490,7 @@ extern "C" int sfb_delta_map(sfb_plan* P, int64_t N, const double* rtp, const do
     if ((rc = stage_inputs(P, N, rtp, w, win_kind, win_a, win_b, &d_rtp, &d_w, &d_wa, &d_wb))) return rc;
     const bool f32 = P->precision != SFB_PREC_FP64;
     for (int pass = 0; pass < 2; ++pass) {
-        if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream))) return rc;
+        if ((rc = catalog_stage(P, N, d_rtp, d_w, P->stream, !f32))) return rc;
         if ((rc = scatter_stage(P, nbar, win_kind, d_wa, d_wb, P->stream, f32))) return rc;
         int32_t over = 0;
         HIP_TRY(hipMemcpyAsync(&over, P->d_flag.as<int32_t>() + 2, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
