@@ -38,11 +38,13 @@ def _run(cmd):
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD_DIR, exist_ok=True)
     setup_o = os.path.join(BUILD_DIR, "setup.o")
+    copy_o = os.path.join(BUILD_DIR, "hostcopy.o")
     sfb_o = os.path.join(BUILD_DIR, "sfb.o")
     _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "setup.cpp"), "-o", setup_o])
+    _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "hostcopy.cpp"), "-o", copy_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "sfb.hip"), "-o", sfb_o])
     tmp = os.path.join(OUT_DIR, LIB_NAME + ".tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, sfb_o, setup_o, "-pthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, sfb_o, setup_o, copy_o, "-pthread"])
     final = os.path.join(OUT_DIR, LIB_NAME)
     os.replace(tmp, final)
     return final

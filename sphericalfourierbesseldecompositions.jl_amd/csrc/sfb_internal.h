@@ -2,7 +2,9 @@
 // library (sfb.hip).  Not part of the public ABI.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -42,6 +44,20 @@ double spline_deriv(int64_t n, const double* x, const double* y, const double* y
 // root finder no evaluations (the reference's doublesecant_method returns NaN after
 // maxevals, src/SphericalBesselGNLs.jl:126-127).  Read at each use; unset, nothing changes.
 bool fault(const char* what);
+
+// Host copies through a process-wide thread pool (hostcopy.cpp).  copy_chunks copies the
+// chunks in order, every pool thread taking a piece of each, and calls ready(c) on the
+// calling thread once chunk c is complete (the pool goes on with the next chunks
+// meanwhile); it returns after the last ready().  copy_threads() is the pool size
+// (SFB_COPY_THREADS, default min(8, hardware threads / 2); 0: no pool, and the host-array
+// entry points stage through pageable copies).
+struct CopyChunk {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+int copy_threads();
+void copy_chunks(const CopyChunk* ch, int n, const std::function<void(int)>& ready);
 
 }  // namespace sfb
 
