@@ -138,8 +138,10 @@ def algorithmic_work(am, N, f32=False, ngroup=None):
     if ngroup is None:
         ngroup = len({am.knot_x[am.knot_off[q]:am.knot_off[q + 1]].tobytes() for q in range(am.nmaps)})
     return {
-        # delta maps written once; per galaxy its pixel slot, segment and t read once
-        "scatter": {"bytes": delta_bytes + N * (vb + 2 + vb) + (npix + 1) * 4 + npix * 8,
+        # delta maps written once; per pixel its slice entry and window factor read once; per
+        # galaxy its radius read once (mixed path: segment and t found from it on the uniform
+        # knots) or its 16-bit segment and t (FP64 path)
+        "scatter": {"bytes": delta_bytes + N * (8 if f32 else 2 + vb) + (npix + 1) * 4 + npix * 8,
                     "flops": 10.0 * N * nmaps},
         # per knot group and galaxy: r read, (16-bit segment, t) written
         "segment": {"bytes": ngroup * N * (8 + 2 + vb), "flops": 0.0},
