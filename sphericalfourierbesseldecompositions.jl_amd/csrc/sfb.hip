@@ -1817,9 +1817,13 @@ __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, d
 // CF: C stored as float (the mixed path's aliasing-ring synthesis); SPL: split K,
 // the second A FP64 (1) or FP32 (2).  The GEMMs are short-K and bound by the loads in
 // flight per CU: FP64 at least 3 waves per SIMD (<= 168 VGPRs; more spills), the
-// all-FP32 mixed-path ones 6 (<= 80 VGPRs: 0.32 -> 0.29 ms for the merged step).
+// all-FP32 mixed-path ones 6 (<= 80 VGPRs: 0.32 -> 0.29 ms for the merged step), the
+// split-BF16 ones 4 (their operand splits spilled 56-88 bytes per lane under 80 VGPRs:
+// C2 bf16x3 6.70 -> 6.42 ms per call).  A second register set of prefetched k-tiles
+// (two tiles in flight) cost a wave per SIMD and ran slower (the four analysis GEMMs of
+// a C2 call: FP64 2.0 -> 2.5 ms, mixed 1.4 -> 1.9 ms).
 template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
-__global__ __launch_bounds__(256, (F32 && AF) ? 6 : 3) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+__global__ __launch_bounds__(256, SB ? 4 : ((F32 && AF) ? 6 : 3)) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
     typedef typename MfmaT<F32>::T T;
