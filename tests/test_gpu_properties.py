@@ -107,6 +107,73 @@ def test_device_entry_matches_host_entry(c2):
             hip.hipFree(p)
 
 
+
+def test_host_staging_edges_match_device_entry():
+    """The host-array entry point stages the caller's arrays through pinned memory in 4 MiB
+    chunks (each chunk's transfer enqueued as soon as a thread pool has copied it); the result
+    must be the device entry point's bit for bit at the sizes around the chunking: no galaxy,
+    one, a catalogue of 4 MiB + 24 bytes (a second chunk of one galaxy), unit and given weights,
+    separable and dense windows, FP64 and mixed."""
+    import ctypes
+    from sfb_amd import _lib as L
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    H2D, D2H = 1, 2
+    am = S.AnlmModes(0.03, 500.0, 1000.0, nside=16)
+    win, _ = separable_unit_window(S, am, nr=100)
+    dense = win.dense()
+    plan = S.Plan(am)
+    nout = 2 * S.getnlmsize(am)
+    ptrs = []
+
+    def dmalloc(nbytes):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(nbytes, 8))) == 0
+        ptrs.append(p)
+        return p.value
+
+    def copy(dst, src, nbytes, kind):
+        if nbytes:
+            assert hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes), kind) == 0
+
+    try:
+        d_out = dmalloc(8 * nout)
+        for N in (0, 1, (4 << 20) // 24 + 1):
+            cat = uniform_shell_catalog(N, 500.0, 1000.0, seed=N + 3) if N else np.zeros((3, 0))
+            nbar = max(N, 1) / shell_volume(500.0, 1000.0)
+            w = np.random.default_rng(N).uniform(0.5, 1.5, N)
+            rtp = np.ascontiguousarray(cat.T)
+            d_rtp, d_w = dmalloc(rtp.nbytes), dmalloc(w.nbytes)
+            copy(d_rtp, rtp.ctypes.data, rtp.nbytes, H2D)
+            copy(d_w, w.ctypes.data, w.nbytes, H2D)
+            for kind, wwin in (("separable", win), ("dense", dense)):
+                if kind == "separable":
+                    wa = np.ascontiguousarray(win.mask, dtype=np.float64)
+                    wb = np.ascontiguousarray(win.w_ln.reshape(-1, order="F"))
+                    code = L.SFB_WIN_SEPARABLE
+                else:
+                    wa, wb, code = np.ascontiguousarray(dense.reshape(-1, order="F")), None, L.SFB_WIN_DENSE
+                d_wa = dmalloc(wa.nbytes)
+                copy(d_wa, wa.ctypes.data, wa.nbytes, H2D)
+                d_wb = None
+                if wb is not None:
+                    d_wb = dmalloc(wb.nbytes)
+                    copy(d_wb, wb.ctypes.data, wb.nbytes, H2D)
+                for prec in ("fp64", "mixed"):
+                    plan.set_precision(prec)
+                    for ww, dw in ((None, None), (w, d_w)):
+                        say("host staging", N, kind, prec, "unit" if ww is None else "weights")
+                        host = plan.cat2amln(cat, nbar, wwin, ww)
+                        out = np.full(nout, np.nan)
+                        copy(d_out, out.ctypes.data, out.nbytes, H2D)
+                        plan.cat2amln_device(N, d_rtp if N else None, dw if N else None, nbar, code, d_wa, d_wb,
+                                             d_out)
+                        copy(out.ctypes.data, d_out, out.nbytes, D2H)
+                        assert np.array_equal(out[0::2] + 1j * out[1::2], host), (N, kind, prec, ww is None)
+    finally:
+        plan.close()
+        for p in ptrs:
+            hip.hipFree(p)
+
 def test_c2_sample_against_oracle(c2):
     am, win, cat, nbar, plan = c2
     say("C2 oracle sample")
