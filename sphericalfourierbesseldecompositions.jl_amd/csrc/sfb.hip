@@ -1040,23 +1040,32 @@ __global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, i
     }
     if (perm) {
         // slot-order maps: per (side, pixel pair) the two slots are read once and the pair
-        // gathered for each of the block's TQ maps
+        // gathered for each of the block's TQ maps, every load issued before the first LDS
+        // store (loads at clamped addresses, values dropped after): one memory round trip per
+        // pixel pair instead of one per map (a load/store per map ran the mixed FFT's load
+        // phase at ~0.45 ms of its 1.12 ms at nside 128)
+        constexpr int TQM = 8;  // the largest TQ the tile search tries
         for (int e = threadIdx.x; e < 2 * M; e += blockDim.x) {
             const int side = e >> logM, jj = e & (M - 1);  // M = 2^logM
             const int rj = (int)(__brev((unsigned)jj) >> (32 - logM));
             uint2 sl = make_uint2(0u, 0u);
             const bool live = !(side && eq);  // sample pairs start at even pixels
             if (live) sl = *(const uint2*)(perm + (side ? Rg.start_s : Rg.start_n) + 2 * jj);
-            for (int ql = 0; ql < TQ; ++ql) {
-                const int q = q0 + ql;
-                R re = 0, im = 0;
-                if (live && q < nq) {
-                    re = (R)delta[(int64_t)q * npix + sl.x];
-                    im = (R)delta[(int64_t)q * npix + sl.y];
+            TS re[TQM], im[TQM];
+#pragma unroll
+            for (int ql = 0; ql < TQM; ++ql)
+                if (ql < TQ) {
+                    const int64_t qo = (int64_t)min(q0 + ql, nq - 1) * npix;
+                    re[ql] = delta[qo + sl.x];
+                    im[ql] = delta[qo + sl.y];
                 }
-                zs[(2 * ql + side) * Mp + fpad<R>(rj)].x = re;
-                zs[(2 * ql + side) * Mp + fpad<R>(rj)].y = im;
-            }
+#pragma unroll
+            for (int ql = 0; ql < TQM; ++ql)
+                if (ql < TQ) {
+                    const bool ok = live && q0 + ql < nq;
+                    zs[(2 * ql + side) * Mp + fpad<R>(rj)].x = ok ? (R)re[ql] : (R)0;
+                    zs[(2 * ql + side) * Mp + fpad<R>(rj)].y = ok ? (R)im[ql] : (R)0;
+                }
         }
     } else {
         for (int e = threadIdx.x; e < nf * M; e += blockDim.x) {
@@ -1531,7 +1540,9 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
     const int nv = npr >> 2, nvt = 2 * TQ * nv;
     if (perm) {
         // slot-order maps: per (side, four pixels) the slots are read once and the four
-        // samples gathered for each of the block's TQ maps
+        // samples gathered for each of the block's TQ maps (issuing four maps' loads before
+        // their stores, as k_phi_fft does, changed nothing here: 0.85 -> 0.87 ms mixed at
+        // nside 128; this kernel is VALU-bound)
         for (int e = threadIdx.x; e < 2 * nv; e += blockDim.x) {
             const int side = e / nv, j4 = e - side * nv;
             const uint4 sl = *(const uint4*)(perm + (side ? Rg.start_s : Rg.start_n) + 4 * j4);
