@@ -666,7 +666,9 @@ __device__ __forceinline__ void load_rows(RowLoad<UNIT, R>& L, __amdgpu_buffer_r
 // (an FP32 running sum cost up to 4e-7 of an (n,l) block at nside 128), FP32 delta.
 // ONFLY (FP32, knots uniform to 1e-9 of their spacing): no (i, t) arrays; each row's
 // radii (it_r) are resolved by seg_uniform against record tables staged one slot up.
-template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false, bool ONFLY = false>
+// TS > 0: the tables' stride in records as a constant (tstride <= TS), so a row's gathers
+// from its NM tables share one address register and carry the table offsets as immediates.
+template <bool DENSE, int NM, bool UNIT, typename R, bool SPLIT = false, bool ONFLY = false, int TS = 0>
 __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslice, int32_t spc, int64_t Tcap,
                                                      const int64_t* __restrict__ Tdev, const double* __restrict__ msel,
                                                      const ScatterBlock* __restrict__ blk, int32_t per_xcd,
@@ -684,7 +686,9 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                                                      const double* __restrict__ it_r)
 {
     static_assert(!ONFLY || sizeof(R) == 4, "radii resolved in the scatter: FP32 records only");
+    static_assert(TS == 0 || (sizeof(R) == 4 && !SPLIT), "constant stride: FP32 records only");
     typedef typename Vec4T<R>::T R4;
+    if (TS) tstride = TS;
     // msel (mixed path): the window's mask range (min, max); the split-record kernel runs when
     // the mask varies, the plain FP32 one when it does not (both are launched, one leaves)
     if (msel && SPLIT != (msel[0] != msel[1])) return;
@@ -719,7 +723,8 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
                 lds_rec_put(sm4 + k * tstride, tstride, sj, rec4[m.koff + q + j]);
                 if (SPLIT) lo_sm[k * tstride + sj] = reclo[m.koff + q + j];
             }
-        tab[k] = sm4 + (has[k] ? k : 0) * tstride;
+        // TS: a missing map reads its own (unstaged, in-bounds) table and writes nothing
+        tab[k] = sm4 + (TS ? k : (has[k] ? k : 0)) * tstride;
         col[k] = m.l + (lmax + 1) * (int64_t)(m.n - 1);
         wcol[k] = DENSE ? 0.0 : win_b[col[k]];
     }
@@ -801,23 +806,55 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
     RowLoad<UNIT, R> cur, nxt;
     load_rows<UNIT, R, ONFLY>(cur, gi, gt, gw, r, lane);
     for (; r <= rlast; r += 4) {
-        load_rows<UNIT, R, ONFLY>(nxt, gi, gt, gw, r + 4, lane);
-        if (ONFLY)
+        if (ONFLY) {
+            // the radii resolve to (slot, t), then the next rows' radii load into the same
+            // registers (their weights, if any, into nxt)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 float tf;
                 seg_uniform(cur.x[u], ainv, ux0, fnkm1, cur.i[u], tf);
                 cur.t[u] = (R)tf;
             }
+            const uint32_t o = (uint32_t)((r + 4) * 64 + lane);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cur.x[u] = buf_load<double>(gi, o, u);
+                if (!UNIT) nxt.w[u] = buf_load<R>(gw, o, u);
+            }
+        } else {
+            load_rows<UNIT, R, ONFLY>(nxt, gi, gt, gw, r + 4, lane);
+        }
         R4 c[4][NM];
         float clo[4][NM];
+        if constexpr (TS > 0) {
+            // one LDS byte address per row for tables 0 and 1 (table 1 at the immediate offset
+            // 16 TS), a second for tables 2 and 3
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            typedef __attribute__((address_space(3))) const f4v LR4;
+            const uint32_t sbase = (uint32_t)(uintptr_t)(LR4*)sm4;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t a = sbase + 16u * (uint32_t)cur.i[u];
+                LR4* pa = (LR4*)(uintptr_t)a;
+                LR4* pb = (LR4*)(uintptr_t)(a + 32u * (uint32_t)TS);
 #pragma unroll
-            for (int k = 0; k < NM; ++k) {
-                c[u][k] = lds_rec_get(tab[k], tstride, cur.i[u]);  // (d, c, b, a)
-                if (SPLIT) clo[u][k] = lo_sm[(has[k] ? k : 0) * tstride + cur.i[u]];
+                for (int k = 0; k < NM; ++k) {
+                    const f4v v = k < 2 ? pa[k * TS] : pb[(k - 2) * TS];
+                    c[u][k].x = v.x;
+                    c[u][k].y = v.y;
+                    c[u][k].z = v.z;
+                    c[u][k].w = v.w;
+                }
             }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < NM; ++k) {
+                    c[u][k] = lds_rec_get(tab[k], tstride, cur.i[u]);  // (d, c, b, a)
+                    if (SPLIT) clo[u][k] = lo_sm[(has[k] ? k : 0) * tstride + cur.i[u]];
+                }
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (r + u > rlast) break;
@@ -851,7 +888,12 @@ __global__ __launch_bounds__(1024) void k_scatter_it(int64_t npix, int32_t nslic
             }
             close_slices(r + u + 1);
         }
-        cur = nxt;
+        if (!ONFLY) {
+            cur = nxt;
+        } else if (!UNIT) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cur.w[u] = nxt.w[u];
+        }
     }
 }
 
@@ -2848,6 +2890,9 @@ struct sfb_plan {
     // every group's knots uniform to 1e-9 of their spacing: the mixed-path scatter finds
     // (i, t) from the radii itself (no k_seg_t arrays); SFB_SEG_SEARCH=1 keeps the search
     bool seg_uniform = false;
+    // the uniform-knot FP32 scatter with its table stride as a constant (SFB_SCATTER_STRIDE=0:
+    // the runtime stride; same sums)
+    bool scatter_cstride = true;
     DevBuf d_groups, d_gx, d_grh, d_rec4, d_qorder, d_it_i, d_it_t, d_sblk, d_rec4f, d_swf, d_sblk32;
     int32_t sc_nchunk_it = 0;
     int32_t sc_nm[3] = {1, 1, 1}, sc_per_xcd[3] = {0, 0, 0};  // [0] FP64, [1] FP32, [2] FP32 split records
@@ -2992,6 +3037,8 @@ static int plan_tables(sfb_plan* P, const int64_t* lmax_n, const int64_t* knot_o
         rec4.reserve((size_t)(knot_off[P->nmaps] + P->nmaps) * 4);
         const char* seg_env = std::getenv("SFB_SEG_SEARCH");
         P->seg_uniform = !(seg_env && seg_env[0] == '1');
+        const char* cs_env = std::getenv("SFB_SCATTER_STRIDE");
+        P->scatter_cstride = !(cs_env && cs_env[0] == '0');
         for (int64_t qq = 0; qq < P->nmaps; ++qq) {
             const int64_t o = knot_off[qq], nk = knot_off[qq + 1] - knot_off[qq];
             int32_t g = -1;
@@ -3752,6 +3799,18 @@ static int launch_scatter_nm(sfb_plan* P, bool dense, bool unit, dim3 grid, size
         return SFB_OK;
     };
     constexpr bool F = sizeof(R) == 4;
+    constexpr int TSC = 2560;  // constant table stride: four FP32 tables fill 160 KiB of LDS
+    if constexpr (F && NM == 4 && !SPLIT) {
+        const size_t shm_c = (size_t)TSC * sizeof(typename Vec4T<R>::T) * NM;
+        if (P->seg_uniform && tstride <= TSC && (int64_t)shm_c <= P->lds_max && P->scatter_cstride) {
+            shm = shm_c;
+            if (dense)
+                return unit ? go(k_scatter_it<true, NM, true, R, false, true, TSC>)
+                            : go(k_scatter_it<true, NM, false, R, false, true, TSC>);
+            return unit ? go(k_scatter_it<false, NM, true, R, false, true, TSC>)
+                        : go(k_scatter_it<false, NM, false, R, false, true, TSC>);
+        }
+    }
     if (F && P->seg_uniform) {
         if (dense)
             return unit ? go(k_scatter_it<true, NM, true, R, SPLIT, F>) : go(k_scatter_it<true, NM, false, R, SPLIT, F>);

@@ -169,7 +169,9 @@ def test_mixed_delta_maps_uniform_knots_against_oracle(monkeypatch):
     and against the same path with the per-group segment search (SFB_SEG_SEARCH=1), with
     radii on every knot of two maps, exactly at rmin and rmax, outside the knots on both
     sides and at +-inf (the boundary values), plus given weights: per map,
-    max |delta - ref| <= 2e-6 max |ref|.  A NaN radius still gives the non-finite error."""
+    max |delta - ref| <= 2e-6 max |ref|.  The kernel with its record-table stride as a
+    constant equals the runtime-stride one (SFB_SCATTER_STRIDE=0) bit for bit, maps and
+    a_nlm.  A NaN radius still gives the non-finite error."""
     am = S.AnlmModes(0.05, 500.0, 1000.0, nside=32)
     win, _ = separable_unit_window(S, am)
     rng = np.random.default_rng(77)
@@ -187,6 +189,9 @@ def test_mixed_delta_maps_uniform_knots_against_oracle(monkeypatch):
     monkeypatch.setenv("SFB_SEG_SEARCH", "1")
     search = S.Plan(am, precision="mixed")
     monkeypatch.delenv("SFB_SEG_SEARCH")
+    monkeypatch.setenv("SFB_SCATTER_STRIDE", "0")
+    rstride = S.Plan(am, precision="mixed")
+    monkeypatch.delenv("SFB_SCATTER_STRIDE")
     dense = win.dense()
     for w in (None, rng.uniform(0.5, 1.5, cat.shape[1])):
         for q in (0, qa, qb):
@@ -197,11 +202,13 @@ def test_mixed_delta_maps_uniform_knots_against_oracle(monkeypatch):
             e_ref, e_s = np.max(np.abs(g - ref)) / scale, np.max(np.abs(g - s)) / scale
             say(f"map {q} {'weights' if w is not None else 'unit'}: vs oracle {e_ref:.2e}, vs search {e_s:.2e}")
             assert e_ref <= 2e-6 and e_s <= 2e-6
+            assert np.array_equal(g, rstride.delta_map(cat, nbar, win, w, q))
             # a dense window: the split-record kernel, same segments and t
             ref_d = O.delta_map(om, cat, nbar, 0, dense.reshape(-1, order="F"), None, w, q)
             e_d = np.max(np.abs(onfly.delta_map(cat, nbar, dense, w, q) - ref_d)) / np.max(np.abs(ref_d))
             say(f"map {q} dense window: vs oracle {e_d:.2e}")
             assert e_d <= 2e-6
+        assert np.array_equal(onfly.cat2amln(cat, nbar, win, w), rstride.cat2amln(cat, nbar, win, w))
     bad = cat.copy()
     bad[0, 11] = np.nan
     with pytest.raises(AssertionError):
