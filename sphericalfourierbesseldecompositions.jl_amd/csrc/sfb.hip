@@ -1726,12 +1726,14 @@ __device__ __forceinline__ float4_t mfma_bf16(short4_t a, short4_t b, float4_t c
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
+template <bool F32, int NT, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0, int BK = GT_BK>
 __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                                int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                                typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     static_assert(SB == 0 || F32, "split-BF16 operands replace FP32 ones");
+    static_assert(SB == 0 || BK == 16, "a split-BF16 MFMA takes the whole 16-deep tile");
+    constexpr int ALD = GT_BM * BK / 256;  // A elements per thread per k-step
     typedef MfmaT<F32> M;
     typedef typename M::T T;
     constexpr int BNW = 16 * NT;  // active columns of this block
@@ -1740,16 +1742,17 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
     typename M::V acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = M::zero();
-    constexpr int BLD = BNW * GT_BK / 256;  // B elements per thread per k-step
+    constexpr int BTOT = BNW * BK;                // B elements per k-step
+    constexpr int BLD = (BTOT + 255) / 256;       // per thread (the last round partial)
     // A stored as FP32 under FP64 MFMA: the raw FP32 values stay in rf until the LDS store,
     // so no conversion waits on the prefetch loads while the MFMAs run
     constexpr bool AWIDEN = AF && !F32;
     static_assert(!(AWIDEN && SPL), "FP32 A under FP64 MFMA has no split-K form");
-    T ra[GT_ALD], rb[BLD];
-    float rf[GT_ALD];  // SPL 2 (second A) or AWIDEN: FP32 values, converted when stored to LDS
+    T ra[ALD], rb[BLD];
+    float rf[ALD];  // SPL 2 (second A) or AWIDEN: FP32 values, converted when stored to LDS
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < GT_ALD; ++u) {
+        for (int u = 0; u < ALD; ++u) {
             const int e = tid + 256 * u;
             const int kk = e >> 6, ii = e & 63;
             const int gk = k0 + kk, gi = i0 + ii;
@@ -1771,6 +1774,7 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
             const int e = tid + 256 * u;
+            if (BTOT % 256 && e >= BTOT) break;
             const int kk = e / BNW, nn = e - kk * BNW;
             const int gk = k0 + kk, gn = n0 + nn;
             if (BF && SPL && gk >= d.K1)  // BF: B and B2 stored as float (F32 operands)
@@ -1784,22 +1788,23 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
     };
     fetch(0);
-    for (int k0 = 0; k0 < d.K; k0 += GT_BK) {
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
 #pragma unroll
-        for (int u = 0; u < GT_ALD; ++u) {
+        for (int u = 0; u < ALD; ++u) {
             const int e = tid + 256 * u;
             As[e >> 6][e & 63] = (AWIDEN || (SPL == 2 && !F32 && k0 + (e >> 6) >= d.K1)) ? (T)rf[u] : ra[u];
         }
 #pragma unroll
         for (int u = 0; u < BLD; ++u) {
             const int e = tid + 256 * u;
+            if (BTOT % 256 && e >= BTOT) break;
             Bs[e / BNW][e % BNW] = rb[u];
         }
         __syncthreads();
-        if (k0 + GT_BK < d.K) fetch(k0 + GT_BK);
+        if (k0 + BK < d.K) fetch(k0 + BK);
         if constexpr (SB == 0) {
 #pragma unroll
-            for (int ks = 0; ks < GT_BK; ks += 4) {
+            for (int ks = 0; ks < BK; ks += 4) {
                 const int kk = ks + (lane >> 4);
                 const T fb = As[kk][wi + (lane & 15)];
 #pragma unroll
@@ -1852,41 +1857,44 @@ __device__ __forceinline__ void gemm_tall_body(const GemmDesc& d, int32_t rows, 
         }
 }
 
-template <bool F32, bool AF, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
+template <bool F32, bool AF, bool CF = false, int SPL = 0, bool BF = false, int SB = 0, int BK = GT_BK>
 __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, double alpha, double beta, int i0,
                                               int n0, typename MfmaT<F32>::T (*As)[GT_BM],
                                               typename MfmaT<F32>::T (*Bs)[GT_BNW])
 {
     const int ntl = min(GT_NTM, (d.N - n0 + 15) / 16);
     switch (ntl) {
-        case 1: gemm_tall_body<F32, 1, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 2: gemm_tall_body<F32, 2, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 3: gemm_tall_body<F32, 3, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        case 4: gemm_tall_body<F32, 4, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
-        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 1: gemm_tall_body<F32, 1, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 2: gemm_tall_body<F32, 2, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 3: gemm_tall_body<F32, 3, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        case 4: gemm_tall_body<F32, 4, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs); break;
+        default: gemm_tall_body<F32, GT_NTM, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs); break;
     }
 }
 
 // CF: C stored as float (the mixed path's aliasing-ring synthesis); SPL: split K,
 // the second A FP64 (1) or FP32 (2).  The GEMMs are short-K and bound by the loads in
-// flight per CU: FP64 at least 3 waves per SIMD (<= 168 VGPRs; more spills), the
-// all-FP32 mixed-path ones 6 (<= 80 VGPRs: 0.32 -> 0.29 ms for the merged step), the
-// split-BF16 ones 4 (their operand splits spilled 56-88 bytes per lane under 80 VGPRs:
-// C2 bf16x3 6.70 -> 6.42 ms per call).  A second register set of prefetched k-tiles
-// (two tiles in flight) cost a wave per SIMD and ran slower (the four analysis GEMMs of
-// a C2 call: FP64 2.0 -> 2.5 ms, mixed 1.4 -> 1.9 ms).
-template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0>
-__global__ __launch_bounds__(256, SB ? 4 : ((F32 && AF) ? 6 : 3)) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+// flight per CU, which the waves per SIMD set more than the depth of each wave's
+// prefetch: FP64 at least 3 waves per SIMD (<= 168 VGPRs; more spills), the split-BF16
+// ones 4 (their operand splits spilled 56-88 bytes per lane under 80 VGPRs: C2 bf16x3
+// 6.70 -> 6.42 ms per call), the all-FP32 mixed-path ones 8 with 8-deep k-tiles (BK 8,
+// <= 64 VGPRs; at BK 16 they ran 6 waves per SIMD, the mixed Jacobi GEMMs of a C2 call
+// 0.07 ms slower).  Deeper prefetch lost: a second register set of k-tiles (two tiles
+// in flight) or 32-deep tiles cost waves per SIMD and ran slower (the four analysis
+// GEMMs of a C2 call: FP64 2.0 -> 2.4-2.5 ms, mixed 1.4 -> 1.8-1.9 ms), as did 8-deep
+// tiles at 4-5 waves per SIMD on the FP64 GEMMs (2.01 -> 2.12 ms).
+template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0, int BK = GT_BK>
+__global__ __launch_bounds__(256, SB ? 4 : ((F32 && AF) ? (BK == 8 ? 8 : 6) : 3)) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
     typedef typename MfmaT<F32>::T T;
-    __shared__ T As[GT_BK][GT_BM];
-    __shared__ T Bs[GT_BK][GT_BNW];
+    __shared__ T As[BK][GT_BM];
+    __shared__ T Bs[BK][GT_BNW];
     const GemmDesc d = descs[blockIdx.z];
     const int i0 = blockIdx.x * GT_BM;
     const int n0 = blockIdx.y * GT_BNW;
     if (n0 >= d.N) return;
-    gemm_dispatch<F32, AF, CF, SPL, BF, SB>(d, rows, alpha, beta, i0, n0, As, Bs);
+    gemm_dispatch<F32, AF, CF, SPL, BF, SB, BK>(d, rows, alpha, beta, i0, n0, As, Bs);
 }
 
 // Same GEMM on a 1-D grid for a large A shared by every batch: workgroup b runs on
@@ -1942,9 +1950,9 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
     } else if (x32)
         hipLaunchKernelGGL((k_gemm_tall<false, true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32 && spl == 2)  // all-FP32 operands and C, split K
-        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true, 0, 8>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32)
-        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true, 0, 8>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 1)
         hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 2)
@@ -1954,9 +1962,9 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
     else if (af32)
         hipLaunchKernelGGL((k_gemm_tall<true, true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32)
-        hipLaunchKernelGGL(k_gemm_tall<true>, g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<true>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else
-        hipLaunchKernelGGL(k_gemm_tall<false>, g, dim3(256), 0, st, descs, rows, alpha, beta);
+        hipLaunchKernelGGL((k_gemm_tall<false>), g, dim3(256), 0, st, descs, rows, alpha, beta);
 }
 
 // ---------------------------------------------------------------------------
