@@ -314,9 +314,17 @@ int sfb_map2alm(sfb_plan* p, int64_t nmap, const double* maps, double* alm_out);
  * counterpart: it checks the MFMA kernel that carries the map2alm Legendre sums. */
 int sfb_gemm_f64(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
 
-/* The same GEMM through the FP32-operand MFMA kernel used for the Jacobi corrections
- * of SFB_PREC_MIXED (operands rounded to FP32, FP64 accumulation across k-tiles). */
+/* The same GEMM through the FP32-operand MFMA kernel (operands rounded to FP32, FP32
+ * accumulation on the MFMA, FP64 epilogue), operands stored as FP64. */
 int sfb_gemm_f32(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B, double* C);
+
+/* The kernels of the SFB_PREC_MIXED Jacobi steps and syntheses: A, B (and Cin) rounded to
+ * FP32 and stored as FP32, 8-deep k-tiles, C stored as FP32 and returned widened.  With
+ * 0 < K1 < K, rows k >= K1 of A and B go through the split-K operands (the merged
+ * (I - C0) + residual step); Cin (N x rows, or NULL) is added: C = A^T B + Cin.  No
+ * reference counterpart: a kernel check. */
+int sfb_gemm_f32_stored(int32_t device, int64_t rows, int64_t K, int64_t K1, int64_t N, const double* A,
+                        const double* B, const double* Cin, double* C);
 
 /* The same GEMM with split-BF16 operands in `terms` bf16 terms (2: hi + lo, three products;
  * 3: hi + mid + lo, six products), the kernel of the SFB_PREC_BF16X2 / X3 Jacobi steps. */

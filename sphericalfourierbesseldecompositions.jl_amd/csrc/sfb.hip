@@ -5778,6 +5778,64 @@ extern "C" int sfb_gemm_bf16(int32_t device, int64_t rows, int64_t K, int64_t N,
     return gemm_check(device, rows, K, N, A, B, C, true, terms);
 }
 
+extern "C" int sfb_gemm_f32_stored(int32_t device, int64_t rows, int64_t K, int64_t K1, int64_t N, const double* A,
+                                   const double* B, const double* Cin, double* C)
+{
+    sfb::clear_error();
+    if (rows < 1 || K < 1 || N < 1 || K1 < 0 || rows > (1 << 24) || N > (1 << 24) || K > (1 << 24) || !A || !B || !C)
+        return fail(SFB_ERR_ARG, "bad gemm arguments");
+    HIP_TRY(hipSetDevice(device));
+    const bool split = K1 > 0 && K1 < K;
+    std::vector<float> a((size_t)(K * rows)), b((size_t)(K * N)), c((size_t)(N * rows), 0.0f);
+    for (size_t i = 0; i < a.size(); ++i) a[i] = (float)A[i];
+    for (size_t i = 0; i < b.size(); ++i) b[i] = (float)B[i];
+    if (Cin)
+        for (size_t i = 0; i < c.size(); ++i) c[i] = (float)Cin[i];
+    DevBuf dA, dB, dC, dCin, dD;
+    struct Release {  // every return frees the buffers
+        DevBuf* b[5];
+        ~Release()
+        {
+            for (DevBuf* x : b) x->release();
+        }
+    } release{{&dA, &dB, &dC, &dCin, &dD}};
+    int rc;
+    if ((rc = dA.ensure(sizeof(float) * a.size())) || (rc = dB.ensure(sizeof(float) * b.size())) ||
+        (rc = dC.ensure(sizeof(float) * c.size())) || (rc = dCin.ensure(sizeof(float) * c.size())) ||
+        (rc = dD.ensure(sizeof(GemmDesc))))
+        return rc;
+    hipError_t e = hipMemcpy(dA.p, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB.p, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dCin.p, c.data(), sizeof(float) * c.size(), hipMemcpyHostToDevice);
+    GemmDesc d{};
+    d.A = (const double*)dA.as<float>();  // float storage behind the descriptor's pointer type
+    d.B = (const double*)dB.as<float>();
+    d.C = (double*)dC.as<float>();
+    d.Cin = Cin ? (const double*)dCin.as<float>() : nullptr;
+    d.K = (int32_t)K;
+    d.N = (int32_t)N;
+    d.lda = rows;
+    d.ldb = N;
+    d.ldc = rows;
+    d.K1 = split ? (int32_t)K1 : (int32_t)K;
+    if (split) {
+        d.A2 = (const double*)(dA.as<float>() + K1 * rows);
+        d.B2 = (const double*)(dB.as<float>() + K1 * N);
+        d.lda2 = rows;
+        d.ldb2 = N;
+    }
+    if (e == hipSuccess) e = hipMemcpy(dD.p, &d, sizeof d, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_gemm(dD.as<GemmDesc>(), 1, (int32_t)rows, (int32_t)N, 1.0, Cin ? 1.0 : 0.0, 0, true, true, true,
+                    split ? 2 : 0, false, 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(c.data(), dC.p, sizeof(float) * c.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(SFB_ERR_DEVICE, "gemm: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < c.size(); ++i) C[i] = (double)c[i];
+    return SFB_OK;
+}
+
 static int gemm_check(int32_t device, int64_t rows, int64_t K, int64_t N, const double* A, const double* B,
                       double* C, bool f32, int sb)
 {

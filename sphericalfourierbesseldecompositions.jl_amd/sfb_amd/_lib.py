@@ -70,6 +70,7 @@ SIGNATURES = {
     "sfb_gemm_f64": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
     "sfb_gemm_f32": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P]),
     "sfb_gemm_bf16": (ctypes.c_int, [_i32, _i64, _i64, _i64, _P, _P, _P, _i32]),
+    "sfb_gemm_f32_stored": (ctypes.c_int, [_i32, _i64, _i64, _i64, _i64, _P, _P, _P, _P]),
     "sfb_phi_transform": (ctypes.c_int, [_P, _i64, _P, _P]),
     "sfb_ring_kernel_shapes": (ctypes.c_int, [_i64, _i64, _i64, _P]),
     "sfb_plan_profile": (ctypes.c_int, [_P, _i32]),

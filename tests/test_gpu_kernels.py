@@ -35,6 +35,28 @@ def test_gemm_f32_matches_numpy(rows, K, N):
     assert np.max(np.abs(C - ref)) <= 1e-6 * np.max(np.abs(ref))
 
 
+# the mixed path's kernels: FP32 storage, 8-deep k-tiles (K and K1 off multiples of 8, column
+# counts whose B tile leaves the last load round partial), the split-K merged form and Cin
+@pytest.mark.parametrize("rows,K,K1,N,cin", [(16, 4, 0, 16, False), (70, 37, 0, 23, False), (130, 256, 0, 71, False),
+                                             (65, 3, 0, 17, False), (64, 1, 0, 1, False), (33, 9, 0, 80, True),
+                                             (300, 80, 0, 200, False), (70, 37, 20, 23, True),
+                                             (130, 141, 71, 71, True), (64, 12, 5, 48, False)])
+def test_gemm_f32_stored_matches_numpy(rows, K, K1, N, cin):
+    rng = np.random.default_rng(rows * 7 + K + 3 * K1 + N)
+    A = rng.normal(size=(K, rows))
+    B = rng.normal(size=(K, N))
+    Cin = rng.normal(size=(N, rows)) if cin else None
+    C = np.empty((N, rows))
+    L.check(L.lib().sfb_gemm_f32_stored(0, rows, K, K1, N, L.ptr(A), L.ptr(B), L.ptr(Cin) if cin else None,
+                                        L.ptr(C)))
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    ref = f32(B).T @ f32(A) + (f32(Cin) if cin else 0.0)
+    scale = np.abs(f32(B)).T @ np.abs(f32(A)) + (np.abs(f32(Cin)) if cin else 0.0)
+    err = np.max(np.abs(C - ref) / scale)
+    print(f"rows {rows} K {K} K1 {K1} N {N}: max |C - ref| / (|B|^T |A| + |Cin|) = {err:.2e}", flush=True)
+    assert err <= 1e-6
+
+
 @pytest.mark.parametrize("rows,K,N", [(16, 4, 16), (70, 37, 23), (130, 256, 71), (300, 80, 200), (64, 1, 1)])
 @pytest.mark.parametrize("terms", [2, 3])
 def test_gemm_bf16_split_matches_numpy(rows, K, N, terms):
