@@ -909,6 +909,15 @@ template <> struct Vec2T<float> { typedef float2 T; };
 __host__ __device__ inline int twpad(int kk) { return kk + (kk >> 4); }
 __host__ __device__ inline int64_t tw_len(int64_t N) { return N + (N >> 4) + 1; }
 
+// x conj(w) with two FMAs (the transforms' products are not held to the reference's
+// roundings, unlike the spline arithmetic)
+template <typename R>
+__device__ __forceinline__ void cmul_conj(R xr, R xi, R wr, R wi, R& cr, R& ci)
+{
+    cr = fma(xr, wr, xi * wi);
+    ci = fma(xi, wr, -(xr * wi));
+}
+
 // Writes the ring-Fourier coefficient X_kf (north and south ring) of one map into
 // every m <= L with m = kf (mod npr) as X_kf and m = -kf (mod npr) as conj(X_kf),
 // each times the ring phase e^{-i m phi_0}, phi_0 = shift*pi/npr, in E/O form.
@@ -931,8 +940,9 @@ __device__ __forceinline__ void store_folded(R* __restrict__ X, int64_t M2, int3
             const TW ph64 = twr[TWPAD ? twpad((int)ph) : (int)ph];  // e^{-i m phi0} = (c, -s)
             const R phx = (R)ph64.x, phy = (R)ph64.y;
             const R anr = xnr, ani = (R)cj * xni, asr = xsr, asi = (R)cj * xsi;
-            const R fnr = anr * phx + ani * phy, fni = ani * phx - anr * phy;
-            const R fsr = asr * phx + asi * phy, fsi = asi * phx - asr * phy;
+            R fnr, fni, fsr, fsi;
+            cmul_conj(anr, ani, phx, phy, fnr, fni);  // times conj(c, s) = e^{-i m phi0}
+            cmul_conj(asr, asi, phx, phy, fsr, fsi);
             const int64_t be = ((int64_t)(m * 2 + 0) * K + k) * M2 + 2 * q;
             const int64_t bo = ((int64_t)(m * 2 + 1) * K + k) * M2 + 2 * q;
             typedef typename Vec2T<R>::T R2;  // (re, im) pairs start at even offsets
@@ -1000,15 +1010,16 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
             const R2 w3 = twr[twpad((pos + half) * (tstep >> 1))];
             R2 x0 = zs[a0], x1 = zs[a1], x2 = zs[a2], x3 = zs[a3];
             // stage st: (x0, x1) and (x2, x3) with w1 (e^{-i theta} = (c, -s))
-            R cr = x1.x * w1.x + x1.y * w1.y, ci = x1.y * w1.x - x1.x * w1.y;
+            R cr, ci;
+            cmul_conj(x1.x, x1.y, w1.x, w1.y, cr, ci);
             R2 y0, y1, y2, y3;
             y0.x = x0.x + cr; y0.y = x0.y + ci; y1.x = x0.x - cr; y1.y = x0.y - ci;
-            cr = x3.x * w1.x + x3.y * w1.y; ci = x3.y * w1.x - x3.x * w1.y;
+            cmul_conj(x3.x, x3.y, w1.x, w1.y, cr, ci);
             y2.x = x2.x + cr; y2.y = x2.y + ci; y3.x = x2.x - cr; y3.y = x2.y - ci;
             // stage st+1: (y0, y2) with w2, (y1, y3) with w3
-            cr = y2.x * w2.x + y2.y * w2.y; ci = y2.y * w2.x - y2.x * w2.y;
+            cmul_conj(y2.x, y2.y, w2.x, w2.y, cr, ci);
             x0.x = y0.x + cr; x0.y = y0.y + ci; x2.x = y0.x - cr; x2.y = y0.y - ci;
-            cr = y3.x * w3.x + y3.y * w3.y; ci = y3.y * w3.x - y3.x * w3.y;
+            cmul_conj(y3.x, y3.y, w3.x, w3.y, cr, ci);
             x1.x = y1.x + cr; x1.y = y1.y + ci; x3.x = y1.x - cr; x3.y = y1.y - ci;
             zs[a0] = x0; zs[a1] = x1; zs[a2] = x2; zs[a3] = x3;
         }
@@ -1024,8 +1035,8 @@ __device__ __forceinline__ void fft_passes(typename Vec2T<R>::T* zs, int nf, int
             const int i0 = f * Mp + fpad<R>(l0), i1 = f * Mp + fpad<R>(l0 + half);
             const R2 w = twr[twpad(pos * tstep)];
             const R2 a = zs[i0], c = zs[i1];
-            const R cr = c.x * w.x + c.y * w.y;
-            const R ci = c.y * w.x - c.x * w.y;
+            R cr, ci;
+            cmul_conj(c.x, c.y, w.x, w.y, cr, ci);
             zs[i0].x = a.x + cr; zs[i0].y = a.y + ci;
             zs[i1].x = a.x - cr; zs[i1].y = a.y - ci;
         }
@@ -1147,8 +1158,8 @@ __global__ __launch_bounds__(512) void k_phi_fft(const TS* __restrict__ delta, i
             const R er = (R)0.5 * (zm.x + zc.x), ei = (R)0.5 * (zm.y - zc.y);
             const R orr = (R)0.5 * (zm.y + zc.y), oi = (R)-0.5 * (zm.x - zc.x);
             const R2 w = twr[twpad(2 * m)];
-            xr[side] = er + (w.x * orr + w.y * oi);
-            xi[side] = ei + (w.x * oi - w.y * orr);
+            xr[side] = fma(w.x, orr, fma(w.y, oi, er));
+            xi[side] = fma(w.x, oi, fma(-w.y, orr, ei));
         }
         // ring phases from the LDS table: (m' shift) mod 2N <= L <= N for every m' <= L
         store_folded<TS, true>(X, M2, K, k, eq, N, Rg.shift, twr, Lk, m, q, (TS)xr[0], (TS)xi[0], (TS)xr[1],
