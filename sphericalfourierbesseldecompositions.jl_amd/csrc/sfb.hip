@@ -1689,8 +1689,8 @@ __global__ __launch_bounds__(256) void k_phi_polar4(const R* __restrict__ delta,
             for (int ss = 0; ss < 4; ++ss) {
                 R2 y = Y[ss * nu + u];
                 if (cj) y.y = -y.y;
-                sr += y.x * wx[ss] + y.y * wy[ss];
-                si += y.y * wx[ss] - y.x * wy[ss];
+                sr = fma(y.x, wx[ss], fma(y.y, wy[ss], sr));
+                si = fma(y.y, wx[ss], fma(-y.x, wy[ss], si));
             }
             xr[side] = sr;
             xi[side] = si;
