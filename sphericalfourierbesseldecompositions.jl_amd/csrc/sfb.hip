@@ -1929,12 +1929,13 @@ __device__ __forceinline__ void gemm_dispatch(const GemmDesc& d, int32_t rows, d
 // ones 4 (their operand splits spilled 56-88 bytes per lane under 80 VGPRs: C2 bf16x3
 // 6.70 -> 6.42 ms per call), the all-FP32 mixed-path ones 8 with 8-deep k-tiles (BK 8,
 // <= 64 VGPRs; at BK 16 they ran 6 waves per SIMD, the mixed Jacobi GEMMs of a C2 call
-// 0.07 ms slower).  Deeper prefetch lost: a second register set of k-tiles (two tiles
+// 0.07 ms slower), and the FP64 merged step 5 with BK 8 (the FP64 analysis GEMMs of a C2
+// call 2.01 -> 1.94 ms; BK 8 on the FP64 first analysis and syntheses too ran 2.12 ms).
+// Deeper prefetch lost: a second register set of k-tiles (two tiles
 // in flight) or 32-deep tiles cost waves per SIMD and ran slower (the four analysis
-// GEMMs of a C2 call: FP64 2.0 -> 2.4-2.5 ms, mixed 1.4 -> 1.8-1.9 ms), as did 8-deep
-// tiles at 4-5 waves per SIMD on the FP64 GEMMs (2.01 -> 2.12 ms).
+// GEMMs of a C2 call: FP64 2.0 -> 2.4-2.5 ms, mixed 1.4 -> 1.8-1.9 ms).
 template <bool F32, bool AF = false, bool CF = false, int SPL = 0, bool BF = false, int SB = 0, int BK = GT_BK>
-__global__ __launch_bounds__(256, SB ? 4 : ((F32 && AF) ? (BK == 8 ? 8 : 6) : 3)) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
+__global__ __launch_bounds__(256, SB ? 4 : ((F32 && AF) ? (BK == 8 ? 8 : 6) : (BK == 8 ? 5 : 3))) void k_gemm_tall(const GemmDesc* __restrict__ descs, int32_t rows, double alpha,
                                                    double beta)
 {
     typedef typename MfmaT<F32>::T T;
@@ -2003,8 +2004,8 @@ static inline void launch_gemm(const GemmDesc* descs, unsigned nbat, int32_t row
         hipLaunchKernelGGL((k_gemm_tall<true, true, true, 2, true, 0, 8>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (f32 && af32 && cf32)
         hipLaunchKernelGGL((k_gemm_tall<true, true, true, 0, true, 0, 8>), g, dim3(256), 0, st, descs, rows, alpha, beta);
-    else if (spl == 1)
-        hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1>), g, dim3(256), 0, st, descs, rows, alpha, beta);
+    else if (spl == 1)  // the FP64 merged step: 8-deep k-tiles at 5 waves per SIMD
+        hipLaunchKernelGGL((k_gemm_tall<false, false, false, 1, false, 0, 8>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (spl == 2)
         hipLaunchKernelGGL((k_gemm_tall<false, false, false, 2>), g, dim3(256), 0, st, descs, rows, alpha, beta);
     else if (cf32)
