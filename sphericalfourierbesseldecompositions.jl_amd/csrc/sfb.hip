@@ -3903,12 +3903,14 @@ static int launch_scatter_split(sfb_plan* P, bool dense, hipStream_t st, int64_t
                                              d_wa, d_wb, msel);
 }
 
-// smallest and largest value of n doubles (one block): the separable mask's range
+// smallest and largest value of n doubles: block b writes out[2 b], out[2 b + 1] for its
+// grid-strided share (a second one-block launch over those pairs gives the whole range;
+// min and max are exact, so the order does not matter): the separable mask's range
 __global__ __launch_bounds__(1024) void k_range(int64_t n, const double* __restrict__ v, double* __restrict__ out)
 {
     __shared__ double mn[1024], mx[1024];
     double a = INFINITY, b = -INFINITY;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         a = fmin(a, v[i]);
         b = fmax(b, v[i]);
     }
@@ -3923,10 +3925,11 @@ __global__ __launch_bounds__(1024) void k_range(int64_t n, const double* __restr
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = mn[0];
-        out[1] = mx[0];
+        out[2 * blockIdx.x] = mn[0];
+        out[2 * blockIdx.x + 1] = mx[0];
     }
 }
+constexpr int RANGE_BLOCKS = 64;
 
 template <typename R>
 static int launch_scatter_it(sfb_plan* P, bool dense, hipStream_t st, int64_t T, const R* sw,
@@ -4021,8 +4024,14 @@ static int scatter_stage(sfb_plan* P, double nbar, int32_t win_kind, const doubl
         if (f32 && dense) {
             rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb, nullptr);
         } else if (f32) {
-            if ((rc = P->d_mrange.ensure(16))) return rc;
-            hipLaunchKernelGGL(k_range, dim3(1), dim3(1024), 0, st, P->npix, d_wa, P->d_mrange.as<double>());
+            // (min, max) at [0, 1], the blocks' pairs behind them (one block scanning the
+            // 196608-pixel mask took 46 us)
+            if ((rc = P->d_mrange.ensure(sizeof(double) * (2 + 2 * RANGE_BLOCKS)))) return rc;
+            hipLaunchKernelGGL(k_range, dim3(RANGE_BLOCKS), dim3(1024), 0, st, P->npix, d_wa,
+                               P->d_mrange.as<double>() + 2);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_range, dim3(1), dim3(1024), 0, st, (int64_t)(2 * RANGE_BLOCKS),
+                               P->d_mrange.as<double>() + 2, P->d_mrange.as<double>());
             LAUNCH_CHECK();
             const double* msel = P->d_mrange.as<double>();
             rc = launch_scatter_split(P, dense, st, T, P->d_swf.as<float>(), inv, d_wa, d_wb, msel);
