@@ -4568,7 +4568,8 @@ extern "C" int sfb_cat2amln_device(sfb_plan* P, int64_t N, const double* d_rtp, 
 // Host arrays to the plan's device buffers, enqueued on its stream.  Through pinned staging
 // when the copy pool runs (and the arrays total at most 512 MiB): the pool fills the pinned
 // buffer in 4 MiB chunks and each chunk's DMA is enqueued as soon as it is complete, so the
-// copies overlap the transfer (24 MB of catalogue: 0.69 ms pageable, ~0.5 ms staged).  The
+// copies overlap the transfer (24 MB of catalogue: 0.69 ms pageable, ~0.5 ms staged; 2 MiB
+// chunks ran 0.04 ms and 512 KiB chunks 0.5 ms slower per call, more transfer calls).  The
 // pinned buffer is reused by the next call: every host-array entry point synchronises its
 // stream before returning.  Otherwise one pageable copy per array.
 static int stage_inputs(sfb_plan* P, int64_t N, const double* rtp, const double* w, int32_t win_kind,
